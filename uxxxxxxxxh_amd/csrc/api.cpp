@@ -1,0 +1,484 @@
+// C ABI of the sweep library (include/uhs.h): argument checks, stencil ->
+// kernel dispatch, configuration choice, the ping-pong time loop, and the
+// device plumbing the tests and the benchmark use.
+#include "uhs.h"
+#include "uhs_internal.h"
+#include "uhs_host.h"
+#include "uhs_util.h"
+
+#include <algorithm>
+#include <climits>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(UHS_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+const char *family_name(int f)
+{
+    static const char *names[] = {"star3d", "box3d", "win2d", "gather"};
+    return f >= 0 && f < 4 ? names[f] : "?";
+}
+
+std::mutex g_tuned_mu;
+std::unordered_map<std::string, uhs_config> g_tuned;
+} // namespace
+
+int uhs_set_error(int code, const char *msg)
+{
+    g_err = msg ? msg : "";
+    return code;
+}
+
+int uhs_match_builtin(int dims, int npoints, const int *off)
+{
+    for (int i = 0; i < (int)(sizeof(kBuiltins) / sizeof(kBuiltins[0])); i++) {
+        const BuiltinStencil &b = kBuiltins[i];
+        if (b.dims != dims || b.npoints != npoints) continue;
+        if (memcmp(b.off, off, sizeof(int) * npoints * dims) == 0) return i;
+    }
+    return -1;
+}
+
+std::string uhs_config_key(int family, int prog, int dtype, int dims, const int64_t *ext)
+{
+    char buf[128];
+    snprintf(buf, sizeof buf, "%d/%d/%d/%lldx%lldx%lld", family, prog, dtype, (long long)ext[0],
+             (long long)ext[1], dims == 3 ? (long long)ext[2] : 1LL);
+    return buf;
+}
+
+void uhs_store_tuned(const std::string &key, const uhs_config &c)
+{
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned[key] = c;
+}
+
+bool uhs_lookup_tuned(const std::string &key, uhs_config *c)
+{
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    auto it = g_tuned.find(key);
+    if (it == g_tuned.end()) return false;
+    *c = it->second;
+    return true;
+}
+
+void uhs_clear_tuned()
+{
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned.clear();
+}
+
+static long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
+
+// Workgroups a configuration launches for the given problem.
+long long uhs_config_blocks(const SweepArgs &a, const uhs_config &c)
+{
+    const long long tiles = ceil_div(a.n2, c.tile_x) * (a.dims == 3 ? ceil_div(a.n1, c.tile_y) : 1);
+    return tiles * ceil_div(a.n0, c.tile_z);
+}
+
+int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whylen)
+{
+    const SweepArgs &a = pl.args;
+    const int R = pl.radius;
+    auto bad = [&](const char *fmt, auto... v) {
+        snprintf(why, whylen, fmt, v...);
+        return UHS_E_CONFIG;
+    };
+    if (pl.family == FAM_GATHER) return UHS_OK;
+    if (c.vec != 1 && c.vec != 2 && c.vec != 4) return bad("vec must be 1, 2 or 4 (got %d)", c.vec);
+    if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
+    if (c.tile_x % c.vec) return bad("tile_x %d not a multiple of vec %d", c.tile_x, c.vec);
+    if (a.dims == 2 && c.tile_y != 1) return bad("2D stencils need tile_y = 1");
+    const long long nt = (long long)(c.tile_x / c.vec) * c.tile_y;
+    if (nt > kMaxThreads) return bad("%lld threads per workgroup exceeds %d", nt, kMaxThreads);
+    if (nt < 2 * R) return bad("%lld threads cannot stage a radius-%d halo", nt, R);
+    const size_t esz = a.dtype == DT_F64 ? 8 : 4;
+    size_t lds;
+    if (pl.family == FAM_WIN2D) {
+        lds = 2 * esz * (c.tile_x + 2 * R);
+    } else {
+        lds = 2 * esz * (size_t)(c.tile_x + 2 * R) * (c.tile_y + 2 * R);
+        if (pl.family == FAM_STAR3D &&
+            ceil_div(2LL * R * (c.tile_x + c.tile_y), nt) > kStarHaloPerThread)
+            return bad("halo ring of %d cells needs more than %d per thread",
+                       2 * R * (c.tile_x + c.tile_y), kStarHaloPerThread);
+        if (pl.family == FAM_BOX3D &&
+            ceil_div((long long)(c.tile_x + 2 * R) * (c.tile_y + 2 * R), nt) > kBoxCellsPerThread)
+            return bad("plane tile of %d cells needs more than %d per thread",
+                       (c.tile_x + 2 * R) * (c.tile_y + 2 * R), kBoxCellsPerThread);
+    }
+    if (lds > kMaxLdsBytes) return bad("%zu bytes of LDS exceeds %zu", lds, kMaxLdsBytes);
+    if (uhs_config_blocks(a, c) > INT_MAX) return bad("too many workgroups");
+    LaunchFn fn = nullptr;
+    if (pl.family == FAM_STAR3D) fn = uhs_find_star3d(pl.prog, a.dtype, c.vec);
+    if (pl.family == FAM_BOX3D) fn = uhs_find_box3d(pl.prog, a.dtype, c.vec);
+    if (pl.family == FAM_WIN2D) fn = uhs_find_win2d(pl.prog, a.dtype, c.vec);
+    if (!fn) return bad("no %s kernel compiled for vec %d", family_name(pl.family), c.vec);
+    return UHS_OK;
+}
+
+uhs_config uhs_heuristic_config(const Plan &pl)
+{
+    uhs_config c{};
+    const SweepArgs &a = pl.args;
+    switch (pl.family) {
+    case FAM_STAR3D: c = {64, 8, 64, 2}; break;
+    case FAM_BOX3D: c = {64, 4, 128, 1}; break;
+    case FAM_WIN2D: c = {256, 1, 64, 1}; break;
+    default: c = {0, 0, 0, 0}; return c;
+    }
+    // Small grids: shorten the streamed slab until the launch has enough
+    // workgroups to cover 256 CUs a few times over.
+    while (c.tile_z > 8 && uhs_config_blocks(a, c) < 2048) c.tile_z /= 2;
+    return c;
+}
+
+int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,
+                  const uhs_config *cfg, Plan *pl)
+{
+    if (!ext || !st) return fail(UHS_E_INVAL, "null extents or stencil");
+    if (dims != 2 && dims != 3) return fail(UHS_E_INVAL, "dims must be 2 or 3 (got %d)", dims);
+    if (st->dims != dims) return fail(UHS_E_INVAL, "stencil has %d dims, grid %d", st->dims, dims);
+    if (st->npoints < 1 || st->npoints > UHS_MAX_POINTS)
+        return fail(UHS_E_INVAL, "npoints %d outside [1, %d]", st->npoints, UHS_MAX_POINTS);
+    if (!st->offsets || !st->coeffs) return fail(UHS_E_INVAL, "null offsets or coefficients");
+    if (dtype < DT_F64 || dtype > DT_F32K) return fail(UHS_E_INVAL, "bad dtype %d", dtype);
+    if (halo < 0) return fail(UHS_E_INVAL, "negative halo");
+    for (int k = 0; k < dims; k++)
+        if (ext[k] < 1 || ext[k] + 2LL * halo > INT_MAX / 2)
+            return fail(UHS_E_INVAL, "extent %lld on axis %d out of range", (long long)ext[k], k);
+    int radius = 0;
+    for (int i = 0; i < st->npoints * dims; i++) radius = std::max(radius, std::abs(st->offsets[i]));
+    if (radius > UHS_MAX_RADIUS) return fail(UHS_E_UNSUPPORTED, "radius %d > %d", radius, UHS_MAX_RADIUS);
+    if (radius > halo) return fail(UHS_E_INVAL, "halo %d smaller than the stencil radius %d", halo, radius);
+
+    *pl = Plan{};
+    SweepArgs &a = pl->args;
+    a.dims = dims;
+    a.halo = halo;
+    if (dims == 3) {
+        a.n0 = (int)ext[0]; a.n1 = (int)ext[1]; a.n2 = (int)ext[2];
+        a.sy = ext[2] + 2LL * halo;
+        a.sz = (ext[1] + 2LL * halo) * a.sy;
+        if (a.sz > INT_MAX) return fail(UHS_E_INVAL, "plane of %lld cells exceeds 2^31", a.sz);
+    } else {
+        a.n0 = (int)ext[0]; a.n1 = 1; a.n2 = (int)ext[1];
+        a.sy = ext[1] + 2LL * halo;
+        a.sz = (ext[0] + 2LL * halo) * a.sy;
+    }
+    a.npoints = st->npoints;
+    a.off = st->offsets;
+    a.coef = st->coeffs;
+    a.dtype = dtype;
+    pl->radius = radius;
+    pl->prog = uhs_match_builtin(dims, st->npoints, st->offsets);
+    if (pl->prog >= 0) {
+        const BuiltinStencil &b = kBuiltins[pl->prog];
+        pl->family = dims == 2 ? FAM_WIN2D : (b.star ? FAM_STAR3D : FAM_BOX3D);
+        pl->radius = b.radius;
+    } else {
+        pl->family = FAM_GATHER;
+        if (st->npoints > kGatherMaxPoints)
+            return fail(UHS_E_UNSUPPORTED,
+                        "stencil of %d terms has no specialised kernel and exceeds the "
+                        "%d-term general kernel",
+                        st->npoints, kGatherMaxPoints);
+        if ((long long)radius * (dims == 3 ? a.sz : a.sy) > INT_MAX)
+            return fail(UHS_E_INVAL, "term offsets exceed 2^31 cells");
+    }
+
+    uhs_config c;
+    if (cfg) {
+        c = *cfg;
+    } else if (!uhs_lookup_tuned(uhs_config_key(pl->family, pl->prog, dtype, dims, ext), &c)) {
+        c = uhs_heuristic_config(*pl);
+    }
+    char why[256] = "";
+    if (uhs_check_config(*pl, c, why, sizeof why) != UHS_OK)
+        return fail(UHS_E_CONFIG, "config {%d,%d,%d,%d} rejected: %s", c.tile_x, c.tile_y,
+                    c.tile_z, c.vec, why);
+    a.tx = c.tile_x; a.ty = c.tile_y; a.tz = c.tile_z; a.vec = c.vec;
+    pl->cfg = c;
+    switch (pl->family) {
+    case FAM_STAR3D: pl->fn = uhs_find_star3d(pl->prog, dtype, c.vec); break;
+    case FAM_BOX3D: pl->fn = uhs_find_box3d(pl->prog, dtype, c.vec); break;
+    case FAM_WIN2D: pl->fn = uhs_find_win2d(pl->prog, dtype, c.vec); break;
+    default: pl->fn = uhs_find_gather(dtype); break;
+    }
+    if (!pl->fn) return fail(UHS_E_UNSUPPORTED, "no kernel for this stencil and precision");
+    return UHS_OK;
+}
+
+int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream)
+{
+    if (!A || !B) return fail(UHS_E_INVAL, "null grid pointer");
+    if (A == B) return fail(UHS_E_INVAL, "A and B must be distinct grids");
+    if (timesteps < 0) return fail(UHS_E_INVAL, "negative timestep count");
+    for (int k = 1; k <= timesteps; k++) {
+        pl.args.in = (k & 1) ? A : B;
+        pl.args.out = (k & 1) ? B : A;
+        hipError_t e = pl.fn(pl.args, stream);
+        if (e != hipSuccess) return hip_fail(e, "sweep launch");
+    }
+    return UHS_OK;
+}
+
+extern "C" {
+
+int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
+                  const uhs_stencil *st, int timesteps, const uhs_config *cfg, uhs_stream_t stream)
+{
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, DT_F64, cfg, &pl);
+    if (rc) return rc;
+    return uhs_run_plan(pl, A, B, timesteps, (hipStream_t)stream);
+}
+
+int uhs_sweep_f32(float *A, float *B, int dims, const int64_t *ext, int halo,
+                  const uhs_stencil *st, int timesteps, int compensated, const uhs_config *cfg,
+                  uhs_stream_t stream)
+{
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, compensated ? DT_F32K : DT_F32, cfg, &pl);
+    if (rc) return rc;
+    return uhs_run_plan(pl, A, B, timesteps, (hipStream_t)stream);
+}
+
+int uhs_plan_kind(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,
+                  const uhs_config *cfg, char *buf, size_t buflen)
+{
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, dtype, cfg, &pl);
+    if (rc) return rc;
+    if (buf && buflen) snprintf(buf, buflen, "%s", family_name(pl.family));
+    return UHS_OK;
+}
+
+int uhs_default_config(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,
+                       uhs_config *out)
+{
+    if (!out) return fail(UHS_E_INVAL, "null output");
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, dtype, nullptr, &pl);
+    if (rc) return rc;
+    *out = pl.cfg;
+    return UHS_OK;
+}
+
+int uhs_builtin_count(void) { return (int)(sizeof(kBuiltins) / sizeof(kBuiltins[0])); }
+
+int uhs_builtin_info(int i, const char **case_name, const char **symbol, int *dims, int *radius,
+                     int *npoints, const int **offsets, const double **coeffs)
+{
+    if (i < 0 || i >= uhs_builtin_count()) return fail(UHS_E_INVAL, "no built-in stencil %d", i);
+    const BuiltinStencil &b = kBuiltins[i];
+    if (case_name) *case_name = b.case_name;
+    if (symbol) *symbol = b.symbol;
+    if (dims) *dims = b.dims;
+    if (radius) *radius = b.radius;
+    if (npoints) *npoints = b.npoints;
+    if (offsets) *offsets = b.off;
+    if (coeffs) *coeffs = b.coef;
+    return UHS_OK;
+}
+
+int uhs_builtin_find(const char *case_name)
+{
+    if (!case_name) return fail(UHS_E_INVAL, "null name");
+    for (int i = 0; i < uhs_builtin_count(); i++)
+        if (strcmp(kBuiltins[i].case_name, case_name) == 0) return i;
+    return fail(UHS_E_INVAL, "no built-in stencil named '%s'", case_name);
+}
+
+static int grid_shape(int dims, const int64_t *ext, int halo, GridShape *s)
+{
+    if (!ext || (dims != 2 && dims != 3) || halo < 0) return fail(UHS_E_INVAL, "bad grid shape");
+    for (int k = 0; k < dims; k++)
+        if (ext[k] < 1 || ext[k] + 2LL * halo > INT_MAX / 2) return fail(UHS_E_INVAL, "bad extent");
+    if (dims == 3) *s = {(int)ext[0] + 2 * halo, (int)ext[1] + 2 * halo, (int)ext[2] + 2 * halo, halo, halo};
+    else *s = {1, (int)ext[0] + 2 * halo, (int)ext[1] + 2 * halo, 0, halo};
+    return UHS_OK;
+}
+
+int uhs_fill_uniform_f64(double *grid, int dims, const int64_t *ext, int halo, uint64_t seed,
+                         uhs_stream_t stream)
+{
+    GridShape s;
+    if (int rc = grid_shape(dims, ext, halo, &s)) return rc;
+    if (!grid) return fail(UHS_E_INVAL, "null grid");
+    hipError_t e = uhs_launch_fill(grid, 0, s, seed, (hipStream_t)stream);
+    return e == hipSuccess ? UHS_OK : hip_fail(e, "fill");
+}
+
+int uhs_fill_uniform_f32(float *grid, int dims, const int64_t *ext, int halo, uint64_t seed,
+                         uhs_stream_t stream)
+{
+    GridShape s;
+    if (int rc = grid_shape(dims, ext, halo, &s)) return rc;
+    if (!grid) return fail(UHS_E_INVAL, "null grid");
+    hipError_t e = uhs_launch_fill(grid, 1, s, seed, (hipStream_t)stream);
+    return e == hipSuccess ? UHS_OK : hip_fail(e, "fill");
+}
+
+static uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+int uhs_make_coeffs(int dims, int npoints, const int *offsets, uint64_t seed, double *coeffs)
+{
+    if ((dims != 2 && dims != 3) || npoints < 1 || !offsets || !coeffs)
+        return fail(UHS_E_INVAL, "bad coefficient request");
+    uint64_t draws = 0;
+    for (int t = 0; t < npoints; t++) {
+        int partner = -1;
+        for (int s = 0; s < t && partner < 0; s++) {
+            bool mirror = true;
+            for (int k = 0; k < dims; k++)
+                if (offsets[s * dims + k] != -offsets[t * dims + k]) mirror = false;
+            if (mirror) partner = s;
+        }
+        if (partner >= 0) {
+            coeffs[t] = coeffs[partner];
+        } else {
+            const uint64_t m = mix64(seed + (draws + 1) * 0x9E3779B97F4A7C15ULL);
+            coeffs[t] = (double)(m >> 11) * 0x1.0p-53;
+            draws++;
+        }
+    }
+    double sum = 0.0;
+    for (int t = 0; t < npoints; t++) sum += coeffs[t];
+    for (int t = 0; t < npoints; t++) coeffs[t] /= sum;
+    return UHS_OK;
+}
+
+static int norms(const void *a, int a_f32, const double *b, int dims, const int64_t *ext, int halo,
+                 double out[3], hipStream_t st)
+{
+    GridShape s;
+    if (int rc = grid_shape(dims, ext, halo, &s)) return rc;
+    if (!a || !b || !out) return fail(UHS_E_INVAL, "null argument");
+    const int nparts = 1024;
+    double *dev = nullptr;
+    hipError_t e = hipMalloc(&dev, sizeof(double) * (3 * nparts + 3));
+    if (e != hipSuccess) return hip_fail(e, "norm workspace");
+    e = uhs_launch_norms(a, a_f32, b, s, dev, nparts, dev + 3 * nparts, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dev + 3 * nparts, 3 * sizeof(double),
+                                            hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(dev);
+    return e == hipSuccess ? UHS_OK : hip_fail(e, "error norms");
+}
+
+int uhs_error_norms_f32_f64(const float *a, const double *b, int dims, const int64_t *ext, int halo,
+                            double out[3], uhs_stream_t stream)
+{
+    return norms(a, 1, b, dims, ext, halo, out, (hipStream_t)stream);
+}
+
+int uhs_error_norms_f64(const double *a, const double *b, int dims, const int64_t *ext, int halo,
+                        double out[3], uhs_stream_t stream)
+{
+    return norms(a, 0, b, dims, ext, halo, out, (hipStream_t)stream);
+}
+
+const char *uhs_last_error(void) { return g_err.c_str(); }
+const char *uhs_version(void) { return "uhs-mi355x 0.1 (gfx950)"; }
+
+int uhs_device_count(void)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    return n;
+}
+
+int uhs_device_name(char *buf, size_t buflen)
+{
+    hipDeviceProp_t p;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    snprintf(buf, buflen, "%s %s %d CUs", p.name, p.gcnArchName, p.multiProcessorCount);
+    return UHS_OK;
+}
+
+#define HIPCALL(expr, what)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        return e_ == hipSuccess ? UHS_OK : hip_fail(e_, what);                                    \
+    } while (0)
+
+int uhs_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return fail(UHS_E_INVAL, "null pointer");
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) return fail(UHS_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return UHS_OK;
+}
+int uhs_free(void *ptr) { HIPCALL(hipFree(ptr), "hipFree"); }
+int uhs_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+int uhs_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+int uhs_memset(void *dst, int value, size_t bytes) { HIPCALL(hipMemset(dst, value, bytes), "hipMemset"); }
+int uhs_synchronize(void) { HIPCALL(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+int uhs_stream_create(uhs_stream_t *s)
+{
+    if (!s) return fail(UHS_E_INVAL, "null pointer");
+    HIPCALL(hipStreamCreate((hipStream_t *)s), "hipStreamCreate");
+}
+int uhs_stream_destroy(uhs_stream_t s) { HIPCALL(hipStreamDestroy((hipStream_t)s), "hipStreamDestroy"); }
+int uhs_stream_synchronize(uhs_stream_t s)
+{
+    HIPCALL(hipStreamSynchronize((hipStream_t)s), "hipStreamSynchronize");
+}
+int uhs_event_create(uhs_event_t *e)
+{
+    if (!e) return fail(UHS_E_INVAL, "null pointer");
+    HIPCALL(hipEventCreate((hipEvent_t *)e), "hipEventCreate");
+}
+int uhs_event_destroy(uhs_event_t e) { HIPCALL(hipEventDestroy((hipEvent_t)e), "hipEventDestroy"); }
+int uhs_event_record(uhs_event_t e, uhs_stream_t s)
+{
+    HIPCALL(hipEventRecord((hipEvent_t)e, (hipStream_t)s), "hipEventRecord");
+}
+int uhs_event_elapsed_ms(uhs_event_t start, uhs_event_t stop, float *ms)
+{
+    if (!ms) return fail(UHS_E_INVAL, "null pointer");
+    hipError_t e = hipEventSynchronize((hipEvent_t)stop);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    HIPCALL(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
+}
+
+} // extern "C"

@@ -1,0 +1,34 @@
+// star3d kernel instantiations: the three 3D star benchmark stencils, FP64,
+// FP32 and FP32 compensated, 1/2/4 points per thread along the unit-stride axis.
+#include "sweep_kernels.h"
+
+namespace {
+template <typename T, bool K, class S> LaunchFn pick_vec(int vec)
+{
+    switch (vec) {
+    case 1: return launch_star3d<T, S, 1, K>;
+    case 2: return launch_star3d<T, S, 2, K>;
+    case 4: return launch_star3d<T, S, 4, K>;
+    }
+    return nullptr;
+}
+template <class S> LaunchFn pick(int dtype, int vec)
+{
+    switch (dtype) {
+    case DT_F64: return pick_vec<double, false, S>(vec);
+    case DT_F32: return pick_vec<float, false, S>(vec);
+    case DT_F32K: return pick_vec<float, true, S>(vec);
+    }
+    return nullptr;
+}
+} // namespace
+
+LaunchFn uhs_find_star3d(int prog, int dtype, int vec)
+{
+    switch (prog) {
+    case Prog_3d1r_star::ID: return pick<Prog_3d1r_star>(dtype, vec);
+    case Prog_3d2r_star::ID: return pick<Prog_3d2r_star>(dtype, vec);
+    case Prog_3d3r_star::ID: return pick<Prog_3d3r_star>(dtype, vec);
+    }
+    return nullptr;
+}

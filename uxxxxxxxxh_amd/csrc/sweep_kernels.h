@@ -1,0 +1,491 @@
+// The sweep kernel family for gfx950.  One template per data-flow shape; the
+// stencil program S (term offsets in DSL order) is a compile-time parameter
+// so every offset and register index folds to a constant, while the tiling
+// (tile_x/tile_y/tile_z) and the coefficients are runtime arguments.
+//
+// What one kernel launch computes is one sweep of the reference's generated
+// kernel stencil_<name>_kernel (ConvertStencilToStandard.cpp:91-149 loop nest
+// over the interior, :228-272 loads at the term offsets, :339-381 store):
+//     out[p] = c0*in[p+o0] + c1*in[p+o1] + ...      (term order)
+// with halo cells of `out` untouched.  How it is computed is MI355X-specific:
+//
+//   star3d  3D star stencils.  A workgroup owns a tile_x * tile_y column of
+//           the grid and streams a slab of tile_z planes along axis 0.  Each
+//           thread keeps its own column of 2R+1 values in registers
+//           (the axis-0 neighbours), stages the current plane plus an R-wide
+//           ring of halo cells in LDS (the axis-1/axis-2 neighbours), and
+//           prefetches the next plane while computing this one.
+//   box3d   3D stencils whose terms come plane by plane in ascending axis-0
+//           offset (every 3D box in the benchmark set).  The whole plane tile
+//           with halo is staged in LDS; each thread keeps one running sum per
+//           pending output plane and adds the terms of each incoming plane in
+//           term order, so the summation order is exactly the term order.
+//   win2d   2D stencils of any term order.  A workgroup streams a strip of
+//           rows; each thread keeps the (2R+1)x(2R+1) window around its
+//           point(s) in registers, fed one row per step through LDS.
+//   gather  any stencil (fallback): one thread per point, terms read from
+//           global memory through the caches.
+//
+// All loads of out-of-tile cells are clamped into the allocation
+// ([-R, n+R-1] per axis in interior coordinates), so partial edge tiles stay
+// in bounds; their results are simply not stored.
+#pragma once
+#include "uhs_internal.h"
+
+// --------------------------------------------------------------------------
+// star3d
+// --------------------------------------------------------------------------
+template <typename T, class S, int VX, bool KAHAN>
+__global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
+{
+    constexpr int R = S::R, P = S::P, Q = 2 * R + 1, NH = kStarHaloPerThread;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+
+    const int NT = blockDim.x, tid = threadIdx.x;
+    const int BXT = p.BXT, TX = p.TX, TY = p.TY;
+    const int LW = TX + 2 * R, LSZ = LW * (TY + 2 * R);
+    int bid = xcd_remap(blockIdx.x, p.nblocks);
+    const int tile_x = bid % p.tiles_x;
+    bid /= p.tiles_x;
+    const int tile_y = bid % p.tiles_y, chunk = bid / p.tiles_y;
+    const int x0 = tile_x * TX, y0 = tile_y * TY;
+    const int zs = chunk * p.ZC, ze = min(zs + p.ZC, p.n0);
+    const int tx = tid % BXT, ty = tid / BXT;
+    const T *__restrict__ in = p.in;
+    T *__restrict__ out = p.out;
+    const long long sz = p.sz;
+    const int sy = p.sy;
+
+    // This thread's points (x0 + tx + v*BXT, y0 + ty).
+    int coff[VX], lc[VX];
+    bool act[VX];
+    const int yg = y0 + ty, ycl = clampi(yg, -R, p.n1 + R - 1);
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int xg = x0 + tx + v * BXT;
+        act[v] = xg < p.n2 && yg < p.n1;
+        coff[v] = ycl * sy + clampi(xg, -R, p.n2 + R - 1);
+        lc[v] = (ty + R) * LW + tx + v * BXT + R;
+    }
+    // Halo ring cells this thread stages: R rows above/below the tile, R
+    // columns left/right of it (corners are not needed by a star).
+    const int H = 2 * R * (TX + TY);
+    int hoff[NH], hl[NH];
+#pragma unroll
+    for (int k = 0; k < NH; k++) {
+        const int c = tid + k * NT;
+        hl[k] = -1;
+        hoff[k] = 0;
+        if (c < H) {
+            int ly, lx;
+            if (c < 2 * R * TX) {
+                const int row = c / TX;
+                ly = row < R ? row : TY + row;
+                lx = R + c % TX;
+            } else {
+                const int cc = c - 2 * R * TX, col = cc % (2 * R);
+                ly = R + cc / (2 * R);
+                lx = col < R ? col : TX + col;
+            }
+            hl[k] = ly * LW + lx;
+            hoff[k] = clampi(y0 - R + ly, -R, p.n1 + R - 1) * sy +
+                      clampi(x0 - R + lx, -R, p.n2 + R - 1);
+        }
+    }
+
+    Coefs<T, P> cf(p);
+    // Register queue: q[v][j] = plane z-R+j at this thread's point v.
+    T q[VX][Q], nq[VX], nh[NH];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+#pragma unroll
+        for (int j = 0; j < 2 * R; j++) q[v][j] = in[(long long)(zs - R + j) * sz + coff[v]];
+        nq[v] = in[(long long)(zs + R) * sz + coff[v]];
+    }
+#pragma unroll
+    for (int k = 0; k < NH; k++)
+        if (hl[k] >= 0) nh[k] = in[(long long)zs * sz + hoff[k]];
+
+    for (int z = zs; z < ze; ++z) {
+        T *L = lds + (z & 1) * LSZ;
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+            q[v][2 * R] = nq[v];
+            L[lc[v]] = q[v][R];
+        }
+#pragma unroll
+        for (int k = 0; k < NH; k++)
+            if (hl[k] >= 0) L[hl[k]] = nh[k];
+        // Prefetch plane z+1+R (own column) and the halo of plane z+1.
+        {
+            const T *pn = in + (long long)min(z + 1 + R, p.n0 + R - 1) * sz;
+            const T *ph = in + (long long)(z + 1) * sz;
+#pragma unroll
+            for (int v = 0; v < VX; v++) nq[v] = pn[coff[v]];
+#pragma unroll
+            for (int k = 0; k < NH; k++)
+                if (hl[k] >= 0) nh[k] = ph[hoff[k]];
+        }
+        __syncthreads();
+        cf.refresh();
+        T *oz = out + (long long)z * sz;
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+            Acc<T, KAHAN> a;
+            static_for<P>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
+                T val;
+                if constexpr (dy == 0 && dx == 0) {
+                    val = q[v][R + dz];
+                } else {
+                    static_assert(dz == 0, "star3d needs axis-aligned terms");
+                    val = L[lc[v] + dy * LW + dx];
+                }
+                if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                else a.add(cf.template get<t>(), val);
+            });
+            if (act[v]) oz[coff[v]] = a.value();
+        }
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+#pragma unroll
+            for (int j = 0; j < 2 * R; j++) q[v][j] = q[v][j + 1];
+    }
+}
+
+// --------------------------------------------------------------------------
+// box3d (terms grouped by ascending axis-0 offset)
+// --------------------------------------------------------------------------
+template <class S> constexpr bool plane_major()
+{
+    for (int t = 1; t < S::P; t++)
+        if (S::OFF[t][0] < S::OFF[t - 1][0]) return false;
+    return true;
+}
+
+// True when the terms form NJ equal plane groups (one per axis-0 offset,
+// ascending) that list the same in-plane cells in the same order -- every
+// 3D box of the benchmark set.  Then each staged LDS value can be applied to
+// all NJ pending sums at once (one LDS read per cell instead of one per
+// term) while every sum still receives its terms in term order.
+template <class S> constexpr bool uniform_planes()
+{
+    if (!plane_major<S>()) return false;
+    constexpr int NJ = S::OFF[S::P - 1][0] - S::OFF[0][0] + 1;
+    if (S::P % NJ) return false;
+    constexpr int G = S::P / NJ;
+    for (int g = 0; g < NJ; g++)
+        for (int i = 0; i < G; i++) {
+            if (S::OFF[g * G + i][0] != S::OFF[0][0] + g) return false;
+            if (S::OFF[g * G + i][1] != S::OFF[i][1]) return false;
+            if (S::OFF[g * G + i][2] != S::OFF[i][2]) return false;
+        }
+    return true;
+}
+
+// Position of term t in the coefficient pack the box3d kernel reads: cell
+// major (all NJ coefficients of one in-plane cell adjacent, so one scalar
+// load fetches them) when uniform_planes<S>(), term order otherwise.
+template <class S> constexpr int box_pack_index(int t)
+{
+    if constexpr (uniform_planes<S>()) {
+        constexpr int NJ = S::OFF[S::P - 1][0] - S::OFF[0][0] + 1;
+        constexpr int G = S::P / NJ;
+        return (t % G) * NJ + t / G;
+    } else {
+        return t;
+    }
+}
+
+template <typename T, class S, int VX, bool KAHAN>
+__global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
+{
+    static_assert(plane_major<S>(), "box3d needs terms in ascending axis-0 offset");
+    constexpr int R = S::R, P = S::P, NL = kBoxCellsPerThread;
+    constexpr int DZMIN = S::OFF[0][0], DZMAX = S::OFF[P - 1][0];
+    constexpr int NJ = DZMAX - DZMIN + 1; // pending output planes per point
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+
+    const int NT = blockDim.x, tid = threadIdx.x;
+    const int BXT = p.BXT, TX = p.TX, TY = p.TY;
+    const int LW = TX + 2 * R, LSZ = LW * (TY + 2 * R);
+    int bid = xcd_remap(blockIdx.x, p.nblocks);
+    const int tile_x = bid % p.tiles_x;
+    bid /= p.tiles_x;
+    const int tile_y = bid % p.tiles_y, chunk = bid / p.tiles_y;
+    const int x0 = tile_x * TX, y0 = tile_y * TY;
+    const int zs = chunk * p.ZC, ze = min(zs + p.ZC, p.n0);
+    const int tx = tid % BXT, ty = tid / BXT;
+    const T *__restrict__ in = p.in;
+    T *__restrict__ out = p.out;
+    const long long sz = p.sz;
+    const int sy = p.sy;
+
+    int coff[VX], lc[VX];
+    bool act[VX];
+    const int yg = y0 + ty;
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int xg = x0 + tx + v * BXT;
+        act[v] = xg < p.n2 && yg < p.n1;
+        coff[v] = act[v] ? yg * sy + xg : 0;
+        lc[v] = (ty + R) * LW + tx + v * BXT + R;
+    }
+    // Plane-tile cells (with R-wide halo, corners included) staged by this thread.
+    int goff[NL];
+    bool gv[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int c = tid + k * NT;
+        gv[k] = c < LSZ;
+        const int ly = gv[k] ? c / LW : 0, lx = gv[k] ? c % LW : 0;
+        goff[k] = clampi(y0 - R + ly, -R, p.n1 + R - 1) * sy +
+                  clampi(x0 - R + lx, -R, p.n2 + R - 1);
+    }
+
+    T pl[NL];
+    {
+        const T *pp = in + (long long)(zs + DZMIN) * sz;
+#pragma unroll
+        for (int k = 0; k < NL; k++)
+            if (gv[k]) pl[k] = pp[goff[k]];
+    }
+    Coefs<T, P> cf(p);
+    // acc[v][j] is the running sum of output plane zp - DZMAX + j.
+    Acc<T, KAHAN> acc[VX][NJ];
+#pragma unroll
+    for (int v = 0; v < VX; v++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++) acc[v][j].zero();
+    const int steps = (ze - zs) + NJ - 1;
+    for (int s = 0; s < steps; ++s) {
+        const int zp = zs + DZMIN + s;
+        T *L = lds + (s & 1) * LSZ;
+#pragma unroll
+        for (int k = 0; k < NL; k++)
+            if (gv[k]) L[tid + k * NT] = pl[k];
+        {
+            const T *pn = in + (long long)min(zp + 1, p.n0 + R - 1) * sz;
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+                if (gv[k]) pl[k] = pn[goff[k]];
+        }
+        __syncthreads();
+        cf.refresh();
+        const int zo = zp - DZMAX;
+        T *oz = out + (long long)zo * sz;
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+            if constexpr (uniform_planes<S>()) {
+                // Cell by cell: the value and the NJ coefficients of cell i+1
+                // are fetched while cell i is applied to the NJ pending sums;
+                // pin() keeps the compiler from hoisting every LDS read to the
+                // top of the step (which would hold all G cells live).
+                constexpr int G = P / NJ;
+                T cur = L[lc[v] + S::OFF[0][1] * LW + S::OFF[0][2]];
+                T cc[NJ];
+                static_for<NJ>([&](auto gc) {
+                    constexpr int g = decltype(gc)::value;
+                    cc[g] = cf.template get<box_pack_index<S>(g * G)>();
+                });
+                static_for<G>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    T nxt = cur, nc[NJ];
+                    if constexpr (i + 1 < G) {
+                        nxt = L[lc[v] + S::OFF[i + 1][1] * LW + S::OFF[i + 1][2]];
+                        static_for<NJ>([&](auto gc) {
+                            constexpr int g = decltype(gc)::value;
+                            nc[g] = cf.template get<box_pack_index<S>(g * G + i + 1)>();
+                        });
+                    }
+                    static_for<NJ>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        constexpr int t = g * G + i, j = NJ - 1 - g;
+                        if constexpr (t == 0) acc[v][j].first(cc[g], cur);
+                        else acc[v][j].add(cc[g], cur);
+                    });
+                    if constexpr (i + 1 < G) {
+#pragma unroll
+                        for (int j = 0; j < NJ; j++) acc[v][j].pin();
+                        cur = nxt;
+#pragma unroll
+                        for (int g = 0; g < NJ; g++) cc[g] = nc[g];
+                    }
+                });
+            } else {
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
+                    constexpr int j = DZMAX - dz;
+                    const T val = L[lc[v] + dy * LW + dx];
+                    if constexpr (t == 0) acc[v][j].first(cf.template get<t>(), val);
+                    else acc[v][j].add(cf.template get<t>(), val);
+                });
+            }
+            if (zo >= zs && act[v]) oz[coff[v]] = acc[v][0].value();
+#pragma unroll
+            for (int j = 0; j + 1 < NJ; j++) acc[v][j] = acc[v][j + 1];
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// win2d
+// --------------------------------------------------------------------------
+template <typename T, class S, int VX, bool KAHAN>
+__global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
+{
+    constexpr int R = S::R, P = S::P, W = 2 * R + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+
+    const int tid = threadIdx.x;
+    const int BXT = p.BXT, TX = p.TX, LW = TX + 2 * R;
+    const int bid = xcd_remap(blockIdx.x, p.nblocks);
+    const int tile_x = bid % p.tiles_x, chunk = bid / p.tiles_x;
+    const int x0 = tile_x * TX;
+    const int ys = chunk * p.ZC, ye = min(ys + p.ZC, p.n0);
+    const T *__restrict__ in = p.in;
+    T *__restrict__ out = p.out;
+    const long long sy = p.sy;
+
+    int xcl[VX], xg[VX];
+    bool act[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        xg[v] = x0 + tid + v * BXT;
+        act[v] = xg[v] < p.n2;
+        xcl[v] = clampi(xg[v], -R, p.n2 + R - 1);
+    }
+    // Threads 0..2R-1 also stage the R halo columns on each side.
+    const bool hthr = tid < 2 * R;
+    const int hlx = tid < R ? tid : TX + tid;
+    const int hoff = clampi(x0 - R + hlx, -R, p.n2 + R - 1);
+
+    T nrow[VX], nh = T(0);
+    {
+        const T *pr = in + (long long)(ys - R) * sy;
+#pragma unroll
+        for (int v = 0; v < VX; v++) nrow[v] = pr[xcl[v]];
+        if (hthr) nh = pr[hoff];
+    }
+    Coefs<T, P> cf(p);
+    // w[v][a][b] = in[row - R + a][x_v - R + b] for the output row being built.
+    T w[VX][W][W];
+#pragma unroll
+    for (int v = 0; v < VX; v++)
+#pragma unroll
+        for (int a = 0; a < W; a++)
+#pragma unroll
+            for (int b = 0; b < W; b++) w[v][a][b] = T(0);
+    const int steps = (ye - ys) + 2 * R;
+    for (int s = 0; s < steps; ++s) {
+        const int yr = ys - R + s;
+        T *L = lds + (s & 1) * LW;
+#pragma unroll
+        for (int v = 0; v < VX; v++) L[R + tid + v * BXT] = nrow[v];
+        if (hthr) L[hlx] = nh;
+        {
+            const T *pr = in + (long long)min(yr + 1, p.n0 + R - 1) * sy;
+#pragma unroll
+            for (int v = 0; v < VX; v++) nrow[v] = pr[xcl[v]];
+            if (hthr) nh = pr[hoff];
+        }
+        __syncthreads();
+        cf.refresh();
+        const int yo = yr - R;
+        T *orow = out + (long long)yo * sy;
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+#pragma unroll
+            for (int a = 0; a < 2 * R; a++)
+#pragma unroll
+                for (int b = 0; b < W; b++) w[v][a][b] = w[v][a + 1][b];
+#pragma unroll
+            for (int b = 0; b < W; b++) w[v][2 * R][b] = L[tid + v * BXT + b];
+            if (s >= 2 * R) {
+                Acc<T, KAHAN> acc;
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int dy = S::OFF[t][0], dx = S::OFF[t][1];
+                    const T val = w[v][R + dy][R + dx];
+                    if constexpr (t == 0) acc.first(cf.template get<t>(), val);
+                    else acc.add(cf.template get<t>(), val);
+                });
+                if (act[v]) orow[xg[v]] = acc.value();
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// gather (any stencil)
+// --------------------------------------------------------------------------
+template <typename T> struct GParams {
+    const T *in;
+    T *out;
+    int n0, n1, n2, P;
+    int sy;
+    long long sz;
+    int loff[kGatherMaxPoints];
+    T c[kGatherMaxPoints];
+};
+
+template <typename T, bool KAHAN>
+__global__ void __launch_bounds__(256) k_gather(GParams<T> p)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long plane = (long long)p.n1 * p.n2;
+    if (i >= plane * p.n0) return;
+    const int z = (int)(i / plane);
+    const int rem = (int)(i - (long long)z * plane);
+    const int y = rem / p.n2, x = rem - y * p.n2;
+    const long long pt = (long long)z * p.sz + (long long)y * p.sy + x;
+    const T *__restrict__ in = p.in + pt;
+    Acc<T, KAHAN> a;
+    a.first(p.c[0], in[p.loff[0]]);
+    for (int t = 1; t < p.P; t++) a.add(p.c[t], in[p.loff[t]]);
+    p.out[pt] = a.value();
+}
+
+// --------------------------------------------------------------------------
+// Launchers
+// --------------------------------------------------------------------------
+template <typename T> constexpr int dtype_of(bool kahan)
+{
+    return sizeof(T) == 8 ? DT_F64 : (kahan ? DT_F32K : DT_F32);
+}
+
+template <typename T, class S, int VX, bool KAHAN>
+hipError_t launch_star3d(const SweepArgs &a, hipStream_t st)
+{
+    auto p = make_params<T, S::P>(a);
+    const int nt = (a.tx / VX) * a.ty;
+    const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R) * (a.ty + 2 * S::R);
+    hipLaunchKernelGGL((k_star3d<T, S, VX, KAHAN>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    return hipGetLastError();
+}
+
+template <typename T, class S, int VX, bool KAHAN>
+hipError_t launch_box3d(const SweepArgs &a, hipStream_t st)
+{
+    auto p = make_params<T, S::P>(a);
+    for (int t = 0; t < S::P; t++) p.c[box_pack_index<S>(t)] = (T)a.coef[t];
+    const int nt = (a.tx / VX) * a.ty;
+    const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R) * (a.ty + 2 * S::R);
+    hipLaunchKernelGGL((k_box3d<T, S, VX, KAHAN>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    return hipGetLastError();
+}
+
+template <typename T, class S, int VX, bool KAHAN>
+hipError_t launch_win2d(const SweepArgs &a, hipStream_t st)
+{
+    auto p = make_params<T, S::P>(a);
+    const int nt = a.tx / VX;
+    const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R);
+    hipLaunchKernelGGL((k_win2d<T, S, VX, KAHAN>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    return hipGetLastError();
+}

@@ -1,0 +1,32 @@
+// Host-side planning shared by the C ABI, the batch runner and the autotuner.
+#pragma once
+#include "uhs.h"
+#include "uhs_internal.h"
+#include <string>
+
+enum Family { FAM_STAR3D = 0, FAM_BOX3D = 1, FAM_WIN2D = 2, FAM_GATHER = 3 };
+
+// A resolved sweep: which kernel, with which arguments and configuration.
+struct Plan {
+    int family;
+    int prog;     // index into kBuiltins, -1 for a general stencil
+    int radius;
+    uhs_config cfg;
+    LaunchFn fn;
+    SweepArgs args;
+};
+
+int uhs_set_error(int code, const char *msg);
+int uhs_match_builtin(int dims, int npoints, const int *off);
+int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,
+                  const uhs_config *cfg, Plan *pl);
+int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream);
+int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whylen);
+uhs_config uhs_heuristic_config(const Plan &pl);
+long long uhs_config_blocks(const SweepArgs &a, const uhs_config &c);
+
+// Tuned configurations, keyed by kernel family, stencil, precision and extents.
+std::string uhs_config_key(int family, int prog, int dtype, int dims, const int64_t *ext);
+void uhs_store_tuned(const std::string &key, const uhs_config &c);
+bool uhs_lookup_tuned(const std::string &key, uhs_config *c);
+void uhs_clear_tuned();

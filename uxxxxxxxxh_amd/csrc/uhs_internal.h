@@ -1,0 +1,189 @@
+// Internal declarations shared by the HIP kernel files and the host library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <utility>
+
+struct BuiltinStencil {
+    const char *case_name; // UH-GPU case directory, e.g. "3d1r_star"
+    const char *symbol;    // reference stencil name, e.g. "stencil_3d7pt"
+    int dims, radius, npoints, star;
+    const int *off;        // [npoints][dims], DSL term order
+    const double *coef;    // DSL coefficients
+};
+
+#include "shape_table.inc"
+
+enum Dtype { DT_F64 = 0, DT_F32 = 1, DT_F32K = 2 };
+
+// Everything a launcher needs; built by the host dispatcher.
+struct SweepArgs {
+    const void *in;        // halo-inclusive source grid (base pointer)
+    void *out;             // halo-inclusive destination grid
+    int dims;
+    int n0, n1, n2;        // interior extents; 2D grids use n0 = rows, n1 = 1, n2 = cols
+    int halo;
+    long long sy, sz;      // row and plane strides in elements
+    int tx, ty, tz, vec;   // tiling configuration
+    int npoints;
+    const int *off;        // host copy of the offsets (gather path)
+    const double *coef;    // host copy of the coefficients
+    int dtype;
+};
+
+typedef hipError_t (*LaunchFn)(const SweepArgs &, hipStream_t);
+
+// Per-family kernel tables (nullptr when the combination is not compiled).
+LaunchFn uhs_find_star3d(int prog, int dtype, int vec);
+LaunchFn uhs_find_box3d(int prog, int dtype, int vec);
+LaunchFn uhs_find_win2d(int prog, int dtype, int vec);
+LaunchFn uhs_find_gather(int dtype);
+
+// Limits baked into the kernels; the host rejects configurations beyond them.
+constexpr int kMaxThreads = 512;      // __launch_bounds__ of every sweep kernel
+constexpr int kStarHaloPerThread = 4; // star3d: halo cells staged per thread per plane
+constexpr int kBoxCellsPerThread = 8; // box3d: plane-tile cells staged per thread
+constexpr int kGatherMaxPoints = 256; // gather: terms passed in kernel arguments
+constexpr size_t kMaxLdsBytes = 64 * 1024;
+
+// Number of XCDs workgroups are dealt across (round-robin by workgroup id).
+constexpr int kNumXcd = 8;
+
+#if defined(__HIPCC__)
+// Calls f(std::integral_constant<int, I>) for I = 0..N-1, fully unrolled at
+// compile time, so term offsets and register indices are constants.
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+        (f(std::integral_constant<int, I>{}), ...);
+    }(std::make_integer_sequence<int, N>{});
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Workgroups are dealt round-robin to the XCDs, each with its own L2.  Give
+// every XCD one contiguous range of logical tiles, so tiles that share halo
+// rows also share an L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
+{
+    const int per = nblocks / kNumXcd, rem = nblocks % kNumXcd;
+    const int x = bid % kNumXcd, i = bid / kNumXcd;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+}
+
+// pin(): an empty asm that takes the running sum as an operand and clobbers
+// memory.  Placed after a group of terms, it makes the compiler finish those
+// terms before any later LDS read, which bounds how many staged values are
+// live at once.
+//
+// Term accumulation.  The first term is a product; every later term is one
+// fused multiply-add (FP64/FP32), or a Kahan-compensated add of the fused
+// product (FP32 compensated form):
+//     y = fma(c, v, -e); u = s + y; e = (u - s) - y; s = u.
+template <typename T, bool KAHAN> struct Acc;
+
+template <> struct Acc<double, false> {
+    double s;
+    __device__ __forceinline__ void zero() { s = 0.0; }
+    __device__ __forceinline__ void first(double c, double v) { s = c * v; }
+    __device__ __forceinline__ void add(double c, double v) { s = __builtin_fma(c, v, s); }
+    __device__ __forceinline__ double value() const { return s; }
+    __device__ __forceinline__ void pin() { asm volatile("" : "+v"(s)::"memory"); }
+};
+template <> struct Acc<float, false> {
+    float s;
+    __device__ __forceinline__ void zero() { s = 0.0f; }
+    __device__ __forceinline__ void first(float c, float v) { s = c * v; }
+    __device__ __forceinline__ void add(float c, float v) { s = __builtin_fmaf(c, v, s); }
+    __device__ __forceinline__ float value() const { return s; }
+    __device__ __forceinline__ void pin() { asm volatile("" : "+v"(s)::"memory"); }
+};
+template <> struct Acc<float, true> {
+    float s, e;
+    __device__ __forceinline__ void zero() { s = 0.0f; e = 0.0f; }
+    __device__ __forceinline__ void first(float c, float v) { s = c * v; e = 0.0f; }
+    __device__ __forceinline__ void add(float c, float v)
+    {
+        const float y = __builtin_fmaf(c, v, -e);
+        const float u = s + y;
+        e = (u - s) - y;
+        s = u;
+    }
+    __device__ __forceinline__ float value() const { return s; }
+    __device__ __forceinline__ void pin() { asm volatile("" : "+v"(s), "+v"(e)::"memory"); }
+};
+
+// Kernel parameters, passed by value (coefficients travel in the kernel
+// argument segment: no device allocation, nothing to copy per call).
+template <typename T, int P> struct KParams {
+    const T *in;           // interior origin (0,0,0) of the source grid
+    T *out;                // interior origin of the destination grid
+    int n0, n1, n2;
+    int sy;
+    long long sz;
+    int TX, TY, ZC, BXT;
+    int tiles_x, tiles_y, nblocks;
+    T c[P];
+};
+
+// Coefficient access.  Up to kHoistPoints coefficients stay in SGPRs for the
+// whole kernel.  Beyond that they cannot (a 343-term FP64 stencil needs 686
+// SGPRs): the compiler would spill them into VGPR lanes and pay a v_readlane
+// per term.  Large stencils therefore re-read their coefficients with scalar
+// loads from the kernel-argument segment every step, through a pointer the
+// compiler cannot treat as loop-invariant.
+constexpr int kHoistPoints = 32;
+
+template <typename T, int P> struct Coefs {
+    typedef const __attribute__((address_space(4))) T *kptr;
+    kptr ptr;
+    T hoisted[P <= kHoistPoints ? P : 1];
+    __device__ __forceinline__ explicit Coefs(const KParams<T, P> &p)
+    {
+        if constexpr (P <= kHoistPoints) {
+            for (int t = 0; t < P; t++) hoisted[t] = p.c[t];
+            ptr = nullptr;
+        } else {
+            auto base = (const __attribute__((address_space(4))) char *)
+                __builtin_amdgcn_kernarg_segment_ptr();
+            using KP = KParams<T, P>;
+            ptr = (kptr)(base + __builtin_offsetof(KP, c));
+        }
+    }
+    // Call once per step: hides the pointer's value from loop-invariant code
+    // motion so the scalar loads are issued inside the step.
+    __device__ __forceinline__ void refresh()
+    {
+        if constexpr (P > kHoistPoints) asm volatile("" : "+s"(ptr));
+    }
+    template <int t> __device__ __forceinline__ T get() const
+    {
+        if constexpr (P <= kHoistPoints) return hoisted[t];
+        else return ptr[t];
+    }
+};
+
+template <typename T, int P>
+inline KParams<T, P> make_params(const SweepArgs &a)
+{
+    KParams<T, P> p;
+    const long long h = a.halo;
+    const long long origin = (a.dims == 3 ? h * a.sz : 0) + h * a.sy + h;
+    p.in = static_cast<const T *>(a.in) + origin;
+    p.out = static_cast<T *>(a.out) + origin;
+    p.n0 = a.n0; p.n1 = a.n1; p.n2 = a.n2;
+    p.sy = (int)a.sy; p.sz = a.sz;
+    p.TX = a.tx; p.TY = a.ty; p.ZC = a.tz; p.BXT = a.tx / a.vec;
+    p.tiles_x = (a.n2 + a.tx - 1) / a.tx;
+    p.tiles_y = a.dims == 3 ? (a.n1 + a.ty - 1) / a.ty : 1;
+    const int chunks = (a.n0 + a.tz - 1) / a.tz;
+    p.nblocks = p.tiles_x * p.tiles_y * chunks;
+    for (int t = 0; t < P; t++) p.c[t] = (T)a.coef[t];
+    return p;
+}
+#endif

@@ -40,6 +40,14 @@ class Config(ctypes.Structure):
         return "Config(%d,%d,%d,%d)" % (self.tile_x, self.tile_y, self.tile_z, self.vec)
 
 
+class DslInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("kernel", ctypes.c_char * 64), ("dims", ctypes.c_int),
+                ("is_float", ctypes.c_int), ("iteration", ctypes.c_int), ("halo", ctypes.c_int),
+                ("npoints", ctypes.c_int), ("grid", ctypes.c_int64 * 3), ("domain_lo", ctypes.c_int64 * 3),
+                ("domain_hi", ctypes.c_int64 * 3), ("extents", ctypes.c_int64 * 3),
+                ("offsets", ctypes.c_int * (343 * 3)), ("coeffs", ctypes.c_double * 343)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -58,6 +66,8 @@ _SIGS = {
     "uhs_make_coeffs": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     "uhs_error_norms_f32_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_error_norms_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
+    "uhs_parse_dsl": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
+    "uhs_parse_dsl_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
     "uhs_last_error": (ctypes.c_char_p, []),
     "uhs_version": (ctypes.c_char_p, []),
     "uhs_device_count": (_I, []),
@@ -152,6 +162,19 @@ def builtin(case, coeffs=None):
     s = StencilSpec(dims.value, offs, c, name.value.decode(), rad.value)
     s.symbol = sym.value.decode()
     return s
+
+
+def parse_dsl(text=None, path=None):
+    """Read UHStencil DSL text (or a .dsl file) into (StencilSpec, DslInfo)."""
+    info = DslInfo()
+    if path is not None:
+        check(lib().uhs_parse_dsl_file(path.encode(), ctypes.byref(info)))
+    else:
+        check(lib().uhs_parse_dsl(text.encode(), ctypes.byref(info)))
+    d, n = info.dims, info.npoints
+    offs = np.array(info.offsets[:n * d], dtype=np.int32)
+    st = StencilSpec(d, offs, np.array(info.coeffs[:n]), info.name.decode())
+    return st, info
 
 
 def make_coeffs(stencil, seed):

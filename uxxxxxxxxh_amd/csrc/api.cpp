@@ -103,18 +103,23 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
 {
     const SweepArgs &a = pl.args;
     const int R = pl.radius;
-    auto bad = [&](const char *fmt, auto... v) {
-        snprintf(why, whylen, fmt, v...);
+    auto bad = [&](const std::string &msg) {
+        snprintf(why, whylen, "%s", msg.c_str());
         return UHS_E_CONFIG;
     };
+    auto str = [](const char *fmt, long long a = 0, long long b = 0) {
+        char buf[160];
+        snprintf(buf, sizeof buf, fmt, a, b);
+        return std::string(buf);
+    };
     if (pl.family == FAM_GATHER) return UHS_OK;
-    if (c.vec != 1 && c.vec != 2 && c.vec != 4) return bad("vec must be 1, 2 or 4 (got %d)", c.vec);
+    if (c.vec != 1 && c.vec != 2 && c.vec != 4) return bad(str("vec must be 1, 2 or 4 (got %lld)", c.vec));
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
-    if (c.tile_x % c.vec) return bad("tile_x %d not a multiple of vec %d", c.tile_x, c.vec);
+    if (c.tile_x % c.vec) return bad(str("tile_x %lld not a multiple of vec %lld", c.tile_x, c.vec));
     if (a.dims == 2 && c.tile_y != 1) return bad("2D stencils need tile_y = 1");
     const long long nt = (long long)(c.tile_x / c.vec) * c.tile_y;
-    if (nt > kMaxThreads) return bad("%lld threads per workgroup exceeds %d", nt, kMaxThreads);
-    if (nt < 2 * R) return bad("%lld threads cannot stage a radius-%d halo", nt, R);
+    if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
+    if (nt < 2 * R) return bad(str("%lld threads cannot stage a radius-%lld halo", nt, R));
     const size_t esz = a.dtype == DT_F64 ? 8 : 4;
     size_t lds;
     if (pl.family == FAM_WIN2D) {
@@ -123,20 +128,20 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         lds = 2 * esz * (size_t)(c.tile_x + 2 * R) * (c.tile_y + 2 * R);
         if (pl.family == FAM_STAR3D &&
             ceil_div(2LL * R * (c.tile_x + c.tile_y), nt) > kStarHaloPerThread)
-            return bad("halo ring of %d cells needs more than %d per thread",
-                       2 * R * (c.tile_x + c.tile_y), kStarHaloPerThread);
+            return bad(str("halo ring of %lld cells needs more than %lld per thread",
+                           2 * R * (c.tile_x + c.tile_y), kStarHaloPerThread));
         if (pl.family == FAM_BOX3D &&
             ceil_div((long long)(c.tile_x + 2 * R) * (c.tile_y + 2 * R), nt) > kBoxCellsPerThread)
-            return bad("plane tile of %d cells needs more than %d per thread",
-                       (c.tile_x + 2 * R) * (c.tile_y + 2 * R), kBoxCellsPerThread);
+            return bad(str("plane tile of %lld cells needs more than %lld per thread",
+                           (c.tile_x + 2 * R) * (c.tile_y + 2 * R), kBoxCellsPerThread));
     }
-    if (lds > kMaxLdsBytes) return bad("%zu bytes of LDS exceeds %zu", lds, kMaxLdsBytes);
+    if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
     if (uhs_config_blocks(a, c) > INT_MAX) return bad("too many workgroups");
     LaunchFn fn = nullptr;
     if (pl.family == FAM_STAR3D) fn = uhs_find_star3d(pl.prog, a.dtype, c.vec);
     if (pl.family == FAM_BOX3D) fn = uhs_find_box3d(pl.prog, a.dtype, c.vec);
     if (pl.family == FAM_WIN2D) fn = uhs_find_win2d(pl.prog, a.dtype, c.vec);
-    if (!fn) return bad("no %s kernel compiled for vec %d", family_name(pl.family), c.vec);
+    if (!fn) return bad(std::string("no ") + family_name(pl.family) + str(" kernel compiled for vec %lld", c.vec));
     return UHS_OK;
 }
 

@@ -75,9 +75,13 @@ typedef struct uhs_stencil {
  *   tile_y  rows per workgroup (3D only; must be 1 for 2D)
  *   tile_z  length of the slab streamed through registers along axis 0
  *   vec     points per thread along the unit-stride axis (1, 2 or 4)
+ *   flags   UHS_CFG_NT_STORE: output stores bypass the caches;
+ *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight
  * Threads per workgroup = tile_x / vec * tile_y. */
+#define UHS_CFG_NT_STORE 1
+#define UHS_CFG_PREFETCH2 2
 typedef struct uhs_config {
-    int tile_x, tile_y, tile_z, vec;
+    int tile_x, tile_y, tile_z, vec, flags;
 } uhs_config;
 
 /* ---- the sweep ---------------------------------------------------------- */
@@ -144,6 +148,47 @@ int uhs_error_norms_f32_f64(const float *a, const double *b, int dims,
 int uhs_error_norms_f64(const double *a, const double *b, int dims,
                         const int64_t *ext, int halo, double out[3],
                         uhs_stream_t stream);
+
+/* ---- batched grids: one hipGraph, persistent work-queue launches -------- */
+
+/* One grid of a batch: halo-inclusive FP64 grids A and B, interior extents,
+ * halo, and its stencil (one of the six built-in 3D stencils). */
+typedef struct uhs_grid_desc {
+    double *A, *B;
+    int dims;            /* 3 */
+    int64_t ext[3];
+    int halo;
+    uhs_stencil stencil;
+} uhs_grid_desc;
+
+/* Tiling of the batched launches.  Star stencils are advanced star_depth
+ * steps per launch (temporal blocking with ghost zones, 1, 2 or 4); box
+ * stencils one step per launch.  box_tile_x * box_tile_y must be 256. */
+typedef struct uhs_batch_config {
+    int star_depth;
+    int star_tile_x, star_tile_y, star_tile_z;
+    int box_tile_x, box_tile_y, box_tile_z;
+} uhs_batch_config;
+
+typedef struct uhs_batch *uhs_batch_t;
+
+/* Plans T timesteps of every grid, uploads the task lists and captures the
+ * whole run as one hipGraph (a counter-reset node, then one persistent
+ * launch per time block in which workgroups take tiles of any grid from a
+ * device-side queue, largest first).  Grids keep the ping-pong convention:
+ * after the run each grid's result is in A for even T, in B for odd T; the
+ * other buffer holds an earlier state.  cfg NULL = the default. */
+int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps,
+                     const uhs_batch_config *cfg, uhs_batch_t *out);
+/* Launches the captured graph on `stream` (asynchronous; reusable). */
+int uhs_batch_run(uhs_batch_t b, uhs_stream_t stream);
+int uhs_batch_destroy(uhs_batch_t b);
+int uhs_batch_info(uhs_batch_t b, int *launches, long long *tasks,
+                   int *max_workgroups, size_t *max_lds_bytes);
+int uhs_batch_default_config(uhs_batch_config *out);
+/* The unbatched baseline: uhs_sweep_f64 on each grid in turn, same stream. */
+int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps,
+                       uhs_stream_t stream);
 
 /* ---- plumbing: device memory, streams, timing ---------------------------- */
 

@@ -1,0 +1,91 @@
+"""The batched path (one hipGraph of persistent work-queue launches, star
+grids temporally blocked) against the oracle: each grid's result buffer is
+bitwise equal to T single sweeps of the oracle's fused mode, for uneven
+extents, mixed stencils, nonzero halos that differ between A and B, every
+temporal depth and odd/even T."""
+import sys
+
+import numpy as np
+import pytest
+
+from tests.oracle_py import FUSED
+
+pytestmark = pytest.mark.gpu
+
+
+def make_grids(u, orc, kinds, seed, halo_extra=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, case in enumerate(kinds):
+        st = u.builtin(case)
+        st = st.with_coeffs(u.make_coeffs(st, 100 + i))
+        h = st.radius + halo_extra
+        ext = [int(rng.integers(5, 41)) for _ in range(3)]
+        shape = [e + 2 * h for e in ext]
+        A0 = rng.random(shape)
+        B0 = rng.random(shape)
+        out.append((case, st, ext, h, A0, B0))
+    return out
+
+
+def check_batch(u, orc, grids, T, cfg):
+    dev = []
+    for case, st, ext, h, A0, B0 in grids:
+        dev.append((u.DeviceArray.from_host(A0), u.DeviceArray.from_host(B0), ext, h, st))
+    b = u.Batch(dev, T, cfg)
+    print("batch", cfg, b.info(), file=sys.stderr, flush=True)
+    b.run()
+    u.synchronize()
+    for (case, st, ext, h, A0, B0), (dA, dB, _, _, _) in zip(grids, dev):
+        A, B = A0.copy(), B0.copy()
+        orc.iterate(A, B, ext, h, st.offsets, st.coeffs, T, FUSED)
+        res, ref = (dA.download(), A) if T % 2 == 0 else (dB.download(), B)
+        assert np.array_equal(res, ref), (case, ext, h, T, cfg)
+    # the graph is reusable: a second run continues from the current state
+    b.close()
+    for dA, dB, _, _, _ in dev:
+        dA.close()
+        dB.close()
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4])
+@pytest.mark.parametrize("T", [1, 2, 5, 8])
+def test_batch_mix_star_box(gpu, orc, depth, T):
+    u = gpu
+    kinds = ["3d1r_star", "3d2r_box"] * 4
+    grids = make_grids(u, orc, kinds, seed=depth * 10 + T)
+    cfg = u.batch_default_config()
+    cfg.star_depth = depth
+    cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = 16, 8, 7
+    cfg.box_tile_x, cfg.box_tile_y, cfg.box_tile_z = 32, 8, 9
+    check_batch(u, orc, grids, T, cfg)
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4])
+def test_batch_all_3d_kinds_wide_halo(gpu, orc, depth):
+    u = gpu
+    kinds = ["3d1r_star", "3d2r_star", "3d3r_star", "3d1r_box", "3d2r_box", "3d3r_box"]
+    grids = make_grids(u, orc, kinds, seed=7 + depth, halo_extra=1)
+    cfg = u.batch_default_config()
+    cfg.star_depth = depth
+    cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = 32, 4, 16
+    check_batch(u, orc, grids, 6, cfg)
+
+
+def test_batch_default_config_and_rejections(gpu, orc):
+    u = gpu
+    grids = make_grids(u, orc, ["3d1r_star", "3d2r_box", "3d1r_star"], seed=3)
+    check_batch(u, orc, grids, 4, None)
+    case, st, ext, h, A0, B0 = grids[0]
+    dA = u.DeviceArray.from_host(A0)
+    dB = u.DeviceArray.from_host(B0)
+    bad = u.batch_default_config()
+    bad.star_depth = 3
+    with pytest.raises(u.UhsError) as e:
+        u.Batch([(dA, dB, ext, h, st)], 2, bad)
+    assert e.value.code == -3
+    two_d = u.builtin("2d1r_star")
+    with pytest.raises(u.UhsError):
+        u.Batch([(dA, dB, ext, h, two_d)], 2)
+    dA.close()
+    dB.close()

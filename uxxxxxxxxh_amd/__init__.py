@@ -18,6 +18,7 @@ UHS_OK = 0
 ERRORS = {-1: "UHS_E_INVAL", -2: "UHS_E_UNSUPPORTED", -3: "UHS_E_CONFIG", -4: "UHS_E_HIP",
           -5: "UHS_E_NOMEM", -6: "UHS_E_IO", -7: "UHS_E_PARSE"}
 DT_F64, DT_F32, DT_F32K = 0, 1, 2
+CFG_NT_STORE, CFG_PREFETCH2 = 1, 2
 
 
 class UhsError(RuntimeError):
@@ -34,10 +35,13 @@ class Stencil(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int),
-                ("tile_z", ctypes.c_int), ("vec", ctypes.c_int)]
+                ("tile_z", ctypes.c_int), ("vec", ctypes.c_int), ("flags", ctypes.c_int)]
 
     def __repr__(self):
-        return "Config(%d,%d,%d,%d)" % (self.tile_x, self.tile_y, self.tile_z, self.vec)
+        return "Config(%d,%d,%d,%d,%d)" % (self.tile_x, self.tile_y, self.tile_z, self.vec, self.flags)
+
+    def tuple(self):
+        return (self.tile_x, self.tile_y, self.tile_z, self.vec, self.flags)
 
 
 class DslInfo(ctypes.Structure):
@@ -46,6 +50,22 @@ class DslInfo(ctypes.Structure):
                 ("npoints", ctypes.c_int), ("grid", ctypes.c_int64 * 3), ("domain_lo", ctypes.c_int64 * 3),
                 ("domain_hi", ctypes.c_int64 * 3), ("extents", ctypes.c_int64 * 3),
                 ("offsets", ctypes.c_int * (343 * 3)), ("coeffs", ctypes.c_double * 343)]
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("dims", ctypes.c_int),
+                ("ext", ctypes.c_int64 * 3), ("halo", ctypes.c_int), ("stencil", Stencil)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("star_depth", ctypes.c_int), ("star_tile_x", ctypes.c_int), ("star_tile_y", ctypes.c_int),
+                ("star_tile_z", ctypes.c_int), ("box_tile_x", ctypes.c_int), ("box_tile_y", ctypes.c_int),
+                ("box_tile_z", ctypes.c_int)]
+
+    def __repr__(self):
+        return "BatchConfig(depth=%d star=%dx%dx%d box=%dx%dx%d)" % (
+            self.star_depth, self.star_tile_x, self.star_tile_y, self.star_tile_z,
+            self.box_tile_x, self.box_tile_y, self.box_tile_z)
 
 
 _P = ctypes.c_void_p
@@ -68,6 +88,13 @@ _SIGS = {
     "uhs_error_norms_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_parse_dsl": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
     "uhs_parse_dsl_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
+    "uhs_batch_create": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(BatchConfig), ctypes.POINTER(_P)]),
+    "uhs_batch_run": (_I, [_P, _P]),
+    "uhs_batch_destroy": (_I, [_P]),
+    "uhs_batch_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I),
+                            ctypes.POINTER(ctypes.c_size_t)]),
+    "uhs_batch_default_config": (_I, [ctypes.POINTER(BatchConfig)]),
+    "uhs_sweep_each_f64": (_I, [ctypes.POINTER(GridDesc), _I, _I, _P]),
     "uhs_last_error": (ctypes.c_char_p, []),
     "uhs_version": (ctypes.c_char_p, []),
     "uhs_device_count": (_I, []),
@@ -297,3 +324,51 @@ class Timer:
         v = ctypes.c_float()
         check(lib().uhs_event_elapsed_ms(self.a, self.b, ctypes.byref(v)))
         return v.value
+
+
+class Batch:
+    """A planned, graph-captured batch of 3D FP64 grids (uhs_batch_*)."""
+
+    def __init__(self, grids, timesteps, cfg=None):
+        """grids: list of (A, B, ext, halo, StencilSpec) with DeviceArray A/B."""
+        self._keep = [g[4] for g in grids]
+        arr = (GridDesc * len(grids))()
+        for i, (A, B, ext, halo, st) in enumerate(grids):
+            arr[i] = GridDesc(A.ptr.value, B.ptr.value, 3, (ctypes.c_int64 * 3)(*ext), halo, st.c)
+        self.desc = arr
+        self.n = len(grids)
+        self.timesteps = timesteps
+        h = ctypes.c_void_p()
+        check(lib().uhs_batch_create(arr, len(grids), timesteps, ctypes.byref(cfg) if cfg else None,
+                                     ctypes.byref(h)))
+        self.h = h
+
+    def run(self, stream=None):
+        check(lib().uhs_batch_run(self.h, stream))
+
+    def info(self):
+        nl, wg = ctypes.c_int(), ctypes.c_int()
+        nt, lds = ctypes.c_longlong(), ctypes.c_size_t()
+        check(lib().uhs_batch_info(self.h, ctypes.byref(nl), ctypes.byref(nt), ctypes.byref(wg), ctypes.byref(lds)))
+        return {"launches": nl.value, "tasks": nt.value, "workgroups": wg.value, "lds": lds.value}
+
+    def run_each(self, stream=None):
+        """The unbatched baseline on the same grids."""
+        check(lib().uhs_sweep_each_f64(self.desc, self.n, self.timesteps, stream))
+
+    def close(self):
+        if self.h:
+            lib().uhs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_default_config():
+    c = BatchConfig()
+    check(lib().uhs_batch_default_config(ctypes.byref(c)))
+    return c

@@ -138,22 +138,45 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
     if (uhs_config_blocks(a, c) > INT_MAX) return bad("too many workgroups");
     LaunchFn fn = nullptr;
-    if (pl.family == FAM_STAR3D) fn = uhs_find_star3d(pl.prog, a.dtype, c.vec);
+    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2)) return bad("unknown config flags");
+    if ((c.flags & UHS_CFG_PREFETCH2) && pl.family != FAM_STAR3D)
+        return bad("two-plane prefetch exists for 3D star stencils only");
+    if (pl.family == FAM_STAR3D)
+        fn = uhs_find_star3d(pl.prog, a.dtype, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
     if (pl.family == FAM_BOX3D) fn = uhs_find_box3d(pl.prog, a.dtype, c.vec);
     if (pl.family == FAM_WIN2D) fn = uhs_find_win2d(pl.prog, a.dtype, c.vec);
     if (!fn) return bad(std::string("no ") + family_name(pl.family) + str(" kernel compiled for vec %lld", c.vec));
     return UHS_OK;
 }
 
+// Defaults per built-in stencil: the fastest configuration of a timed sweep
+// over the configuration space on MI355X at 512^3 / 8192^2 (FP64).
+static const uhs_config kBuiltinDefaults[12] = {
+    {512, 1, 32, 1, 0},   // 2d1r_star
+    {1024, 1, 128, 2, 0}, // 2d2r_star
+    {256, 1, 32, 1, 0},   // 2d3r_star
+    {512, 1, 32, 1, 0},   // 2d1r_box
+    {1024, 1, 128, 2, 0}, // 2d2r_box
+    {64, 1, 256, 1, 0},   // 2d3r_box
+    {128, 16, 128, 4, 0}, // 3d1r_star
+    {128, 16, 64, 4, 0},  // 3d2r_star
+    {64, 16, 128, 2, 0},  // 3d3r_star
+    {32, 16, 64, 2, 0},   // 3d1r_box
+    {32, 16, 128, 1, 0},  // 3d2r_box
+    {64, 4, 128, 1, 0},   // 3d3r_box
+};
+
 uhs_config uhs_heuristic_config(const Plan &pl)
 {
     uhs_config c{};
     const SweepArgs &a = pl.args;
-    switch (pl.family) {
-    case FAM_STAR3D: c = {64, 8, 64, 2}; break;
-    case FAM_BOX3D: c = {64, 4, 128, 1}; break;
-    case FAM_WIN2D: c = {256, 1, 64, 1}; break;
-    default: c = {0, 0, 0, 0}; return c;
+    if (pl.family == FAM_GATHER) return c;
+    c = kBuiltinDefaults[pl.prog];
+    char why[128];
+    if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) {
+        // e.g. an FP32 form whose vec variant is not compiled
+        c.vec = 1;
+        c.flags = 0;
     }
     // Small grids: shorten the streamed slab until the launch has enough
     // workgroups to cover 256 CUs a few times over.
@@ -223,12 +246,14 @@ int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
     }
     char why[256] = "";
     if (uhs_check_config(*pl, c, why, sizeof why) != UHS_OK)
-        return fail(UHS_E_CONFIG, "config {%d,%d,%d,%d} rejected: %s", c.tile_x, c.tile_y,
-                    c.tile_z, c.vec, why);
-    a.tx = c.tile_x; a.ty = c.tile_y; a.tz = c.tile_z; a.vec = c.vec;
+        return fail(UHS_E_CONFIG, "config {%d,%d,%d,%d,%d} rejected: %s", c.tile_x, c.tile_y,
+                    c.tile_z, c.vec, c.flags, why);
+    a.tx = c.tile_x; a.ty = c.tile_y; a.tz = c.tile_z; a.vec = c.vec; a.flags = c.flags;
     pl->cfg = c;
     switch (pl->family) {
-    case FAM_STAR3D: pl->fn = uhs_find_star3d(pl->prog, dtype, c.vec); break;
+    case FAM_STAR3D:
+        pl->fn = uhs_find_star3d(pl->prog, dtype, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
+        break;
     case FAM_BOX3D: pl->fn = uhs_find_box3d(pl->prog, dtype, c.vec); break;
     case FAM_WIN2D: pl->fn = uhs_find_win2d(pl->prog, dtype, c.vec); break;
     default: pl->fn = uhs_find_gather(dtype); break;

@@ -1,0 +1,341 @@
+// Batched sweep of many independent 3D FP64 grids (uneven extents, mixed
+// stencils) as one hipGraph: a counter-reset node, then one persistent
+// work-queue launch per time block.  Star-stencil grids advance `depth`
+// steps per launch (temporal blocking); box-stencil grids one step.
+#include "batch_kernels.h"
+#include "uhs_host.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+using MixFn = void (*)(BLaunch);
+
+// The benchmark mix (7-point star + 125-point box) gets its own kernel so
+// its register footprint is the larger of the two, not of all six.
+template <int K> MixFn mix_kernel() { return k_batch<K, Prog_3d1r_star, Prog_3d2r_box>; }
+template <int K> MixFn all_kernel()
+{
+    return k_batch<K, Prog_3d1r_star, Prog_3d2r_star, Prog_3d3r_star, Prog_3d1r_box, Prog_3d2r_box,
+                   Prog_3d3r_box>;
+}
+
+MixFn pick_kernel(bool mix, int K)
+{
+    switch (K) {
+    case 1: return mix ? mix_kernel<1>() : all_kernel<1>();
+    case 2: return mix ? mix_kernel<2>() : all_kernel<2>();
+    case 4: return mix ? mix_kernel<4>() : all_kernel<4>();
+    }
+    return nullptr;
+}
+
+struct LaunchRec {
+    int depth;
+    size_t first, count; // slice of the task array
+    size_t lds;
+    int workgroups;
+    MixFn fn;
+};
+
+std::string g_batch_err;
+
+int berr(int code, const std::string &m)
+{
+    return uhs_set_error(code, m.c_str());
+}
+
+} // namespace
+
+struct uhs_batch {
+    BGrid *d_grids = nullptr;
+    BTask *d_tasks = nullptr;
+    unsigned *d_counters = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<LaunchRec> launches;
+    long long ntasks = 0;
+    int timesteps = 0;
+};
+
+static std::vector<int> star_schedule(int T, int K)
+{
+    std::vector<int> d;
+    if (K <= 1) return std::vector<int>(T, 1);
+    for (int i = 0; i < T / K; i++) d.push_back(K);
+    for (int i = 0; i < T % K; i++) d.push_back(1);
+    // The result must end in A for even T and in B for odd T: the number of
+    // launches (each reads one buffer and writes the other) must have T's
+    // parity.  Splitting one K-block into K single steps flips it (K even).
+    if ((int)(d.size() % 2) != T % 2 && !d.empty() && d[0] == K) {
+        d.erase(d.begin());
+        d.insert(d.begin(), K, 1);
+    }
+    return d;
+}
+
+extern "C" {
+
+int uhs_batch_default_config(uhs_batch_config *c)
+{
+    if (!c) return berr(UHS_E_INVAL, "null config");
+    *c = uhs_batch_config{2, 64, 8, 64, 64, 4, 512};
+    return UHS_OK;
+}
+
+int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs_batch_config *cfg_in,
+                     uhs_batch_t *out)
+{
+    if (!grids || n < 1 || !out) return berr(UHS_E_INVAL, "batch needs grids and an output handle");
+    if (timesteps < 1) return berr(UHS_E_INVAL, "batch needs timesteps >= 1");
+    uhs_batch_config cfg;
+    if (cfg_in) cfg = *cfg_in;
+    else uhs_batch_default_config(&cfg);
+    const int K = cfg.star_depth;
+    if (K != 1 && K != 2 && K != 4) return berr(UHS_E_CONFIG, "star_depth must be 1, 2 or 4");
+    if (cfg.box_tile_x * cfg.box_tile_y != kBatchThreads || cfg.box_tile_x < 1)
+        return berr(UHS_E_CONFIG, "box tiles must hold exactly 256 points (tile_x * tile_y)");
+    if (cfg.star_tile_x < 1 || cfg.star_tile_y < 1 || cfg.star_tile_z < 1 || cfg.box_tile_z < 1)
+        return berr(UHS_E_CONFIG, "tile sizes must be positive");
+
+    int dev = 0, cus = 0, max_lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return berr(UHS_E_HIP, "cannot query the device");
+
+    // Per-grid records and validation.
+    std::vector<BGrid> hg(n);
+    bool mix = true;
+    for (int i = 0; i < n; i++) {
+        const uhs_grid_desc &d = grids[i];
+        char msg[160];
+        if (d.dims != 3 || d.stencil.dims != 3 || !d.A || !d.B || d.A == d.B) {
+            snprintf(msg, sizeof msg, "grid %d: the batch takes 3D FP64 grids with distinct A and B", i);
+            return berr(UHS_E_INVAL, msg);
+        }
+        const int prog = uhs_match_builtin(3, d.stencil.npoints, d.stencil.offsets);
+        if (prog < 0 || !d.stencil.coeffs) {
+            snprintf(msg, sizeof msg, "grid %d: the batch runs the six built-in 3D stencils only", i);
+            return berr(UHS_E_UNSUPPORTED, msg);
+        }
+        const BuiltinStencil &b = kBuiltins[prog];
+        if (d.halo < b.radius) {
+            snprintf(msg, sizeof msg, "grid %d: halo %d smaller than radius %d", i, d.halo, b.radius);
+            return berr(UHS_E_INVAL, msg);
+        }
+        for (int k = 0; k < 3; k++)
+            if (d.ext[k] < 1 || d.ext[k] + 2LL * d.halo > (1 << 20)) {
+                snprintf(msg, sizeof msg, "grid %d: extent out of range", i);
+                return berr(UHS_E_INVAL, msg);
+            }
+        BGrid &g = hg[i];
+        memset(&g, 0, sizeof g);
+        g.n0 = (int)d.ext[0]; g.n1 = (int)d.ext[1]; g.n2 = (int)d.ext[2];
+        g.halo = d.halo;
+        g.sy = (int)(d.ext[2] + 2 * d.halo);
+        g.sz = (long long)g.sy * (d.ext[1] + 2 * d.halo);
+        if (g.sz > (1LL << 31)) return berr(UHS_E_INVAL, "plane too large");
+        g.A = d.A; g.B = d.B;
+        g.kind = prog;
+        if (b.star) {
+            g.TX = cfg.star_tile_x; g.TY = cfg.star_tile_y;
+            for (int t = 0; t < b.npoints; t++) g.coef[t] = d.stencil.coeffs[t];
+        } else {
+            g.TX = cfg.box_tile_x; g.TY = cfg.box_tile_y;
+            // box3d reads the coefficients cell-major (box_pack_index).
+            for (int t = 0; t < b.npoints; t++) {
+                int idx = t;
+                switch (prog) {
+                case Prog_3d1r_box::ID: idx = box_pack_index<Prog_3d1r_box>(t); break;
+                case Prog_3d2r_box::ID: idx = box_pack_index<Prog_3d2r_box>(t); break;
+                case Prog_3d3r_box::ID: idx = box_pack_index<Prog_3d3r_box>(t); break;
+                }
+                g.coef[idx] = d.stencil.coeffs[t];
+            }
+            if ((long long)(g.TX + 2 * b.radius) * (g.TY + 2 * b.radius) > kBoxCellsPerThread * kBatchThreads)
+                return berr(UHS_E_CONFIG, "box tile plus halo exceeds the per-thread staging");
+        }
+        if (prog != Prog_3d1r_star::ID && prog != Prog_3d2r_box::ID) mix = false;
+    }
+
+    // Schedules and tasks.
+    const std::vector<int> ssched = star_schedule(timesteps, K);
+    const int L = std::max<int>((int)ssched.size(), timesteps);
+    std::vector<std::vector<BTask>> per(L);
+    std::vector<std::vector<double>> cost(L);
+    for (int i = 0; i < n; i++) {
+        const BGrid &g = hg[i];
+        const BuiltinStencil &b = kBuiltins[g.kind];
+        const int tz = b.star ? cfg.star_tile_z : cfg.box_tile_z;
+        const long long origin = (long long)g.halo * g.sz + (long long)g.halo * g.sy + g.halo;
+        int k0 = 0;
+        const int nl = b.star ? (int)ssched.size() : timesteps;
+        for (int l = 0; l < nl; l++) {
+            const int depth = b.star ? ssched[l] : 1;
+            const double *src = (l & 1) ? g.B : g.A;
+            const double *dst = (l & 1) ? g.A : g.B;
+            for (int z = 0; z < g.n0; z += tz)
+                for (int y = 0; y < g.n1; y += g.TY)
+                    for (int x = 0; x < g.n2; x += g.TX) {
+                        BTask t{};
+                        t.src = src + origin;
+                        t.dst = const_cast<double *>(dst) + origin;
+                        t.grid = i;
+                        t.x0 = x; t.y0 = y; t.zs = z; t.ze = std::min(z + tz, g.n0);
+                        t.depth = depth;
+                        t.k0 = k0;
+                        per[l].push_back(t);
+                        const double pts = (double)std::min(g.TX, g.n2 - x) * std::min(g.TY, g.n1 - y) *
+                                           (t.ze - t.zs);
+                        cost[l].push_back(pts * (b.star ? 1.0 : b.npoints / 8.0) * depth);
+                    }
+            k0 += depth;
+        }
+    }
+
+    auto *B = new uhs_batch();
+    B->timesteps = timesteps;
+    std::vector<BTask> all;
+    for (int l = 0; l < L; l++) {
+        // Longest tasks first, so the tail of the launch is made of small ones.
+        std::vector<size_t> ord(per[l].size());
+        for (size_t k = 0; k < ord.size(); k++) ord[k] = k;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[l][a] > cost[l][b]; });
+        LaunchRec r{};
+        r.depth = l < (int)ssched.size() ? ssched[l] : 1;
+        r.first = all.size();
+        r.count = ord.size();
+        size_t lds = 0;
+        for (size_t k : ord) {
+            const BTask &t = per[l][k];
+            const BGrid &g = hg[t.grid];
+            const BuiltinStencil &b = kBuiltins[g.kind];
+            const int R = b.radius;
+            size_t need;
+            if (b.star) {
+                const size_t wc = (size_t)(g.TX + 2 * t.depth * R) * (g.TY + 2 * t.depth * R);
+                need = sizeof(double) * t.depth * (2 * R + 1) * wc;
+            } else {
+                need = 2 * sizeof(double) * (size_t)(g.TX + 2 * R) * (g.TY + 2 * R);
+            }
+            lds = std::max(lds, need);
+            all.push_back(t);
+        }
+        if (lds > (size_t)max_lds) {
+            delete B;
+            return berr(UHS_E_CONFIG, "tile and depth need more LDS than a workgroup may use");
+        }
+        r.lds = lds;
+        r.fn = pick_kernel(mix, r.depth);
+        B->launches.push_back(r);
+    }
+    B->ntasks = (long long)all.size();
+
+    auto fail_hip = [&](hipError_t e, const char *what) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        uhs_batch_destroy(B);
+        return berr(UHS_E_HIP, m);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&B->d_grids, sizeof(BGrid) * n)) != hipSuccess) return fail_hip(e, "grid table");
+    if ((e = hipMalloc(&B->d_tasks, sizeof(BTask) * std::max<size_t>(1, all.size()))) != hipSuccess)
+        return fail_hip(e, "task table");
+    if ((e = hipMalloc(&B->d_counters, sizeof(unsigned) * L)) != hipSuccess) return fail_hip(e, "counters");
+    if ((e = hipMemcpy(B->d_grids, hg.data(), sizeof(BGrid) * n, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail_hip(e, "grid upload");
+    if (!all.empty() &&
+        (e = hipMemcpy(B->d_tasks, all.data(), sizeof(BTask) * all.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail_hip(e, "task upload");
+
+    for (auto &r : B->launches) {
+        if (r.lds > 64 * 1024 &&
+            (e = hipFuncSetAttribute((const void *)r.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)r.lds)) != hipSuccess)
+            return fail_hip(e, "LDS limit");
+        int per_cu = 0;
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)r.fn, kBatchThreads,
+                                                              r.lds)) != hipSuccess)
+            return fail_hip(e, "occupancy");
+        const long long wg = (long long)std::max(1, per_cu) * cus;
+        r.workgroups = (int)std::max<long long>(1, std::min<long long>(wg, (long long)r.count));
+    }
+
+    // Capture: zero the counters with a kernel (not a memset node), then the launches.
+    hipStream_t cs;
+    if ((e = hipStreamCreate(&cs)) != hipSuccess) return fail_hip(e, "capture stream");
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_zero_u32, dim3((L + 255) / 256), dim3(256), 0, cs, B->d_counters, L);
+        for (int l = 0; l < L; l++) {
+            const LaunchRec &r = B->launches[l];
+            if (!r.count) continue;
+            BLaunch bl{B->d_tasks + r.first, B->d_grids, B->d_counters + l, (int)r.count};
+            hipLaunchKernelGGL(r.fn, dim3(r.workgroups), dim3(kBatchThreads), r.lds, cs, bl);
+        }
+        hipError_t e2 = hipStreamEndCapture(cs, &B->graph);
+        if (e == hipSuccess) e = e2;
+    }
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return fail_hip(e, "graph capture");
+    if ((e = hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return fail_hip(e, "graph instantiate");
+    *out = B;
+    return UHS_OK;
+}
+
+int uhs_batch_run(uhs_batch_t b, uhs_stream_t stream)
+{
+    if (!b || !b->exec) return berr(UHS_E_INVAL, "null batch");
+    hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return berr(UHS_E_HIP, std::string("graph launch: ") + hipGetErrorString(e));
+    return UHS_OK;
+}
+
+int uhs_batch_destroy(uhs_batch_t b)
+{
+    if (!b) return UHS_OK;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_grids) (void)hipFree(b->d_grids);
+    if (b->d_tasks) (void)hipFree(b->d_tasks);
+    if (b->d_counters) (void)hipFree(b->d_counters);
+    delete b;
+    return UHS_OK;
+}
+
+int uhs_batch_info(uhs_batch_t b, int *launches, long long *tasks, int *max_workgroups,
+                   size_t *max_lds)
+{
+    if (!b) return berr(UHS_E_INVAL, "null batch");
+    int wg = 0;
+    size_t lds = 0, nl = 0;
+    for (auto &r : b->launches) {
+        if (!r.count) continue;
+        nl++;
+        wg = std::max(wg, r.workgroups);
+        lds = std::max(lds, r.lds);
+    }
+    if (launches) *launches = (int)nl;
+    if (tasks) *tasks = b->ntasks;
+    if (max_workgroups) *max_workgroups = wg;
+    if (max_lds) *max_lds = lds;
+    return UHS_OK;
+}
+
+int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps, uhs_stream_t stream)
+{
+    if (!grids || n < 1) return berr(UHS_E_INVAL, "no grids");
+    for (int i = 0; i < n; i++) {
+        const uhs_grid_desc &d = grids[i];
+        int rc = uhs_sweep_f64(d.A, d.B, d.dims, d.ext, d.halo, &d.stencil, timesteps, nullptr, stream);
+        if (rc) return rc;
+    }
+    return UHS_OK;
+}
+
+} // extern "C"

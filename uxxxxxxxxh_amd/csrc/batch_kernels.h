@@ -1,0 +1,173 @@
+// Device side of the batched path: many independent 3D FP64 grids, mixed
+// stencils, one persistent launch per time block, workgroups pulling tiles
+// from a device-side queue.
+//
+// star_tb_tile advances a tile of a star stencil by K timesteps at once
+// (temporal blocking with ghost zones).  The workgroup streams planes along
+// axis 0; level 0 is the input plane read from HBM, level j (1..K) is the
+// state after j steps, computed one plane behind level j-1 (R planes) on a
+// region that shrinks by R per level.  Levels 0..K-1 live in LDS rings of
+// 2R+1 planes; level K is written to the destination grid.  Every update is
+// the same product-then-fma chain in term order as the one-step kernels, so
+// the result is bitwise identical to K single sweeps.  Cells of a level that
+// lie outside the interior hold that level's halo, i.e. the halo of the
+// buffer the one-step ping-pong would hold it in (halos never change).
+#pragma once
+#include "uhs.h"
+#include "sweep_kernels.h"
+
+constexpr int kBatchThreads = 256;
+
+// Per-grid record, in device memory.
+struct BGrid {
+    const double *A;
+    const double *B;      // halo-inclusive grids (interior origins below)
+    int n0, n1, n2, halo;
+    int sy;
+    long long sz;
+    int kind;             // built-in stencil id (kBuiltins index)
+    int TX, TY;           // tile
+    int pad[2];
+    double coef[UHS_MAX_POINTS]; // term order (stars) or box_pack_index order (boxes)
+};
+
+// One unit of work: a tile column and slab of one grid, advanced by `depth`
+// steps from global step k0 (src/dst are the interior origins).
+struct BTask {
+    const double *src;
+    double *dst;
+    int grid;
+    int x0, y0, zs, ze;
+    int depth;
+    int k0;
+    int pad;
+};
+
+struct BLaunch {
+    const BTask *tasks;
+    const BGrid *grids;
+    unsigned *counter;
+    int ntasks;
+};
+
+__device__ __forceinline__ int ring_slot(int p, int q)
+{
+    const int r = p % q;
+    return r < 0 ? r + q : r;
+}
+
+template <class S, int K, class CF>
+__device__ __forceinline__ void star_tb_tile(const BGrid &g, const BTask &t, double *lds, CF &cf)
+{
+    constexpr int R = S::R, Q = 2 * R + 1, P = S::P, M = K * R;
+    const int NT = blockDim.x, tid = threadIdx.x;
+    const int TX = g.TX, TY = g.TY;
+    const int WX = TX + 2 * M, WY = TY + 2 * M, WC = WX * WY;
+    const int ox = t.x0 - M, oy = t.y0 - M; // interior coordinates of LDS cell (0, 0)
+    const int h = g.halo, n0 = g.n0, n1 = g.n1, n2 = g.n2, sy = g.sy;
+    const long long sz = g.sz;
+    const double *__restrict__ src = t.src;
+    double *__restrict__ dst = t.dst;
+    const long long origin = (long long)h * sz + (long long)h * sy + h;
+    const double *hA = g.A + origin, *hB = g.B + origin;
+    const int zs = t.zs, ze = t.ze;
+    const int steps = (ze - zs) + 2 * M;
+    for (int s = 0; s < steps; ++s) {
+        const int z0 = zs - M + s;
+        {
+            double *ring = lds + ring_slot(z0, Q) * WC;
+            const double *pz = src + (long long)clampi(z0, -h, n0 + h - 1) * sz;
+            for (int c = tid; c < WC; c += NT) {
+                const int ly = c / WX, lx = c - ly * WX;
+                ring[c] = pz[clampi(oy + ly, -h, n1 + h - 1) * sy + clampi(ox + lx, -h, n2 + h - 1)];
+            }
+        }
+        __syncthreads();
+        cf.refresh();
+        static_for<K>([&](auto jc) {
+            constexpr int j = decltype(jc)::value + 1;
+            const int zj = z0 - j * R;
+            const int m = (K - j) * R;
+            if (zj >= zs - m && zj <= ze - 1 + m) {
+                const double *prev = lds + (j - 1) * Q * WC;
+                const double *pp[Q];
+#pragma unroll
+                for (int d = 0; d < Q; d++) pp[d] = prev + ring_slot(zj - R + d, Q) * WC;
+                double *cur = lds + (j < K ? j : 0) * Q * WC + ring_slot(zj, Q) * WC;
+                // Level j is the state after global step k0 + j: its halo is
+                // the halo of B when that step is odd, of A when even.
+                const double *hbuf = ((t.k0 + j) & 1) ? hB : hA;
+                const bool zin = zj >= 0 && zj < n0;
+                const int RW = TX + 2 * m, RH = TY + 2 * m;
+                double *oz = dst + (long long)zj * sz;
+                for (int c = tid; c < RW * RH; c += NT) {
+                    const int ry = c / RW, rx = c - ry * RW;
+                    const int ly = M - m + ry, lx = M - m + rx;
+                    const int gy = oy + ly, gx = ox + lx;
+                    const bool inside = zin && gy >= 0 && gy < n1 && gx >= 0 && gx < n2;
+                    const int li = ly * WX + lx;
+                    if (inside) {
+                        Acc<double, false> a;
+                        static_for<P>([&](auto tc) {
+                            constexpr int tt = decltype(tc)::value;
+                            constexpr int dz = S::OFF[tt][0], dy = S::OFF[tt][1], dx = S::OFF[tt][2];
+                            const double v = pp[R + dz][li + dy * WX + dx];
+                            if constexpr (tt == 0) a.first(cf.template get<tt>(), v);
+                            else a.add(cf.template get<tt>(), v);
+                        });
+                        if constexpr (j < K) cur[li] = a.value();
+                        else oz[gy * sy + gx] = a.value();
+                    } else if constexpr (j < K) {
+                        cur[li] = hbuf[(long long)clampi(zj, -h, n0 + h - 1) * sz +
+                                       clampi(gy, -h, n1 + h - 1) * sy + clampi(gx, -h, n2 + h - 1)];
+                    }
+                }
+            }
+            __syncthreads();
+        });
+    }
+}
+
+// Runs task `t` of grid `g` with program S (star: depth K; box: one step).
+template <class S, int K>
+__device__ __forceinline__ void run_task(const BGrid &g, const BTask &t, double *lds)
+{
+    typedef const __attribute__((address_space(4))) double *kptr;
+    Coefs<double, S::P> cf((kptr)g.coef);
+    if constexpr (S::STAR) {
+        star_tb_tile<S, K>(g, t, lds, cf);
+    } else {
+        const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, g.TX, false};
+        box3d_tile<double, S, 1, false>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+    }
+}
+
+// The persistent kernel: each workgroup takes the next task index from the
+// launch's counter until the list is exhausted (every workgroup reaches the
+// exit).  Progs lists the stencils this instantiation can run; a grid's kind
+// selects one.
+template <int K, class... Progs>
+__global__ void __launch_bounds__(kBatchThreads) k_batch(BLaunch L)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *lds = reinterpret_cast<double *>(smem_raw);
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(L.counter, 1u);
+        __syncthreads();
+        const int i = next;
+        __syncthreads();
+        if (i >= L.ntasks) break;
+        const BTask t = L.tasks[i];
+        const BGrid &g = L.grids[t.grid];
+        const int kind = g.kind;
+        ((kind == Progs::ID ? (run_task<Progs, K>(g, t, lds), 0) : 0), ...);
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(unsigned *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}

@@ -35,7 +35,13 @@
 // --------------------------------------------------------------------------
 // star3d
 // --------------------------------------------------------------------------
-template <typename T, class S, int VX, bool KAHAN>
+template <typename T> __device__ __forceinline__ void store_out(T *p, T v, bool nt)
+{
+    if (nt) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <typename T, class S, int VX, bool KAHAN, int PF>
 __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
 {
     constexpr int R = S::R, P = S::P, Q = 2 * R + 1, NH = kStarHaloPerThread;
@@ -95,13 +101,17 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
     }
 
     Coefs<T, P> cf(p);
-    // Register queue: q[v][j] = plane z-R+j at this thread's point v.
-    T q[VX][Q], nq[VX], nh[NH];
+    // Register queue: q[v][j] = plane z-R+j at this thread's point v; nq[f]
+    // holds plane z+R+f, in flight (PF planes of prefetch).
+    T q[VX][Q], nq[PF][VX], nh[NH];
+    const bool nts = p.flags & UHS_CFG_NT_STORE;
 #pragma unroll
     for (int v = 0; v < VX; v++) {
 #pragma unroll
         for (int j = 0; j < 2 * R; j++) q[v][j] = in[(long long)(zs - R + j) * sz + coff[v]];
-        nq[v] = in[(long long)(zs + R) * sz + coff[v]];
+#pragma unroll
+        for (int f = 0; f < PF; f++)
+            nq[f][v] = in[(long long)min(zs + R + f, p.n0 + R - 1) * sz + coff[v]];
     }
 #pragma unroll
     for (int k = 0; k < NH; k++)
@@ -111,18 +121,22 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
         T *L = lds + (z & 1) * LSZ;
 #pragma unroll
         for (int v = 0; v < VX; v++) {
-            q[v][2 * R] = nq[v];
+            q[v][2 * R] = nq[0][v];
             L[lc[v]] = q[v][R];
         }
 #pragma unroll
         for (int k = 0; k < NH; k++)
             if (hl[k] >= 0) L[hl[k]] = nh[k];
-        // Prefetch plane z+1+R (own column) and the halo of plane z+1.
+        // Prefetch plane z+R+PF (own column) and the halo of plane z+1.
         {
-            const T *pn = in + (long long)min(z + 1 + R, p.n0 + R - 1) * sz;
+            const T *pn = in + (long long)min(z + R + PF, p.n0 + R - 1) * sz;
             const T *ph = in + (long long)(z + 1) * sz;
 #pragma unroll
-            for (int v = 0; v < VX; v++) nq[v] = pn[coff[v]];
+            for (int f = 0; f + 1 < PF; f++)
+#pragma unroll
+                for (int v = 0; v < VX; v++) nq[f][v] = nq[f + 1][v];
+#pragma unroll
+            for (int v = 0; v < VX; v++) nq[PF - 1][v] = pn[coff[v]];
 #pragma unroll
             for (int k = 0; k < NH; k++)
                 if (hl[k] >= 0) nh[k] = ph[hoff[k]];
@@ -146,7 +160,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
                 if constexpr (t == 0) a.first(cf.template get<t>(), val);
                 else a.add(cf.template get<t>(), val);
             });
-            if (act[v]) oz[coff[v]] = a.value();
+            if (act[v]) store_out(oz + coff[v], a.value(), nts);
         }
 #pragma unroll
         for (int v = 0; v < VX; v++)
@@ -199,28 +213,31 @@ template <class S> constexpr int box_pack_index(int t)
     }
 }
 
-template <typename T, class S, int VX, bool KAHAN>
-__global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
+// One box3d tile: the tile_x * tile_y column at (x0, y0), output planes
+// [zs, ze).  `in`/`out` point at the interior origin; CF supplies the
+// coefficients in box_pack_index order.  Shared by the per-grid kernel and
+// the batched work-queue kernel.
+struct TileGeom {
+    int n0, n1, n2;
+    int sy;
+    long long sz;
+    int TX, TY, BXT;
+    bool nts; // non-temporal output stores
+};
+
+template <typename T, class S, int VX, bool KAHAN, class CF>
+__device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restrict__ out,
+                                           const TileGeom &p, int x0, int y0, int zs, int ze,
+                                           T *lds, CF &cf)
 {
     static_assert(plane_major<S>(), "box3d needs terms in ascending axis-0 offset");
     constexpr int R = S::R, P = S::P, NL = kBoxCellsPerThread;
     constexpr int DZMIN = S::OFF[0][0], DZMAX = S::OFF[P - 1][0];
     constexpr int NJ = DZMAX - DZMIN + 1; // pending output planes per point
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-
     const int NT = blockDim.x, tid = threadIdx.x;
     const int BXT = p.BXT, TX = p.TX, TY = p.TY;
     const int LW = TX + 2 * R, LSZ = LW * (TY + 2 * R);
-    int bid = xcd_remap(blockIdx.x, p.nblocks);
-    const int tile_x = bid % p.tiles_x;
-    bid /= p.tiles_x;
-    const int tile_y = bid % p.tiles_y, chunk = bid / p.tiles_y;
-    const int x0 = tile_x * TX, y0 = tile_y * TY;
-    const int zs = chunk * p.ZC, ze = min(zs + p.ZC, p.n0);
     const int tx = tid % BXT, ty = tid / BXT;
-    const T *__restrict__ in = p.in;
-    T *__restrict__ out = p.out;
     const long long sz = p.sz;
     const int sy = p.sy;
 
@@ -253,7 +270,6 @@ __global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
         for (int k = 0; k < NL; k++)
             if (gv[k]) pl[k] = pp[goff[k]];
     }
-    Coefs<T, P> cf(p);
     // acc[v][j] is the running sum of output plane zp - DZMAX + j.
     Acc<T, KAHAN> acc[VX][NJ];
 #pragma unroll
@@ -325,11 +341,26 @@ __global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
                     else acc[v][j].add(cf.template get<t>(), val);
                 });
             }
-            if (zo >= zs && act[v]) oz[coff[v]] = acc[v][0].value();
+            if (zo >= zs && act[v]) store_out(oz + coff[v], acc[v][0].value(), p.nts);
 #pragma unroll
             for (int j = 0; j + 1 < NJ; j++) acc[v][j] = acc[v][j + 1];
         }
     }
+}
+
+template <typename T, class S, int VX, bool KAHAN>
+__global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int bid = xcd_remap(blockIdx.x, p.nblocks);
+    const int tile_x = bid % p.tiles_x;
+    bid /= p.tiles_x;
+    const int tile_y = bid % p.tiles_y, chunk = bid / p.tiles_y;
+    const int zs = chunk * p.ZC;
+    const TileGeom g{p.n0, p.n1, p.n2, p.sy, p.sz, p.TX, p.TY, p.BXT, (p.flags & UHS_CFG_NT_STORE) != 0};
+    Coefs<T, S::P> cf(p);
+    box3d_tile<T, S, VX, KAHAN>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs,
+                                min(zs + p.ZC, p.n0), reinterpret_cast<T *>(smem_raw), cf);
 }
 
 // --------------------------------------------------------------------------
@@ -351,6 +382,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
     const T *__restrict__ in = p.in;
     T *__restrict__ out = p.out;
     const long long sy = p.sy;
+    const bool nts = p.flags & UHS_CFG_NT_STORE;
 
     int xcl[VX], xg[VX];
     bool act[VX];
@@ -415,7 +447,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
                     if constexpr (t == 0) acc.first(cf.template get<t>(), val);
                     else acc.add(cf.template get<t>(), val);
                 });
-                if (act[v]) orow[xg[v]] = acc.value();
+                if (act[v]) store_out(orow + xg[v], acc.value(), nts);
             }
         }
     }
@@ -459,13 +491,13 @@ template <typename T> constexpr int dtype_of(bool kahan)
     return sizeof(T) == 8 ? DT_F64 : (kahan ? DT_F32K : DT_F32);
 }
 
-template <typename T, class S, int VX, bool KAHAN>
+template <typename T, class S, int VX, bool KAHAN, int PF>
 hipError_t launch_star3d(const SweepArgs &a, hipStream_t st)
 {
     auto p = make_params<T, S::P>(a);
     const int nt = (a.tx / VX) * a.ty;
     const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R) * (a.ty + 2 * S::R);
-    hipLaunchKernelGGL((k_star3d<T, S, VX, KAHAN>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    hipLaunchKernelGGL((k_star3d<T, S, VX, KAHAN, PF>), dim3(p.nblocks), dim3(nt), lds, st, p);
     return hipGetLastError();
 }
 

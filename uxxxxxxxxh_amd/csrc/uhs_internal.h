@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "uhs.h"
 #include <stddef.h>
 #include <utility>
 
@@ -26,6 +27,7 @@ struct SweepArgs {
     int halo;
     long long sy, sz;      // row and plane strides in elements
     int tx, ty, tz, vec;   // tiling configuration
+    int flags;             // UHS_CFG_* bits
     int npoints;
     const int *off;        // host copy of the offsets (gather path)
     const double *coef;    // host copy of the coefficients
@@ -35,7 +37,7 @@ struct SweepArgs {
 typedef hipError_t (*LaunchFn)(const SweepArgs &, hipStream_t);
 
 // Per-family kernel tables (nullptr when the combination is not compiled).
-LaunchFn uhs_find_star3d(int prog, int dtype, int vec);
+LaunchFn uhs_find_star3d(int prog, int dtype, int vec, int prefetch);
 LaunchFn uhs_find_box3d(int prog, int dtype, int vec);
 LaunchFn uhs_find_win2d(int prog, int dtype, int vec);
 LaunchFn uhs_find_gather(int dtype);
@@ -128,6 +130,7 @@ template <typename T, int P> struct KParams {
     long long sz;
     int TX, TY, ZC, BXT;
     int tiles_x, tiles_y, nblocks;
+    int flags;
     T c[P];
 };
 
@@ -155,6 +158,17 @@ template <typename T, int P> struct Coefs {
             ptr = (kptr)(base + __builtin_offsetof(KP, c));
         }
     }
+    // Coefficients stored in device memory (the batched path's per-grid
+    // table), read through the constant address space so the loads are scalar.
+    __device__ __forceinline__ explicit Coefs(kptr table)
+    {
+        if constexpr (P <= kHoistPoints) {
+            for (int t = 0; t < P; t++) hoisted[t] = table[t];
+            ptr = nullptr;
+        } else {
+            ptr = table;
+        }
+    }
     // Call once per step: hides the pointer's value from loop-invariant code
     // motion so the scalar loads are issued inside the step.
     __device__ __forceinline__ void refresh()
@@ -179,6 +193,7 @@ inline KParams<T, P> make_params(const SweepArgs &a)
     p.n0 = a.n0; p.n1 = a.n1; p.n2 = a.n2;
     p.sy = (int)a.sy; p.sz = a.sz;
     p.TX = a.tx; p.TY = a.ty; p.ZC = a.tz; p.BXT = a.tx / a.vec;
+    p.flags = a.flags;
     p.tiles_x = (a.n2 + a.tx - 1) / a.tx;
     p.tiles_y = a.dims == 3 ? (a.n1 + a.ty - 1) / a.ty : 1;
     const int chunks = (a.n0 + a.tz - 1) / a.tz;

@@ -1,0 +1,109 @@
+/*
+ * uhs_tune.h -- autotuning of the sweep's runtime tiling, and the learned
+ * cost model that prunes the search.
+ *
+ * The reference tunes by random search over compile-time tiling parameters,
+ * recompiling and profiling every point (scripts/evaluation_scripts_GPU/
+ * test_driver.py:36-99,101-164,169-285; collect.py:30-69).  Here the tiling is
+ * a runtime argument of one kernel family, so the tuner times configurations
+ * directly.  Timing runs sweeps on the caller's grids: their contents are
+ * overwritten.
+ */
+#ifndef UHS_TUNE_H
+#define UHS_TUNE_H
+#include <stddef.h>
+#include <stdint.h>
+#include "uhs.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct uhs_tune_result {
+    uhs_config best;
+    double best_ms;     /* per sweep */
+    int evaluated;      /* configurations timed or read from the database */
+    int failed;         /* configurations that could not run (skipped) */
+    int reused;         /* timings taken from the database instead of re-run */
+    double seconds;     /* wall time of the search */
+} uhs_tune_result;
+
+/* Exhaustive timed search over the legal configuration space of this
+ * problem.  Every timing is appended to the CSV database at db_path (NULL:
+ * none); timings already there for this device and problem are reused, so
+ * an interrupted search resumes.  The winner becomes the configuration
+ * uhs_sweep_* use for this problem when cfg is NULL (this process). */
+int uhs_tune_sweep(void *A, void *B, int dims, const int64_t *ext, int halo,
+                   const uhs_stencil *st, int dtype, int reps,
+                   const char *db_path, uhs_tune_result *out,
+                   uhs_stream_t stream);
+/* The legal configurations the search visits: returns their number and
+ * copies up to cap of them into out (out may be NULL). */
+int uhs_tune_space(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
+                   int dtype, uhs_config *out, int cap);
+/* Milliseconds per sweep of one configuration (one warm-up sweep, then reps). */
+int uhs_tune_time(void *A, void *B, int dims, const int64_t *ext, int halo,
+                  const uhs_stencil *st, int dtype, const uhs_config *cfg,
+                  int reps, double *ms, uhs_stream_t stream);
+/* Installs, for every built-in problem in the database measured on this
+ * device, its fastest configuration as the default.  Returns the count. */
+int uhs_tune_load_db(const char *db_path);
+int uhs_tune_clear(void);
+
+/* ---- learned cost model ---------------------------------------------------
+ * An MLP (UHS_MLP_INPUTS -> 128 -> 128 -> 128 -> 1, ReLU, FP32) mapping a
+ * problem and configuration to log(ms per sweep).  Forward, backward, Adam and
+ * inference run as HIP kernels on the GPU.  Inputs are standardised inside
+ * the model with the training set's mean and deviation. */
+#define UHS_MLP_INPUTS 20
+#define UHS_MLP_HIDDEN 128
+
+typedef struct uhs_mlp *uhs_mlp_t;
+
+/* Feature vector of (problem, configuration): dims one-hot (2), radius
+ * one-hot (3), star/box one-hot (2), log2 extents (3, axis 0 first, 0 for a
+ * missing axis), log2 tile x/y/z (3), log2 threads per workgroup (1),
+ * points per thread (1), temporal depth one-hot {1,2,4} (3), non-temporal
+ * store flag (1), two-plane prefetch flag (1). */
+int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext,
+                     const uhs_config *cfg, int depth, float *out20);
+
+int uhs_mlp_create(uint64_t seed, uhs_mlp_t *out);
+int uhs_mlp_destroy(uhs_mlp_t m);
+int uhs_mlp_num_params(void);
+/* Parameters as one FP32 vector: W1[20][128], b1[128], W2[128][128], b2,
+ * W3[128][128], b3, W4[128], b4 (row-major, input index first). */
+int uhs_mlp_get_params(uhs_mlp_t m, float *out, int n);
+int uhs_mlp_set_params(uhs_mlp_t m, const float *in, int n);
+/* Standardisation: x_std = (x - mean) / scale per input; target likewise. */
+int uhs_mlp_set_norm(uhs_mlp_t m, const float *mean20, const float *scale20,
+                     float tmean, float tscale);
+/* Predicted log(ms) for n raw feature rows (host arrays). */
+int uhs_mlp_predict(uhs_mlp_t m, const float *x, int n, float *y);
+/* Loss 0.5*mean((pred_std - t_std)^2) and its gradient w.r.t. the
+ * parameters for n rows (host arrays), computed by the backward kernels. */
+int uhs_mlp_grad(uhs_mlp_t m, const float *x, const float *t, int n,
+                 float *grad, float *loss);
+/* Adam (b1 0.9, b2 0.999, eps 1e-8) for `epochs` passes over (x, t) in
+ * shuffled minibatches, all on the GPU.  Sets the standardisation from the
+ * data first when fit_norm != 0.  Returns the last epoch's mean loss. */
+int uhs_mlp_train(uhs_mlp_t m, const float *x, const float *t, int n,
+                  int epochs, int batch, float lr, int fit_norm,
+                  uint64_t seed, float *final_loss);
+/* One Adam step with an explicit gradient (host array), step count t >= 1. */
+int uhs_mlp_adam_step(uhs_mlp_t m, const float *grad, float lr, int t);
+int uhs_mlp_save(uhs_mlp_t m, const char *path);
+int uhs_mlp_load(uhs_mlp_t m, const char *path);
+
+/* Model-pruned search: predict every legal configuration, time the topk
+ * predicted fastest, keep the best (becomes the default like
+ * uhs_tune_sweep).  res->seconds includes the inference. */
+int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *ext,
+                         int halo, const uhs_stencil *st, int dtype,
+                         uhs_mlp_t model, int topk, int reps,
+                         const char *db_path, uhs_tune_result *out,
+                         uhs_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UHS_TUNE_H */

@@ -1,0 +1,30 @@
+"""Pins the numpy MLP restatement (the checker of the GPU kernels) by
+central finite differences, and the feature encoding of the C ABI."""
+import numpy as np
+import pytest
+
+from tests import mlp_ref
+
+
+def test_numpy_backprop_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    n = sum(i * o + o for i, o in mlp_ref.SIZES)
+    p = rng.normal(0, 0.2, n)
+    xs = rng.normal(0, 1, (9, 20))
+    ts = rng.normal(0, 1, 9)
+    loss, g = mlp_ref.loss_grad(p, xs, ts)
+    for k in rng.choice(n, 40, replace=False):
+        e = np.zeros(n); e[k] = 1e-6
+        num = (mlp_ref.loss_grad(p + e, xs, ts)[0] - mlp_ref.loss_grad(p - e, xs, ts)[0]) / 2e-6
+        assert abs(num - g[k]) <= 1e-6 * max(1.0, abs(num)), k
+
+
+def test_feature_encoding(uhs):
+    f = uhs.mlp_features(3, 2, False, [64, 128, 256], (64, 8, 32, 2, 3), depth=4)
+    assert f.tolist() == [0, 1, 0, 1, 0, 0, 1, 6, 7, 8, 6, 3, 5, 8, 2, 0, 0, 1, 1, 1]
+    f = uhs.mlp_features(2, 1, True, [1024, 2048], (512, 1, 64, 1, 0))
+    assert f.tolist() == [1, 0, 1, 0, 0, 1, 0, 10, 11, 0, 9, 0, 6, 9, 1, 1, 0, 0, 0, 0]
+
+
+def test_parameter_count(uhs):
+    assert uhs.lib().uhs_mlp_num_params() == sum(i * o + o for i, o in mlp_ref.SIZES) == 35841

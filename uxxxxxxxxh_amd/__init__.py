@@ -68,6 +68,11 @@ class BatchConfig(ctypes.Structure):
             self.box_tile_x, self.box_tile_y, self.box_tile_z)
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("best", Config), ("best_ms", ctypes.c_double), ("evaluated", ctypes.c_int),
+                ("failed", ctypes.c_int), ("reused", ctypes.c_int), ("seconds", ctypes.c_double)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -95,6 +100,31 @@ _SIGS = {
                             ctypes.POINTER(ctypes.c_size_t)]),
     "uhs_batch_default_config": (_I, [ctypes.POINTER(BatchConfig)]),
     "uhs_sweep_each_f64": (_I, [ctypes.POINTER(GridDesc), _I, _I, _P]),
+    "uhs_tune_sweep": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, _I, ctypes.c_char_p,
+                            ctypes.POINTER(TuneResult), _P]),
+    "uhs_tune_sweep_model": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, _P, _I, _I, ctypes.c_char_p,
+                                  ctypes.POINTER(TuneResult), _P]),
+    "uhs_tune_space": (_I, [_I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), _I]),
+    "uhs_tune_time": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), _I,
+                           ctypes.POINTER(ctypes.c_double), _P]),
+    "uhs_tune_load_db": (_I, [ctypes.c_char_p]),
+    "uhs_tune_clear": (_I, []),
+    "uhs_mlp_features": (_I, [_I, _I, _I, _I64P, ctypes.POINTER(Config), _I, ctypes.POINTER(ctypes.c_float)]),
+    "uhs_mlp_create": (_I, [ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "uhs_mlp_destroy": (_I, [_P]),
+    "uhs_mlp_num_params": (_I, []),
+    "uhs_mlp_get_params": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
+    "uhs_mlp_set_params": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
+    "uhs_mlp_set_norm": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                              ctypes.c_float, ctypes.c_float]),
+    "uhs_mlp_predict": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(ctypes.c_float)]),
+    "uhs_mlp_grad": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _I,
+                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "uhs_mlp_train": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _I, _I, _I,
+                           ctypes.c_float, _I, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]),
+    "uhs_mlp_adam_step": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_float, _I]),
+    "uhs_mlp_save": (_I, [_P, ctypes.c_char_p]),
+    "uhs_mlp_load": (_I, [_P, ctypes.c_char_p]),
     "uhs_last_error": (ctypes.c_char_p, []),
     "uhs_version": (ctypes.c_char_p, []),
     "uhs_device_count": (_I, []),
@@ -372,3 +402,110 @@ def batch_default_config():
     c = BatchConfig()
     check(lib().uhs_batch_default_config(ctypes.byref(c)))
     return c
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def tune_space(ext, halo, stencil, dtype=DT_F64):
+    L = lib()
+    n = check(L.uhs_tune_space(len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype, None, 0))
+    arr = (Config * max(1, n))()
+    check(L.uhs_tune_space(len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype, arr, n))
+    return [Config(*arr[i].tuple()) for i in range(n)]
+
+
+def tune_sweep(A, B, ext, halo, stencil, dtype=DT_F64, reps=3, db=None, model=None, topk=5, stream=None):
+    r = TuneResult()
+    dbp = db.encode() if db else None
+    if model is None:
+        check(lib().uhs_tune_sweep(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,
+                                   reps, dbp, ctypes.byref(r), stream))
+    else:
+        check(lib().uhs_tune_sweep_model(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
+                                         dtype, model.h, topk, reps, dbp, ctypes.byref(r), stream))
+    return r
+
+
+def tune_time(A, B, ext, halo, stencil, cfg, dtype=DT_F64, reps=3, stream=None):
+    ms = ctypes.c_double()
+    c = cfg if isinstance(cfg, Config) else Config(*cfg)
+    check(lib().uhs_tune_time(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,
+                              ctypes.byref(c), reps, ctypes.byref(ms), stream))
+    return ms.value
+
+
+def mlp_features(dims, radius, star, ext, cfg, depth=1):
+    out = np.zeros(20, np.float32)
+    c = cfg if isinstance(cfg, Config) else Config(*cfg)
+    e = list(ext) + [0] * (3 - len(ext))
+    check(lib().uhs_mlp_features(dims, radius, 1 if star else 0, ext_array(e), ctypes.byref(c), depth, _fp(out)))
+    return out
+
+
+class MLP:
+    """The cost model (uhs_mlp_*): parameters and training live on the GPU."""
+
+    def __init__(self, seed=1):
+        h = ctypes.c_void_p()
+        check(lib().uhs_mlp_create(seed, ctypes.byref(h)))
+        self.h = h
+        self.nparam = lib().uhs_mlp_num_params()
+
+    def params(self):
+        out = np.zeros(self.nparam, np.float32)
+        check(lib().uhs_mlp_get_params(self.h, _fp(out), self.nparam))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(lib().uhs_mlp_set_params(self.h, _fp(p), self.nparam))
+
+    def set_norm(self, mean, scale, tmean, tscale):
+        m = np.ascontiguousarray(mean, np.float32)
+        s = np.ascontiguousarray(scale, np.float32)
+        check(lib().uhs_mlp_set_norm(self.h, _fp(m), _fp(s), tmean, tscale))
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(len(x), np.float32)
+        check(lib().uhs_mlp_predict(self.h, _fp(x), len(x), _fp(y)))
+        return y
+
+    def grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.nparam, np.float32)
+        loss = ctypes.c_float()
+        check(lib().uhs_mlp_grad(self.h, _fp(x), _fp(t), len(x), _fp(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def adam_step(self, g, lr, t):
+        g = np.ascontiguousarray(g, np.float32)
+        check(lib().uhs_mlp_adam_step(self.h, _fp(g), lr, t))
+
+    def train(self, x, t, epochs=100, batch=256, lr=1e-3, fit_norm=True, seed=1):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = ctypes.c_float()
+        check(lib().uhs_mlp_train(self.h, _fp(x), _fp(t), len(x), epochs, batch, lr, 1 if fit_norm else 0, seed,
+                                  ctypes.byref(loss)))
+        return loss.value
+
+    def save(self, path):
+        check(lib().uhs_mlp_save(self.h, path.encode()))
+
+    def load(self, path):
+        check(lib().uhs_mlp_load(self.h, path.encode()))
+
+    def close(self):
+        if self.h:
+            lib().uhs_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

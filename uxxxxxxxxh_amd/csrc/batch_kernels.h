@@ -75,11 +75,21 @@ __device__ __forceinline__ void star_tb_tile(const BGrid &g, const BTask &t, dou
     for (int s = 0; s < steps; ++s) {
         const int z0 = zs - M + s;
         {
+            // Level 0 is the state after global step k0: interior cells come
+            // from src, which holds it; halo cells from the buffer the
+            // one-step ping-pong would hold it in (src itself unless earlier
+            // launches advanced an even number of steps per buffer swap).
             double *ring = lds + ring_slot(z0, Q) * WC;
-            const double *pz = src + (long long)clampi(z0, -h, n0 + h - 1) * sz;
+            const long long zoff = (long long)clampi(z0, -h, n0 + h - 1) * sz;
+            const double *pz = src + zoff;
+            const double *hz = ((t.k0 & 1) ? hB : hA) + zoff;
+            const bool zin = z0 >= 0 && z0 < n0;
             for (int c = tid; c < WC; c += NT) {
                 const int ly = c / WX, lx = c - ly * WX;
-                ring[c] = pz[clampi(oy + ly, -h, n1 + h - 1) * sy + clampi(ox + lx, -h, n2 + h - 1)];
+                const int gy = oy + ly, gx = ox + lx;
+                const int off = clampi(gy, -h, n1 + h - 1) * sy + clampi(gx, -h, n2 + h - 1);
+                const bool inside = zin && gy >= 0 && gy < n1 && gx >= 0 && gx < n2;
+                ring[c] = inside ? pz[off] : hz[off];
             }
         }
         __syncthreads();

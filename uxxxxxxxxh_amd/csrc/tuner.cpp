@@ -1,0 +1,309 @@
+// Autotuner: timed search over the runtime tiling space of a sweep, with an
+// append-only tuning database that makes a search resumable and doubles as
+// the cost model's training set.
+//
+// Replaces the reference's Python random search (searchSpace_2d/3d and
+// FilterParams2d/3d in scripts/evaluation_scripts_GPU/test_driver.py:36-99,
+// 169-196,258-285; collect.py:30-69), where every configuration was a
+// recompile plus an nsys run.  Here a configuration is a runtime argument, so
+// a point of the space costs a few sweeps.  The reference's legality filters
+// (threads per block, LDS size, exact divisibility) become uhs_check_config
+// (register staging, 64 KiB LDS, 512 threads; partial tiles are allowed).
+//
+// Database line (CSV, one per timed configuration):
+//   device,stencil,dtype,dims,e0,e1,e2,halo,tile_x,tile_y,tile_z,vec,flags,ms,status
+// `stencil` is the built-in case name or "custom"; status 0 = timed, <0 = the
+// UHS_E_* code the configuration failed with (skipped, the search goes on).
+#include "uhs.h"
+#include "uhs_host.h"
+#include "uhs_tune.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+std::string device_tag()
+{
+    hipDeviceProp_t p;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return "unknown";
+    char b[96];
+    snprintf(b, sizeof b, "%s-%dcu", p.gcnArchName, p.multiProcessorCount);
+    for (char *c = b; *c; c++)
+        if (*c == ',' || *c == ':') *c = '_';
+    return b;
+}
+
+struct Key {
+    std::string dev, stencil;
+    int dtype, dims;
+    long long e0, e1, e2;
+    int halo;
+    bool operator<(const Key &o) const
+    {
+        return std::tie(dev, stencil, dtype, dims, e0, e1, e2, halo) <
+               std::tie(o.dev, o.stencil, o.dtype, o.dims, o.e0, o.e1, o.e2, o.halo);
+    }
+};
+
+typedef std::tuple<int, int, int, int, int> CfgT;
+
+CfgT ct(const uhs_config &c) { return CfgT(c.tile_x, c.tile_y, c.tile_z, c.vec, c.flags); }
+
+// Timings read back from a database file, per problem.
+std::map<Key, std::map<CfgT, std::pair<double, int>>> load_db(const char *path)
+{
+    std::map<Key, std::map<CfgT, std::pair<double, int>>> db;
+    if (!path) return db;
+    FILE *f = fopen(path, "r");
+    if (!f) return db;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        char dev[96], st[64];
+        int dt, dims, halo, tx, ty, tz, vec, flags, status;
+        long long e0, e1, e2;
+        double ms;
+        if (sscanf(line, "%95[^,],%63[^,],%d,%d,%lld,%lld,%lld,%d,%d,%d,%d,%d,%d,%lf,%d", dev, st, &dt,
+                   &dims, &e0, &e1, &e2, &halo, &tx, &ty, &tz, &vec, &flags, &ms, &status) != 15)
+            continue; // header or a torn last line
+        db[Key{dev, st, dt, dims, e0, e1, e2, halo}][CfgT(tx, ty, tz, vec, flags)] = {ms, status};
+    }
+    fclose(f);
+    return db;
+}
+
+std::mutex g_db_mu;
+
+void append_db(const char *path, const Key &k, const uhs_config &c, double ms, int status)
+{
+    if (!path) return;
+    std::lock_guard<std::mutex> lk(g_db_mu);
+    FILE *f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%s,%s,%d,%d,%lld,%lld,%lld,%d,%d,%d,%d,%d,%d,%.6f,%d\n", k.dev.c_str(), k.stencil.c_str(),
+            k.dtype, k.dims, k.e0, k.e1, k.e2, k.halo, c.tile_x, c.tile_y, c.tile_z, c.vec, c.flags, ms,
+            status);
+    fclose(f);
+}
+
+Key problem_key(const Plan &pl, int dims, const int64_t *ext, int halo, int dtype)
+{
+    Key k;
+    k.dev = device_tag();
+    k.stencil = pl.prog >= 0 ? kBuiltins[pl.prog].case_name : "custom";
+    k.dtype = dtype;
+    k.dims = dims;
+    k.e0 = ext[0];
+    k.e1 = ext[1];
+    k.e2 = dims == 3 ? ext[2] : 1;
+    k.halo = halo;
+    return k;
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+} // namespace
+
+// The candidate space of a plan, filtered by uhs_check_config.
+std::vector<uhs_config> uhs_config_space(const Plan &pl)
+{
+    std::vector<uhs_config> out;
+    if (pl.family == FAM_GATHER) return out;
+    static const int txs3[] = {32, 64, 128, 256};
+    static const int tys3[] = {1, 2, 4, 8, 16};
+    static const int tzs[] = {16, 32, 64, 128, 256, 512};
+    static const int txs2[] = {64, 128, 256, 512, 1024};
+    static const int vecs[] = {1, 2, 4};
+    const bool d3 = pl.args.dims == 3;
+    const int nflags = pl.family == FAM_STAR3D ? 4 : 2;
+    char why[160];
+    for (int tx : (d3 ? std::vector<int>(txs3, txs3 + 4) : std::vector<int>(txs2, txs2 + 5)))
+        for (int ty : (d3 ? std::vector<int>(tys3, tys3 + 5) : std::vector<int>{1}))
+            for (int tz : tzs)
+                for (int v : vecs)
+                    for (int fl = 0; fl < nflags; fl++) {
+                        uhs_config c{tx, ty, tz, v, fl};
+                        if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) continue;
+                        // Skip slabs much longer than the streamed axis: they
+                        // time the same as the axis length.
+                        if (tz > 2 * pl.args.n0 && tz > 16) continue;
+                        out.push_back(c);
+                    }
+    return out;
+}
+
+// Times `reps` sweeps of one configuration (after one warm-up sweep).
+int uhs_time_config(Plan &pl, void *A, void *B, const uhs_config &c, int reps, hipStream_t st,
+                    double *ms)
+{
+    Plan p = pl;
+    char why[160];
+    if (uhs_check_config(p, c, why, sizeof why) != UHS_OK) return UHS_E_CONFIG;
+    p.cfg = c;
+    p.args.tx = c.tile_x; p.args.ty = c.tile_y; p.args.tz = c.tile_z; p.args.vec = c.vec;
+    p.args.flags = c.flags;
+    switch (p.family) {
+    case FAM_STAR3D:
+        p.fn = uhs_find_star3d(p.prog, p.args.dtype, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
+        break;
+    case FAM_BOX3D: p.fn = uhs_find_box3d(p.prog, p.args.dtype, c.vec); break;
+    case FAM_WIN2D: p.fn = uhs_find_win2d(p.prog, p.args.dtype, c.vec); break;
+    default: return UHS_E_UNSUPPORTED;
+    }
+    if (!p.fn) return UHS_E_CONFIG;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return UHS_E_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return UHS_E_HIP;
+    }
+    int rc = uhs_run_plan(p, A, B, 1, st);
+    if (rc == UHS_OK) {
+        (void)hipEventRecord(e0, st);
+        rc = uhs_run_plan(p, A, B, reps, st);
+        (void)hipEventRecord(e1, st);
+    }
+    float t = 0.f;
+    if (rc == UHS_OK) {
+        hipError_t e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) rc = UHS_E_HIP;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return rc;
+}
+
+// Search over `cands` (all of them, or a pruned list), resuming from db.
+int uhs_search(Plan &pl, void *A, void *B, int dims, const int64_t *ext, int halo, int dtype,
+               const std::vector<uhs_config> &cands, int reps, const char *db_path, hipStream_t st,
+               uhs_tune_result *res)
+{
+    const double t0 = now_s();
+    const Key key = problem_key(pl, dims, ext, halo, dtype);
+    auto db = load_db(db_path);
+    auto &known = db[key];
+    uhs_tune_result r{};
+    r.best_ms = 1e30;
+    for (const uhs_config &c : cands) {
+        double ms = 0;
+        int status;
+        auto it = known.find(ct(c));
+        if (it != known.end()) {
+            ms = it->second.first;
+            status = it->second.second;
+            r.reused++;
+        } else {
+            status = uhs_time_config(pl, A, B, c, reps, st, &ms);
+            if (status == UHS_E_HIP) {
+                // A launch failure leaves the stream unusable: stop here.
+                (void)hipGetLastError();
+                append_db(db_path, key, c, 0.0, status);
+                return uhs_set_error(UHS_E_HIP, "a configuration failed to launch; search stopped");
+            }
+            append_db(db_path, key, c, ms, status);
+        }
+        if (status != UHS_OK) {
+            r.failed++;
+            continue;
+        }
+        r.evaluated++;
+        if (ms < r.best_ms) {
+            r.best_ms = ms;
+            r.best = c;
+        }
+    }
+    r.seconds = now_s() - t0;
+    if (r.evaluated == 0) return uhs_set_error(UHS_E_CONFIG, "no configuration could be timed");
+    uhs_store_tuned(uhs_config_key(pl.family, pl.prog, dtype, dims, ext), r.best);
+    if (res) *res = r;
+    return UHS_OK;
+}
+
+extern "C" {
+
+int uhs_tune_sweep(void *A, void *B, int dims, const int64_t *ext, int halo, const uhs_stencil *st,
+                   int dtype, int reps, const char *db_path, uhs_tune_result *out, uhs_stream_t stream)
+{
+    if (!A || !B || A == B) return uhs_set_error(UHS_E_INVAL, "tuning needs two distinct grids");
+    if (reps < 1) reps = 1;
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, dtype, nullptr, &pl);
+    if (rc) return rc;
+    if (pl.family == FAM_GATHER) return uhs_set_error(UHS_E_UNSUPPORTED, "the general kernel has no tiling to tune");
+    return uhs_search(pl, A, B, dims, ext, halo, dtype, uhs_config_space(pl), reps, db_path,
+                      (hipStream_t)stream, out);
+}
+
+int uhs_tune_space(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,
+                   uhs_config *out, int cap)
+{
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, dtype, nullptr, &pl);
+    if (rc) return rc;
+    const auto sp = uhs_config_space(pl);
+    if (out)
+        for (int i = 0; i < (int)sp.size() && i < cap; i++) out[i] = sp[i];
+    return (int)sp.size();
+}
+
+int uhs_tune_time(void *A, void *B, int dims, const int64_t *ext, int halo, const uhs_stencil *st,
+                  int dtype, const uhs_config *cfg, int reps, double *ms, uhs_stream_t stream)
+{
+    if (!A || !B || !cfg || !ms) return uhs_set_error(UHS_E_INVAL, "null argument");
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, dtype, cfg, &pl);
+    if (rc) return rc;
+    return uhs_time_config(pl, A, B, *cfg, reps < 1 ? 1 : reps, (hipStream_t)stream, ms);
+}
+
+int uhs_tune_load_db(const char *db_path)
+{
+    if (!db_path) return uhs_set_error(UHS_E_INVAL, "null path");
+    FILE *f = fopen(db_path, "r");
+    if (!f) return uhs_set_error(UHS_E_IO, "cannot open the tuning database");
+    fclose(f);
+    const std::string dev = device_tag();
+    int loaded = 0;
+    for (auto &kv : load_db(db_path)) {
+        const Key &k = kv.first;
+        if (k.dev != dev || k.stencil == "custom") continue;
+        double best = 1e30;
+        CfgT bc{};
+        for (auto &c : kv.second)
+            if (c.second.second == UHS_OK && c.second.first < best) {
+                best = c.second.first;
+                bc = c.first;
+            }
+        if (best >= 1e30) continue;
+        const int prog = uhs_builtin_find(k.stencil.c_str());
+        if (prog < 0) continue;
+        const int fam = k.dims == 2 ? FAM_WIN2D : (kBuiltins[prog].star ? FAM_STAR3D : FAM_BOX3D);
+        const int64_t ext[3] = {k.e0, k.e1, k.e2};
+        uhs_config c{std::get<0>(bc), std::get<1>(bc), std::get<2>(bc), std::get<3>(bc), std::get<4>(bc)};
+        uhs_store_tuned(uhs_config_key(fam, prog, k.dtype, k.dims, ext), c);
+        loaded++;
+    }
+    return loaded;
+}
+
+int uhs_tune_clear(void)
+{
+    uhs_clear_tuned();
+    return UHS_OK;
+}
+
+} // extern "C"

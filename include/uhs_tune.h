@@ -36,6 +36,12 @@ int uhs_tune_sweep(void *A, void *B, int dims, const int64_t *ext, int halo,
                    const uhs_stencil *st, int dtype, int reps,
                    const char *db_path, uhs_tune_result *out,
                    uhs_stream_t stream);
+/* The same search over an explicit candidate list (e.g. a random sample of
+ * the space); illegal candidates are recorded as failed and skipped. */
+int uhs_tune_sweep_list(void *A, void *B, int dims, const int64_t *ext, int halo,
+                        const uhs_stencil *st, int dtype, const uhs_config *cands,
+                        int ncand, int reps, const char *db_path,
+                        uhs_tune_result *out, uhs_stream_t stream);
 /* The legal configurations the search visits: returns their number and
  * copies up to cap of them into out (out may be NULL). */
 int uhs_tune_space(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
@@ -48,6 +54,16 @@ int uhs_tune_time(void *A, void *B, int dims, const int64_t *ext, int halo,
  * device, its fastest configuration as the default.  Returns the count. */
 int uhs_tune_load_db(const char *db_path);
 int uhs_tune_clear(void);
+
+/* Timed search over batch configurations (temporal depth {1,2,4} x star
+ * tiles x box tiles) for a batch of grids: each candidate is planned,
+ * captured and run for `timesteps` steps (grid contents are overwritten);
+ * the fastest goes to *best.  ms_out (optional, ncand_cap entries) receives
+ * each candidate's time, <0 for candidates that could not be built. */
+int uhs_batch_tune(const uhs_grid_desc *grids, int n, int timesteps,
+                   uhs_batch_config *best, double *best_ms,
+                   uhs_batch_config *cands_out, double *ms_out, int ncand_cap,
+                   int *ncand, uhs_stream_t stream);
 
 /* ---- learned cost model ---------------------------------------------------
  * An MLP (UHS_MLP_INPUTS -> 128 -> 128 -> 128 -> 1, ReLU, FP32) mapping a

@@ -104,6 +104,11 @@ _SIGS = {
                             ctypes.POINTER(TuneResult), _P]),
     "uhs_tune_sweep_model": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, _P, _I, _I, ctypes.c_char_p,
                                   ctypes.POINTER(TuneResult), _P]),
+    "uhs_tune_sweep_list": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), _I, _I,
+                                 ctypes.c_char_p, ctypes.POINTER(TuneResult), _P]),
+    "uhs_batch_tune": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(BatchConfig),
+                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_double),
+                            _I, ctypes.POINTER(_I), _P]),
     "uhs_tune_space": (_I, [_I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), _I]),
     "uhs_tune_time": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), _I,
                            ctypes.POINTER(ctypes.c_double), _P]),
@@ -382,6 +387,16 @@ class Batch:
         check(lib().uhs_batch_info(self.h, ctypes.byref(nl), ctypes.byref(nt), ctypes.byref(wg), ctypes.byref(lds)))
         return {"launches": nl.value, "tasks": nt.value, "workgroups": wg.value, "lds": lds.value}
 
+    def tune(self, timesteps=None, stream=None):
+        """Times the batch-configuration space; returns (best, best_ms, [(cfg, ms)])."""
+        cap = 1024
+        cands = (BatchConfig * cap)()
+        ms = (ctypes.c_double * cap)()
+        best, bms, nc = BatchConfig(), ctypes.c_double(), ctypes.c_int()
+        check(lib().uhs_batch_tune(self.desc, self.n, timesteps or self.timesteps, ctypes.byref(best),
+                                   ctypes.byref(bms), cands, ms, cap, ctypes.byref(nc), stream))
+        return best, bms.value, [(cands[i], ms[i]) for i in range(min(cap, nc.value))]
+
     def run_each(self, stream=None):
         """The unbatched baseline on the same grids."""
         check(lib().uhs_sweep_each_f64(self.desc, self.n, self.timesteps, stream))
@@ -425,6 +440,14 @@ def tune_sweep(A, B, ext, halo, stencil, dtype=DT_F64, reps=3, db=None, model=No
     else:
         check(lib().uhs_tune_sweep_model(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
                                          dtype, model.h, topk, reps, dbp, ctypes.byref(r), stream))
+    return r
+
+
+def tune_list(A, B, ext, halo, stencil, cands, dtype=DT_F64, reps=2, db=None, stream=None):
+    arr = (Config * len(cands))(*[c if isinstance(c, Config) else Config(*c) for c in cands])
+    r = TuneResult()
+    check(lib().uhs_tune_sweep_list(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,
+                                    arr, len(cands), reps, db.encode() if db else None, ctypes.byref(r), stream))
     return r
 
 

@@ -4,6 +4,7 @@
 // steps per launch (temporal blocking); box-stencil grids one step.
 #include "batch_kernels.h"
 #include "uhs_host.h"
+#include "uhs_tune.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -324,6 +325,60 @@ int uhs_batch_info(uhs_batch_t b, int *launches, long long *tasks, int *max_work
     if (tasks) *tasks = b->ntasks;
     if (max_workgroups) *max_workgroups = wg;
     if (max_lds) *max_lds = lds;
+    return UHS_OK;
+}
+
+int uhs_batch_tune(const uhs_grid_desc *grids, int n, int timesteps, uhs_batch_config *best, double *best_ms,
+                   uhs_batch_config *cands_out, double *ms_out, int ncand_cap, int *ncand, uhs_stream_t stream)
+{
+    if (!grids || n < 1 || !best) return berr(UHS_E_INVAL, "batch tuning needs grids and an output");
+    static const int depths[] = {1, 2, 4};
+    static const int stiles[][2] = {{32, 8}, {64, 4}, {64, 8}, {128, 4}, {32, 16}};
+    static const int stz[] = {64, 512};
+    static const int btiles[][2] = {{32, 8}, {64, 4}, {16, 16}};
+    static const int btz[] = {128, 512};
+    std::vector<uhs_batch_config> cand;
+    for (int d : depths)
+        for (auto &s : stiles)
+            for (int sz : stz)
+                for (auto &b : btiles)
+                    for (int bz : btz) cand.push_back(uhs_batch_config{d, s[0], s[1], sz, b[0], b[1], bz});
+    double bms = 1e30;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return berr(UHS_E_HIP, "events");
+    for (size_t i = 0; i < cand.size(); i++) {
+        double ms = -1.0;
+        uhs_batch_t b = nullptr;
+        if (uhs_batch_create(grids, n, timesteps, &cand[i], &b) == UHS_OK) {
+            float t = 0.f;
+            bool ok = uhs_batch_run(b, stream) == UHS_OK; // warm-up
+            ok = ok && hipEventRecord(e0, (hipStream_t)stream) == hipSuccess;
+            ok = ok && uhs_batch_run(b, stream) == UHS_OK;
+            ok = ok && hipEventRecord(e1, (hipStream_t)stream) == hipSuccess;
+            ok = ok && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+            uhs_batch_destroy(b);
+            if (!ok) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return berr(UHS_E_HIP, "batch run failed during tuning");
+            }
+            ms = t;
+            if (ms < bms) {
+                bms = ms;
+                *best = cand[i];
+            }
+        }
+        if ((int)i < ncand_cap) {
+            if (cands_out) cands_out[i] = cand[i];
+            if (ms_out) ms_out[i] = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ncand) *ncand = (int)cand.size();
+    if (bms >= 1e30) return berr(UHS_E_CONFIG, "no batch configuration could be built");
+    if (best_ms) *best_ms = bms;
     return UHS_OK;
 }
 

@@ -248,6 +248,20 @@ int uhs_tune_sweep(void *A, void *B, int dims, const int64_t *ext, int halo, con
                       (hipStream_t)stream, out);
 }
 
+int uhs_tune_sweep_list(void *A, void *B, int dims, const int64_t *ext, int halo, const uhs_stencil *st,
+                        int dtype, const uhs_config *cands, int ncand, int reps, const char *db_path,
+                        uhs_tune_result *out, uhs_stream_t stream)
+{
+    if (!A || !B || A == B || !cands || ncand < 1)
+        return uhs_set_error(UHS_E_INVAL, "tuning needs two distinct grids and candidates");
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, dtype, nullptr, &pl);
+    if (rc) return rc;
+    if (pl.family == FAM_GATHER) return uhs_set_error(UHS_E_UNSUPPORTED, "the general kernel has no tiling to tune");
+    std::vector<uhs_config> c(cands, cands + ncand);
+    return uhs_search(pl, A, B, dims, ext, halo, dtype, c, reps < 1 ? 1 : reps, db_path, (hipStream_t)stream, out);
+}
+
 int uhs_tune_space(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,
                    uhs_config *out, int cap)
 {

@@ -82,6 +82,71 @@ def copy_bandwidth(u, nbytes=2 << 30, reps=10):
     return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
 
 
+def batched(u, T=50, probe_T=8):
+    """64 uneven 3D grids (extents from {64,96,...,384}), half 3d1r_star and
+    half 3d2r_box (125 points), seeded normalised coefficients, T steps: the
+    plain per-grid sweeps against the batched graph with the configuration
+    the batch autotuner picks (timed on a probe_T-step run)."""
+    rng = np.random.default_rng(2024)
+    choices = list(range(64, 385, 32))
+    grids, pts = [], 0
+    for i in range(64):
+        case = "3d1r_star" if i < 32 else "3d2r_box"
+        st = u.builtin(case)
+        st = st.with_coeffs(u.make_coeffs(st, 500 + i))
+        ext = [int(rng.choice(choices)) for _ in range(3)]
+        h = st.radius
+        A = u.DeviceArray(u.grid_shape(ext, h), np.float64)
+        B = u.DeviceArray(u.grid_shape(ext, h), np.float64)
+        u.fill_uniform(A, ext, h, 7 + i)
+        u.fill_uniform(B, ext, h, 1007 + i)
+        grids.append((A, B, ext, h, st))
+        pts += int(np.prod(ext))
+
+    def timed(fn, reps=2):
+        fn()
+        u.synchronize()
+        tm = u.Timer()
+        tm.start()
+        for _ in range(reps):
+            fn()
+        tm.stop()
+        return tm.ms() / reps
+
+    probe = u.Batch(grids, probe_T)
+    t0 = time.time()
+    best, _, cands = probe.tune()
+    t_tune = time.time() - t0
+    probe.close()
+    b0 = u.Batch(grids, T, u.batch_default_config())
+    ms_plain = timed(b0.run_each)
+    b0.close()
+    b = u.Batch(grids, T, best)
+    ms_batch = timed(b.run)
+    info = b.info()
+    b.close()
+    for A, B, _, _, _ in grids:
+        A.close()
+        B.close()
+    upd = float(pts) * T
+    return {"grids": 64, "timesteps": T, "updates": upd,
+            "plain_ms": round(ms_plain, 3), "plain_GStencils_s": round(upd / ms_plain / 1e6, 2),
+            "batched_ms": round(ms_batch, 3), "batched_GStencils_s": round(upd / ms_batch / 1e6, 2),
+            "speedup": round(ms_plain / ms_batch, 3),
+            "config": {"star_depth": best.star_depth, "star_tile": [best.star_tile_x, best.star_tile_y, best.star_tile_z],
+                       "box_tile": [best.box_tile_x, best.box_tile_y, best.box_tile_z]},
+            "tuning_candidates": len(cands), "tuning_s": round(t_tune, 2), "launches": info["launches"],
+            "tasks": info["tasks"]}
+
+
+def fp32_kahan(u):
+    from uxxxxxxxxh_amd.fp32study import study
+    r = study("3d2r_star", [512, 512, 512], [10, 100, 1000])
+    return {"workload": "3d2r_star FP32 (Kahan-compensated) 512^3 vs the FP64 sweep on identical inputs",
+            "GStencils_s": round(r["f32k_GStencils_s"], 2), "hbm_frac_at_8B_per_update": round(r["f32k_hbm_frac"], 4),
+            "errors": r["checkpoints"]}
+
+
 def cpu_baseline(ext, case="3d1r_star", budget_s=10.0):
     """Single-thread oracle sweep (plain mode) on the same workload shape."""
     from tests.oracle_py import Oracle, PLAIN
@@ -114,6 +179,12 @@ def cpu_baseline(ext, case="3d1r_star", budget_s=10.0):
                       % (case, "x".join(map(str, ext)), sweeps, el)}
 
 
+def kernel_name(cfg):
+    if cfg.flags & 4:
+        return "k_star3d_rt<double,Prog_3d1r_star,%d,false>" % cfg.vec
+    return "k_star3d<double,Prog_3d1r_star,%d,false,%d>" % (cfg.vec, 2 if cfg.flags & 2 else 1)
+
+
 def load_traffic(case, ext):
     p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (case, "x".join(map(str, ext))))
     if os.path.exists(p):
@@ -131,6 +202,8 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-shapes", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -166,11 +239,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded U[0,1) interior, zero halo, symmetric normalised coefficients)",
         "config": {"workload": "3d1r_star FP64 %dx%dx%d interior, halo 1, T=%d per step" % (n, n, n, T),
-                   "tile": [cfg.tile_x, cfg.tile_y, cfg.tile_z, cfg.vec]},
+                   "tile": [cfg.tile_x, cfg.tile_y, cfg.tile_z, cfg.vec, cfg.flags]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "k_star3d<double,Prog_3d1r_star,2,false>",
+                     "kernel": kernel_name(cfg),
                      "algorithmic_bytes_per_launch": int(16 * updates),
                      "avg_launch_ms": round(ms_sweep, 5)},
     }
@@ -193,9 +266,15 @@ def main():
             per[case] = {"GStencils_s": round(g, 2), "ms_per_sweep": round(ms_w, 4),
                          "hbm_frac": round(g * 16 / PEAK_HBM_GBPS, 4),
                          "roof_GStencils_s": round(roof, 1), "roof_frac": round(g / roof, 4),
-                         "tile": [c.tile_x, c.tile_y, c.tile_z, c.vec]}
+                         "tile": [c.tile_x, c.tile_y, c.tile_z, c.vec, c.flags]}
             log("%-10s %8.2f GSt/s  %.3f ms/sweep  roof %.0f  frac %.3f" % (case, g, ms_w, roof, g / roof))
         out["per_shape"] = per
+    if not args.no_batch:
+        out["batched"] = batched(u)
+        log("batched", out["batched"])
+    if not args.no_fp32:
+        out["fp32_kahan"] = fp32_kahan(u)
+        log("fp32", out["fp32_kahan"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ext)
         log("cpu baseline", out["cpu_baseline"])

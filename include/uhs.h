@@ -76,10 +76,16 @@ typedef struct uhs_stencil {
  *   tile_z  length of the slab streamed through registers along axis 0
  *   vec     points per thread along the unit-stride axis (1, 2 or 4)
  *   flags   UHS_CFG_NT_STORE: output stores bypass the caches;
- *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight
- * Threads per workgroup = tile_x / vec * tile_y. */
+ *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight;
+ *           UHS_CFG_REGTILE: 3D stars use the register-tiled kernel (no LDS,
+ *           no barriers): tile_x must be 64 (one wavefront across, 64 - 2r
+ *           columns computed), vec = rows per lane (2, 4 or 8), tile_y a
+ *           multiple of vec (tile_y / vec wavefronts per workgroup)
+ * Threads per workgroup = tile_x / vec * tile_y (64 * tile_y / vec with
+ * UHS_CFG_REGTILE). */
 #define UHS_CFG_NT_STORE 1
 #define UHS_CFG_PREFETCH2 2
+#define UHS_CFG_REGTILE 4
 typedef struct uhs_config {
     int tile_x, tile_y, tile_z, vec, flags;
 } uhs_config;

@@ -101,9 +101,12 @@ def test_dsl_coefficients_and_halo_untouched(gpu, orc, case):
 
 
 CONFIGS = {
-    "star3d": [(64, 8, 64, 2), (32, 4, 16, 1), (128, 4, 8, 4), (64, 2, 5, 2), (256, 2, 3, 4)],
-    "box3d": [(64, 4, 128, 1), (32, 8, 7, 2), (16, 4, 3, 1), (128, 2, 11, 2)],
-    "win2d": [(256, 1, 64, 1), (128, 1, 5, 2), (64, 1, 17, 4), (512, 1, 9, 1)],
+    "star3d": [(64, 8, 64, 2), (32, 4, 16, 1), (128, 4, 8, 4), (64, 2, 5, 2), (256, 2, 3, 4),
+               (128, 16, 9, 4, 2), (64, 8, 6, 2, 3),
+               # register-tiled kernel: 64-lane bands, vec rows per lane
+               (64, 8, 16, 4, 4), (64, 4, 7, 2, 5), (64, 32, 64, 8, 4), (64, 16, 3, 8, 4), (64, 2, 11, 2, 4)],
+    "box3d": [(64, 4, 128, 1), (32, 8, 7, 2), (16, 4, 3, 1), (128, 2, 11, 2), (32, 16, 5, 1, 1)],
+    "win2d": [(256, 1, 64, 1), (128, 1, 5, 2), (64, 1, 17, 4), (512, 1, 9, 1), (1024, 1, 3, 2, 1)],
 }
 
 
@@ -128,6 +131,23 @@ def test_config_invariance(gpu, orc, case):
         assert np.array_equal(gA, A) and np.array_equal(gB, B), (case, cfg)
         ran += 1
     assert ran >= 2
+
+
+@pytest.mark.parametrize("case", ["3d1r_star", "3d2r_star", "3d3r_star"])
+@pytest.mark.parametrize("dtype", ["f64", "f32", "f32k"])
+def test_register_tiled_star_all_precisions(gpu, orc, case, dtype):
+    u = gpu
+    s = SHAPES[case]
+    st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 12))
+    ext, h = [13, 29, 131], s["radius"] + 1
+    A0, B0 = random_grids(ext, h, 13)
+    dt = np.float64 if dtype == "f64" else np.float32
+    A0, B0 = A0.astype(dt), B0.astype(dt)
+    for cfg in ((64, 8, 5, 4, 4), (64, 16, 64, 2, 5)):
+        gA, gB = run_gpu(u, st, A0, B0, ext, h, 3, dt, dtype == "f32k", cfg=cfg)
+        A, B = A0.copy(), B0.copy()
+        orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, 3, KAHAN if dtype == "f32k" else FUSED)
+        assert np.array_equal(gA, A) and np.array_equal(gB, B), cfg
 
 
 @pytest.mark.parametrize("case", CASES)

@@ -93,10 +93,24 @@ void uhs_clear_tuned()
 static long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
 
 // Workgroups a configuration launches for the given problem.
-long long uhs_config_blocks(const SweepArgs &a, const uhs_config &c)
+long long uhs_config_blocks(const SweepArgs &a, const uhs_config &c, int radius)
 {
-    const long long tiles = ceil_div(a.n2, c.tile_x) * (a.dims == 3 ? ceil_div(a.n1, c.tile_y) : 1);
+    const long long xw = (c.flags & UHS_CFG_REGTILE) ? 64 - 2 * radius : c.tile_x;
+    const long long tiles = ceil_div(a.n2, xw) * (a.dims == 3 ? ceil_div(a.n1, c.tile_y) : 1);
     return tiles * ceil_div(a.n0, c.tile_z);
+}
+
+LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
+{
+    const int dt = pl.args.dtype;
+    switch (pl.family) {
+    case FAM_STAR3D:
+        if (c.flags & UHS_CFG_REGTILE) return uhs_find_star3d_rt(pl.prog, dt, c.vec);
+        return uhs_find_star3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
+    case FAM_BOX3D: return uhs_find_box3d(pl.prog, dt, c.vec);
+    case FAM_WIN2D: return uhs_find_win2d(pl.prog, dt, c.vec);
+    default: return uhs_find_gather(dt);
+    }
 }
 
 int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whylen)
@@ -113,8 +127,21 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return std::string(buf);
     };
     if (pl.family == FAM_GATHER) return UHS_OK;
-    if (c.vec != 1 && c.vec != 2 && c.vec != 4) return bad(str("vec must be 1, 2 or 4 (got %lld)", c.vec));
+    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE)) return bad("unknown config flags");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
+    if (c.flags & UHS_CFG_REGTILE) {
+        if (pl.family != FAM_STAR3D) return bad("the register-tiled kernel exists for 3D star stencils only");
+        if (c.flags & UHS_CFG_PREFETCH2) return bad("the register-tiled kernel has no two-plane prefetch variant");
+        if (c.tile_x != 64) return bad("the register-tiled kernel needs tile_x = 64 (one wavefront)");
+        if (c.vec != 2 && c.vec != 4 && c.vec != 8) return bad(str("rows per lane must be 2, 4 or 8 (got %lld)", c.vec));
+        if (c.tile_y % c.vec) return bad(str("tile_y %lld not a multiple of the rows per lane %lld", c.tile_y, c.vec));
+        const long long nt = 64LL * (c.tile_y / c.vec);
+        if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
+        if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
+        if (!uhs_pick_fn(pl, c)) return bad("register-tiled kernel not compiled for this stencil");
+        return UHS_OK;
+    }
+    if (c.vec != 1 && c.vec != 2 && c.vec != 4) return bad(str("vec must be 1, 2 or 4 (got %lld)", c.vec));
     if (c.tile_x % c.vec) return bad(str("tile_x %lld not a multiple of vec %lld", c.tile_x, c.vec));
     if (a.dims == 2 && c.tile_y != 1) return bad("2D stencils need tile_y = 1");
     const long long nt = (long long)(c.tile_x / c.vec) * c.tile_y;
@@ -136,16 +163,10 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
                            (c.tile_x + 2 * R) * (c.tile_y + 2 * R), kBoxCellsPerThread));
     }
     if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
-    if (uhs_config_blocks(a, c) > INT_MAX) return bad("too many workgroups");
-    LaunchFn fn = nullptr;
-    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2)) return bad("unknown config flags");
+    if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
     if ((c.flags & UHS_CFG_PREFETCH2) && pl.family != FAM_STAR3D)
         return bad("two-plane prefetch exists for 3D star stencils only");
-    if (pl.family == FAM_STAR3D)
-        fn = uhs_find_star3d(pl.prog, a.dtype, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
-    if (pl.family == FAM_BOX3D) fn = uhs_find_box3d(pl.prog, a.dtype, c.vec);
-    if (pl.family == FAM_WIN2D) fn = uhs_find_win2d(pl.prog, a.dtype, c.vec);
-    if (!fn) return bad(std::string("no ") + family_name(pl.family) + str(" kernel compiled for vec %lld", c.vec));
+    if (!uhs_pick_fn(pl, c)) return bad(std::string("no ") + family_name(pl.family) + str(" kernel compiled for vec %lld", c.vec));
     return UHS_OK;
 }
 
@@ -180,7 +201,7 @@ uhs_config uhs_heuristic_config(const Plan &pl)
     }
     // Small grids: shorten the streamed slab until the launch has enough
     // workgroups to cover 256 CUs a few times over.
-    while (c.tile_z > 8 && uhs_config_blocks(a, c) < 2048) c.tile_z /= 2;
+    while (c.tile_z > 8 && uhs_config_blocks(a, c, pl.radius) < 2048) c.tile_z /= 2;
     return c;
 }
 
@@ -250,14 +271,7 @@ int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
                     c.tile_z, c.vec, c.flags, why);
     a.tx = c.tile_x; a.ty = c.tile_y; a.tz = c.tile_z; a.vec = c.vec; a.flags = c.flags;
     pl->cfg = c;
-    switch (pl->family) {
-    case FAM_STAR3D:
-        pl->fn = uhs_find_star3d(pl->prog, dtype, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
-        break;
-    case FAM_BOX3D: pl->fn = uhs_find_box3d(pl->prog, dtype, c.vec); break;
-    case FAM_WIN2D: pl->fn = uhs_find_win2d(pl->prog, dtype, c.vec); break;
-    default: pl->fn = uhs_find_gather(dtype); break;
-    }
+    pl->fn = uhs_pick_fn(*pl, c);
     if (!pl->fn) return fail(UHS_E_UNSUPPORTED, "no kernel for this stencil and precision");
     return UHS_OK;
 }

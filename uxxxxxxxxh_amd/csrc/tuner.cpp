@@ -141,6 +141,16 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
                         if (tz > 2 * pl.args.n0 && tz > 16) continue;
                         out.push_back(c);
                     }
+    if (pl.family == FAM_STAR3D) // register-tiled variants
+        for (int rows : {2, 4, 8})
+            for (int waves : {1, 2, 4, 8})
+                for (int tz : tzs)
+                    for (int fl : {UHS_CFG_REGTILE, UHS_CFG_REGTILE | UHS_CFG_NT_STORE}) {
+                        uhs_config c{64, rows * waves, tz, rows, fl};
+                        if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) continue;
+                        if (tz > 2 * pl.args.n0 && tz > 16) continue;
+                        out.push_back(c);
+                    }
     return out;
 }
 
@@ -154,14 +164,7 @@ int uhs_time_config(Plan &pl, void *A, void *B, const uhs_config &c, int reps, h
     p.cfg = c;
     p.args.tx = c.tile_x; p.args.ty = c.tile_y; p.args.tz = c.tile_z; p.args.vec = c.vec;
     p.args.flags = c.flags;
-    switch (p.family) {
-    case FAM_STAR3D:
-        p.fn = uhs_find_star3d(p.prog, p.args.dtype, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
-        break;
-    case FAM_BOX3D: p.fn = uhs_find_box3d(p.prog, p.args.dtype, c.vec); break;
-    case FAM_WIN2D: p.fn = uhs_find_win2d(p.prog, p.args.dtype, c.vec); break;
-    default: return UHS_E_UNSUPPORTED;
-    }
+    p.fn = uhs_pick_fn(p, c);
     if (!p.fn) return UHS_E_CONFIG;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return UHS_E_HIP;

@@ -23,7 +23,9 @@ int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
 int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream);
 int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whylen);
 uhs_config uhs_heuristic_config(const Plan &pl);
-long long uhs_config_blocks(const SweepArgs &a, const uhs_config &c);
+// The launcher for a plan under configuration c (nullptr: not compiled).
+LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c);
+long long uhs_config_blocks(const SweepArgs &a, const uhs_config &c, int radius);
 
 // Tuned configurations, keyed by kernel family, stencil, precision and extents.
 std::string uhs_config_key(int family, int prog, int dtype, int dims, const int64_t *ext);

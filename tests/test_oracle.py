@@ -177,3 +177,25 @@ def test_kahan_beats_plain_f32(orc):
         orc.sweep(x32, o, ext, r, s["offsets"], c, mode)
         errs[mode] = np.abs(interior(o, r) - interior(ref, r)).max()
     assert errs[KAHAN] < errs[FUSED]
+
+
+@pytest.mark.parametrize("case", ["2d3r_box", "3d2r_box", "3d3r_box"])
+def test_kahan_recovers_absorbed_terms(orc, case):
+    """1.0 followed by P-1 terms of 2^-25: in FP32 every plain addition of
+    2^-25 to 1.0 is absorbed (below half an ulp), so the plain sum is exactly
+    1; the compensated sum recovers 1 + (P-1) * 2^-25 to within one rounding."""
+    s = SHAPES[case]
+    d, r, P = s["dims"], s["radius"], s["npoints"]
+    ext = [2 * r + 3] * d
+    p = tuple(e // 2 + r for e in ext)
+    a = np.zeros([e + 2 * r for e in ext], np.float32)
+    for t, o in enumerate(s["offsets"]):
+        a[tuple(pi + oi for pi, oi in zip(p, o))] = 1.0 if t == 0 else 2.0 ** -25
+    exact = 1.0 + (P - 1) * 2.0 ** -25
+    for mode, expect_exact in ((FUSED, False), (KAHAN, True)):
+        b = np.zeros_like(a)
+        orc.sweep(a, b, ext, r, s["offsets"], np.ones(P), mode)
+        if expect_exact:
+            assert abs(float(b[p]) - exact) <= 2.0 ** -23
+        else:
+            assert b[p] == np.float32(1.0)

@@ -199,9 +199,13 @@ uhs_config uhs_heuristic_config(const Plan &pl)
         c.vec = 1;
         c.flags = 0;
     }
-    // Small grids: shorten the streamed slab until the launch has enough
-    // workgroups to cover 256 CUs a few times over.
-    while (c.tile_z > 8 && uhs_config_blocks(a, c, pl.radius) < 2048) c.tile_z /= 2;
+    // Small grids: shorten the streamed slab until the launch has at least
+    // one workgroup per CU (the defaults above are the fastest at 512^3 /
+    // 8192^2, where they launch >= 256) -- but not below the point where
+    // re-reading (star) or re-computing (box: 2R planes of partial sums per
+    // slab) the slab's 2R boundary planes costs > 25%.
+    const int tz_min = std::max(8, 8 * pl.radius);
+    while (c.tile_z / 2 >= tz_min && uhs_config_blocks(a, c, pl.radius) < 256) c.tile_z /= 2;
     return c;
 }
 

@@ -199,13 +199,15 @@ uhs_config uhs_heuristic_config(const Plan &pl)
         c.vec = 1;
         c.flags = 0;
     }
-    // Small grids: shorten the streamed slab until the launch has at least
-    // one workgroup per CU (the defaults above are the fastest at 512^3 /
-    // 8192^2, where they launch >= 256) -- but not below the point where
-    // re-reading (star) or re-computing (box: 2R planes of partial sums per
-    // slab) the slab's 2R boundary planes costs > 25%.
+    // Small grids: shorten the streamed slab until the launch fills the
+    // chip -- one workgroup per CU for the HBM-bound kernels (the defaults
+    // above launch >= 256 at 512^3 / 8192^2 and are fastest there), four per
+    // CU for the VALU-bound boxes, which need the occupancy -- but not below
+    // the point where re-reading (star) or re-computing (box: 2R planes of
+    // partial sums per slab) the slab's 2R boundary planes costs > 25%.
     const int tz_min = std::max(8, 8 * pl.radius);
-    while (c.tile_z / 2 >= tz_min && uhs_config_blocks(a, c, pl.radius) < 256) c.tile_z /= 2;
+    const long long want = pl.family == FAM_BOX3D ? 1024 : 256;
+    while (c.tile_z / 2 >= tz_min && uhs_config_blocks(a, c, pl.radius) < want) c.tile_z /= 2;
     return c;
 }
 

@@ -65,17 +65,20 @@ struct uhs_batch {
 
 static std::vector<int> star_schedule(int T, int K)
 {
-    std::vector<int> d;
+    // Single steps first, then the K-step blocks.  The result must end in A
+    // for even T and in B for odd T: the number of launches (each reads one
+    // buffer and writes the other) must have T's parity; turning one K-block
+    // into K single steps flips it (K even).  With the single steps first, a
+    // single-step launch always reads the buffer the one-step ping-pong
+    // would hold its state in.
     if (K <= 1) return std::vector<int>(T, 1);
-    for (int i = 0; i < T / K; i++) d.push_back(K);
-    for (int i = 0; i < T % K; i++) d.push_back(1);
-    // The result must end in A for even T and in B for odd T: the number of
-    // launches (each reads one buffer and writes the other) must have T's
-    // parity.  Splitting one K-block into K single steps flips it (K even).
-    if ((int)(d.size() % 2) != T % 2 && !d.empty() && d[0] == K) {
-        d.erase(d.begin());
-        d.insert(d.begin(), K, 1);
+    int ones = T % K, blocks = T / K;
+    if ((ones + blocks) % 2 != T % 2 && blocks > 0) {
+        blocks--;
+        ones += K;
     }
+    std::vector<int> d(ones, 1);
+    d.insert(d.end(), blocks, K);
     return d;
 }
 
@@ -145,6 +148,8 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         g.kind = prog;
         if (b.star) {
             g.TX = cfg.star_tile_x; g.TY = cfg.star_tile_y;
+            g.rq = g.TX * g.TY == 2 * kBatchThreads && g.TX % 2 == 0 &&
+                   2 * b.radius * (g.TX + g.TY) <= kStarHaloPerThread * kBatchThreads;
             for (int t = 0; t < b.npoints; t++) g.coef[t] = d.stencil.coeffs[t];
         } else {
             g.TX = cfg.box_tile_x; g.TY = cfg.box_tile_y;
@@ -203,10 +208,31 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
     B->timesteps = timesteps;
     std::vector<BTask> all;
     for (int l = 0; l < L; l++) {
-        // Longest tasks first, so the tail of the launch is made of small ones.
-        std::vector<size_t> ord(per[l].size());
-        for (size_t k = 0; k < ord.size(); k++) ord[k] = k;
-        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[l][a] > cost[l][b]; });
+        // Order: within each kind (star / box) longest first, so a launch
+        // ends on small tasks; across kinds, a proportional interleave, so
+        // HBM-bound star tiles and VALU-bound box tiles run side by side on
+        // every CU instead of one phase after the other.
+        std::vector<size_t> kind_ord[2];
+        double kind_cost[2] = {0, 0};
+        for (size_t k = 0; k < per[l].size(); k++) {
+            const int kd = kBuiltins[hg[per[l][k].grid].kind].star ? 0 : 1;
+            kind_ord[kd].push_back(k);
+            kind_cost[kd] += cost[l][k];
+        }
+        for (auto &v : kind_ord)
+            std::stable_sort(v.begin(), v.end(), [&](size_t a, size_t b) { return cost[l][a] > cost[l][b]; });
+        std::vector<size_t> ord;
+        size_t pos[2] = {0, 0};
+        double taken[2] = {0, 0};
+        while (pos[0] < kind_ord[0].size() || pos[1] < kind_ord[1].size()) {
+            int kd;
+            if (pos[0] >= kind_ord[0].size()) kd = 1;
+            else if (pos[1] >= kind_ord[1].size()) kd = 0;
+            else kd = taken[0] / kind_cost[0] <= taken[1] / kind_cost[1] ? 0 : 1;
+            const size_t k = kind_ord[kd][pos[kd]++];
+            taken[kd] += cost[l][k];
+            ord.push_back(k);
+        }
         LaunchRec r{};
         r.depth = l < (int)ssched.size() ? ssched[l] : 1;
         r.first = all.size();
@@ -221,6 +247,8 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
             if (b.star) {
                 const size_t wc = (size_t)(g.TX + 2 * t.depth * R) * (g.TY + 2 * t.depth * R);
                 need = sizeof(double) * t.depth * (2 * R + 1) * wc;
+                if (t.depth == 1 && g.rq) // star3d_tile: two staged planes
+                    need = 2 * sizeof(double) * (size_t)(g.TX + 2 * R) * (g.TY + 2 * R);
             } else {
                 need = 2 * sizeof(double) * (size_t)(g.TX + 2 * R) * (g.TY + 2 * R);
             }

@@ -27,7 +27,8 @@ struct BGrid {
     long long sz;
     int kind;             // built-in stencil id (kBuiltins index)
     int TX, TY;           // tile
-    int pad[2];
+    int rq;               // stars: single steps use the register-queue tile (star3d_tile, 2 points/thread)
+    int pad;
     double coef[UHS_MAX_POINTS]; // term order (stars) or box_pack_index order (boxes)
 };
 
@@ -145,7 +146,20 @@ __device__ __forceinline__ void run_task(const BGrid &g, const BTask &t, double 
     typedef const __attribute__((address_space(4))) double *kptr;
     Coefs<double, S::P> cf((kptr)g.coef);
     if constexpr (S::STAR) {
-        star_tb_tile<S, K>(g, t, lds, cf);
+        if constexpr (K == 1) {
+            // One step: the register-queue star tile when the tile holds
+            // two points per thread, else the LDS-ring tile.  Single-step
+            // launches come first in a grid's schedule, so src always holds
+            // the halo of its own state (src is the ping-pong buffer).
+            if (g.rq) {
+                const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, g.TX / 2, false};
+                star3d_tile<double, S, 2, false, 1>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+            } else {
+                star_tb_tile<S, 1>(g, t, lds, cf);
+            }
+        } else {
+            star_tb_tile<S, K>(g, t, lds, cf);
+        }
     } else {
         const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, g.TX, false};
         box3d_tile<double, S, 1, false>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);

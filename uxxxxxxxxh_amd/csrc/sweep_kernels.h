@@ -35,31 +35,34 @@
 // --------------------------------------------------------------------------
 // star3d
 // --------------------------------------------------------------------------
+// Geometry of one tile's grid: interior extents, strides, tile shape.
+struct TileGeom {
+    int n0, n1, n2;
+    int sy;
+    long long sz;
+    int TX, TY, BXT;
+    bool nts; // non-temporal output stores
+};
+
 template <typename T> __device__ __forceinline__ void store_out(T *p, T v, bool nt)
 {
     if (nt) __builtin_nontemporal_store(v, p);
     else *p = v;
 }
 
-template <typename T, class S, int VX, bool KAHAN, int PF>
-__global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
+// One star3d tile: the tile_x * tile_y column at (x0, y0), output planes
+// [zs, ze).  `in`/`out` point at the interior origin.  Shared by the
+// per-grid kernel and the batched work-queue kernel.
+template <typename T, class S, int VX, bool KAHAN, int PF, class CF>
+__device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restrict__ out,
+                                            const TileGeom &p, int x0, int y0, int zs, int ze,
+                                            T *lds, CF &cf)
 {
     constexpr int R = S::R, P = S::P, Q = 2 * R + 1, NH = kStarHaloPerThread;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-
     const int NT = blockDim.x, tid = threadIdx.x;
     const int BXT = p.BXT, TX = p.TX, TY = p.TY;
     const int LW = TX + 2 * R, LSZ = LW * (TY + 2 * R);
-    int bid = xcd_remap(blockIdx.x, p.nblocks);
-    const int tile_x = bid % p.tiles_x;
-    bid /= p.tiles_x;
-    const int tile_y = bid % p.tiles_y, chunk = bid / p.tiles_y;
-    const int x0 = tile_x * TX, y0 = tile_y * TY;
-    const int zs = chunk * p.ZC, ze = min(zs + p.ZC, p.n0);
     const int tx = tid % BXT, ty = tid / BXT;
-    const T *__restrict__ in = p.in;
-    T *__restrict__ out = p.out;
     const long long sz = p.sz;
     const int sy = p.sy;
 
@@ -100,11 +103,10 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
         }
     }
 
-    Coefs<T, P> cf(p);
     // Register queue: q[v][j] = plane z-R+j at this thread's point v; nq[f]
     // holds plane z+R+f, in flight (PF planes of prefetch).
     T q[VX][Q], nq[PF][VX], nh[NH];
-    const bool nts = p.flags & UHS_CFG_NT_STORE;
+    const bool nts = p.nts;
 #pragma unroll
     for (int v = 0; v < VX; v++) {
 #pragma unroll
@@ -169,6 +171,21 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
     }
 }
 
+template <typename T, class S, int VX, bool KAHAN, int PF>
+__global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int bid = xcd_remap(blockIdx.x, p.nblocks);
+    const int tile_x = bid % p.tiles_x;
+    bid /= p.tiles_x;
+    const int tile_y = bid % p.tiles_y, chunk = bid / p.tiles_y;
+    const int zs = chunk * p.ZC;
+    const TileGeom g{p.n0, p.n1, p.n2, p.sy, p.sz, p.TX, p.TY, p.BXT, (p.flags & UHS_CFG_NT_STORE) != 0};
+    Coefs<T, S::P> cf(p);
+    star3d_tile<T, S, VX, KAHAN, PF>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs, min(zs + p.ZC, p.n0),
+                                     reinterpret_cast<T *>(smem_raw), cf);
+}
+
 // --------------------------------------------------------------------------
 // box3d (terms grouped by ascending axis-0 offset)
 // --------------------------------------------------------------------------
@@ -217,13 +234,6 @@ template <class S> constexpr int box_pack_index(int t)
 // [zs, ze).  `in`/`out` point at the interior origin; CF supplies the
 // coefficients in box_pack_index order.  Shared by the per-grid kernel and
 // the batched work-queue kernel.
-struct TileGeom {
-    int n0, n1, n2;
-    int sy;
-    long long sz;
-    int TX, TY, BXT;
-    bool nts; // non-temporal output stores
-};
 
 template <typename T, class S, int VX, bool KAHAN, class CF>
 __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restrict__ out,
@@ -296,25 +306,35 @@ __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restri
 #pragma unroll
         for (int v = 0; v < VX; v++) {
             if constexpr (uniform_planes<S>()) {
-                // Cell by cell: the value and the NJ coefficients of cell i+1
-                // are fetched while cell i is applied to the NJ pending sums;
-                // pin() keeps the compiler from hoisting every LDS read to the
-                // top of the step (which would hold all G cells live).
+                // Cell by cell: the value and the NJ coefficients of cell
+                // i+PD are fetched (LDS read, scalar loads) while cell i is
+                // applied to the NJ pending sums, so their latency is covered
+                // by PD cells of FMAs; pin() keeps the compiler from hoisting
+                // every LDS read to the top of the step (which would hold all
+                // G cells live).
                 constexpr int G = P / NJ;
-                T cur = L[lc[v] + S::OFF[0][1] * LW + S::OFF[0][2]];
-                T cc[NJ];
-                static_for<NJ>([&](auto gc) {
-                    constexpr int g = decltype(gc)::value;
-                    cc[g] = cf.template get<box_pack_index<S>(g * G)>();
+                constexpr int PD = G < 3 ? G : 3;
+                T vb[PD], cb[PD][NJ];
+                static_for<PD>([&](auto dc) {
+                    constexpr int d = decltype(dc)::value;
+                    vb[d] = L[lc[v] + S::OFF[d][1] * LW + S::OFF[d][2]];
+                    static_for<NJ>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        cb[d][g] = cf.template get<box_pack_index<S>(g * G + d)>();
+                    });
                 });
                 static_for<G>([&](auto ic) {
                     constexpr int i = decltype(ic)::value;
-                    T nxt = cur, nc[NJ];
-                    if constexpr (i + 1 < G) {
-                        nxt = L[lc[v] + S::OFF[i + 1][1] * LW + S::OFF[i + 1][2]];
+                    constexpr int slot = i % PD;
+                    const T cur = vb[slot];
+                    T cc[NJ];
+#pragma unroll
+                    for (int g = 0; g < NJ; g++) cc[g] = cb[slot][g];
+                    if constexpr (i + PD < G) {
+                        vb[slot] = L[lc[v] + S::OFF[i + PD][1] * LW + S::OFF[i + PD][2]];
                         static_for<NJ>([&](auto gc) {
                             constexpr int g = decltype(gc)::value;
-                            nc[g] = cf.template get<box_pack_index<S>(g * G + i + 1)>();
+                            cb[slot][g] = cf.template get<box_pack_index<S>(g * G + i + PD)>();
                         });
                     }
                     static_for<NJ>([&](auto gc) {
@@ -326,9 +346,6 @@ __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restri
                     if constexpr (i + 1 < G) {
 #pragma unroll
                         for (int j = 0; j < NJ; j++) acc[v][j].pin();
-                        cur = nxt;
-#pragma unroll
-                        for (int g = 0; g < NJ; g++) cc[g] = nc[g];
                     }
                 });
             } else {

@@ -121,7 +121,7 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
 {
     std::vector<uhs_config> out;
     if (pl.family == FAM_GATHER) return out;
-    static const int txs3[] = {32, 64, 128, 256};
+    static const int txs3[] = {32, 64, 128, 256, 512};
     static const int tys3[] = {1, 2, 4, 8, 16};
     static const int tzs[] = {16, 32, 64, 128, 256, 512};
     static const int txs2[] = {64, 128, 256, 512, 1024};
@@ -129,7 +129,7 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
     const bool d3 = pl.args.dims == 3;
     const int nflags = pl.family == FAM_STAR3D ? 4 : 2;
     char why[160];
-    for (int tx : (d3 ? std::vector<int>(txs3, txs3 + 4) : std::vector<int>(txs2, txs2 + 5)))
+    for (int tx : (d3 ? std::vector<int>(txs3, txs3 + 5) : std::vector<int>(txs2, txs2 + 5)))
         for (int ty : (d3 ? std::vector<int>(tys3, tys3 + 5) : std::vector<int>{1}))
             for (int tz : tzs)
                 for (int v : vecs)

@@ -61,20 +61,19 @@ def time_sweeps(u, case, ext, T, steps, warmup, seed=1):
 
 
 def copy_bandwidth(u, nbytes=2 << 30, reps=10):
-    """Device-to-device copy rate (read + write bytes / s), for context."""
-    import ctypes
-    a = u.DeviceArray([nbytes // 8], np.float64)
-    b = u.DeviceArray([nbytes // 8], np.float64)
+    """Rate of the library's streaming copy kernel (read + write bytes / s):
+    the HBM bandwidth this box reaches, for context next to the roofline."""
+    n = nbytes // 8
+    a = u.DeviceArray([n], np.float64)
+    b = u.DeviceArray([n], np.float64)
     L = u.lib()
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
     for _ in range(2):
-        hip.hipMemcpyAsync(b.ptr, a.ptr, nbytes, 3, None)
-    L.uhs_synchronize()
+        u.check(L.uhs_copy_f64(b.ptr, a.ptr, n, None))
+    u.synchronize()
     tm = u.Timer()
     tm.start()
     for _ in range(reps):
-        hip.hipMemcpyAsync(b.ptr, a.ptr, nbytes, 3, None)
+        u.check(L.uhs_copy_f64(b.ptr, a.ptr, n, None))
     tm.stop()
     ms = tm.ms()
     a.close()
@@ -204,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only the copy kernel (traffic-counter calibration: 2 GiB read + written)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -212,6 +213,9 @@ def main():
 
     import uxxxxxxxxh_amd as u
     u.lib()
+    if args.copy_only:
+        print(json.dumps({"copy_GBps": round(copy_bandwidth(u), 1)}))
+        return
     name = __import__("ctypes").create_string_buffer(256)
     u.check(u.lib().uhs_device_name(name, 256))
     log("device:", name.value.decode())

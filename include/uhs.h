@@ -74,9 +74,12 @@ typedef struct uhs_stencil {
  *   tile_x  points along the unit-stride axis per workgroup
  *   tile_y  rows per workgroup (3D only; must be 1 for 2D)
  *   tile_z  length of the slab streamed through registers along axis 0
- *   vec     points per thread along the unit-stride axis (1, 2 or 4)
+ *   vec     points per thread along the unit-stride axis (1, 2 or 4; 8 for
+ *           3D stars)
  *   flags   UHS_CFG_NT_STORE: output stores bypass the caches;
- *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight;
+ *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight,
+ *           3D boxes fetch staged cells three ahead instead of one;
+ *           UHS_CFG_PREFETCH4: 3D stars keep four planes in flight;
  *           UHS_CFG_REGTILE: 3D stars use the register-tiled kernel (no LDS,
  *           no barriers): tile_x must be 64 (one wavefront across, 64 - 2r
  *           columns computed), vec = rows per lane (2, 4 or 8), tile_y a
@@ -86,6 +89,7 @@ typedef struct uhs_stencil {
 #define UHS_CFG_NT_STORE 1
 #define UHS_CFG_PREFETCH2 2
 #define UHS_CFG_REGTILE 4
+#define UHS_CFG_PREFETCH4 8
 typedef struct uhs_config {
     int tile_x, tile_y, tile_z, vec, flags;
 } uhs_config;
@@ -198,6 +202,10 @@ int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps,
 
 /* ---- plumbing: device memory, streams, timing ---------------------------- */
 
+/* Streaming device-to-device copy of n doubles, one element per thread: the
+ * attainable-HBM-bandwidth reference the benchmark reports, and the
+ * calibration kernel for the traffic counters. */
+int uhs_copy_f64(double *dst, const double *src, size_t n, uhs_stream_t stream);
 const char *uhs_last_error(void);
 const char *uhs_version(void);
 int uhs_device_count(void);

@@ -79,8 +79,9 @@ typedef struct uhs_mlp *uhs_mlp_t;
  * one-hot (3), star/box one-hot (2), log2 extents (3, axis 0 first, 0 for a
  * missing axis), log2 tile x/y/z (3), log2 threads per workgroup (1),
  * points per thread (1), temporal depth one-hot {1,2,4} (3), non-temporal
- * store flag (1), kernel variant (1: 0 LDS-staged, 1 LDS-staged with
- * two-plane prefetch, 2 register-tiled). */
+ * store flag (1), kernel variant (1: 0 LDS-staged, 1 deeper prefetch (two
+ * planes for stars, three cells for boxes), 2 four-plane prefetch, 3
+ * register-tiled). */
 int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext,
                      const uhs_config *cfg, int depth, float *out20);
 

@@ -102,10 +102,11 @@ def test_dsl_coefficients_and_halo_untouched(gpu, orc, case):
 
 CONFIGS = {
     "star3d": [(64, 8, 64, 2), (32, 4, 16, 1), (128, 4, 8, 4), (64, 2, 5, 2), (256, 2, 3, 4),
-               (128, 16, 9, 4, 2), (64, 8, 6, 2, 3),
+               (128, 16, 9, 4, 2), (64, 8, 6, 2, 3), (256, 8, 7, 8, 9), (512, 4, 33, 8, 1), (64, 8, 5, 4, 8),
                # register-tiled kernel: 64-lane bands, vec rows per lane
                (64, 8, 16, 4, 4), (64, 4, 7, 2, 5), (64, 32, 64, 8, 4), (64, 16, 3, 8, 4), (64, 2, 11, 2, 4)],
-    "box3d": [(64, 4, 128, 1), (32, 8, 7, 2), (16, 4, 3, 1), (128, 2, 11, 2), (32, 16, 5, 1, 1)],
+    "box3d": [(64, 4, 128, 1), (32, 8, 7, 2), (16, 4, 3, 1), (128, 2, 11, 2), (32, 16, 5, 1, 1),
+              (32, 16, 9, 1, 2), (64, 8, 6, 2, 3)],
     "win2d": [(256, 1, 64, 1), (128, 1, 5, 2), (64, 1, 17, 4), (512, 1, 9, 1), (1024, 1, 3, 2, 1)],
 }
 

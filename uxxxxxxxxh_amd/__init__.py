@@ -130,6 +130,7 @@ _SIGS = {
     "uhs_mlp_adam_step": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_float, _I]),
     "uhs_mlp_save": (_I, [_P, ctypes.c_char_p]),
     "uhs_mlp_load": (_I, [_P, ctypes.c_char_p]),
+    "uhs_copy_f64": (_I, [_P, _P, ctypes.c_size_t, _P]),
     "uhs_last_error": (ctypes.c_char_p, []),
     "uhs_version": (ctypes.c_char_p, []),
     "uhs_device_count": (_I, []),

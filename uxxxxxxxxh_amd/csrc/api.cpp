@@ -106,8 +106,9 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
     switch (pl.family) {
     case FAM_STAR3D:
         if (c.flags & UHS_CFG_REGTILE) return uhs_find_star3d_rt(pl.prog, dt, c.vec);
-        return uhs_find_star3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
-    case FAM_BOX3D: return uhs_find_box3d(pl.prog, dt, c.vec);
+        return uhs_find_star3d(pl.prog, dt, c.vec,
+                               (c.flags & UHS_CFG_PREFETCH4) ? 4 : (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
+    case FAM_BOX3D: return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0);
     case FAM_WIN2D: return uhs_find_win2d(pl.prog, dt, c.vec);
     default: return uhs_find_gather(dt);
     }
@@ -127,7 +128,10 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return std::string(buf);
     };
     if (pl.family == FAM_GATHER) return UHS_OK;
-    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE)) return bad("unknown config flags");
+    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4))
+        return bad("unknown config flags");
+    if ((c.flags & UHS_CFG_PREFETCH4) && ((c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE)) || pl.family != FAM_STAR3D))
+        return bad("four-plane prefetch is a 3D-star variant of its own");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
     if (c.flags & UHS_CFG_REGTILE) {
         if (pl.family != FAM_STAR3D) return bad("the register-tiled kernel exists for 3D star stencils only");
@@ -141,7 +145,8 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         if (!uhs_pick_fn(pl, c)) return bad("register-tiled kernel not compiled for this stencil");
         return UHS_OK;
     }
-    if (c.vec != 1 && c.vec != 2 && c.vec != 4) return bad(str("vec must be 1, 2 or 4 (got %lld)", c.vec));
+    if (c.vec != 1 && c.vec != 2 && c.vec != 4 && !(c.vec == 8 && pl.family == FAM_STAR3D))
+        return bad(str("vec must be 1, 2 or 4, or 8 for 3D stars (got %lld)", c.vec));
     if (c.tile_x % c.vec) return bad(str("tile_x %lld not a multiple of vec %lld", c.tile_x, c.vec));
     if (a.dims == 2 && c.tile_y != 1) return bad("2D stencils need tile_y = 1");
     const long long nt = (long long)(c.tile_x / c.vec) * c.tile_y;
@@ -164,8 +169,8 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     }
     if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
     if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
-    if ((c.flags & UHS_CFG_PREFETCH2) && pl.family != FAM_STAR3D)
-        return bad("two-plane prefetch exists for 3D star stencils only");
+    if ((c.flags & UHS_CFG_PREFETCH2) && pl.family != FAM_STAR3D && pl.family != FAM_BOX3D)
+        return bad("deeper prefetch exists for 3D stencils only");
     if (!uhs_pick_fn(pl, c)) return bad(std::string("no ") + family_name(pl.family) + str(" kernel compiled for vec %lld", c.vec));
     return UHS_OK;
 }
@@ -455,6 +460,15 @@ int uhs_error_norms_f64(const double *a, const double *b, int dims, const int64_
                         double out[3], uhs_stream_t stream)
 {
     return norms(a, 0, b, dims, ext, halo, out, (hipStream_t)stream);
+}
+
+int uhs_copy_f64(double *dst, const double *src, size_t n, uhs_stream_t stream)
+{
+    if (!dst || !src) return fail(UHS_E_INVAL, "null pointer");
+    if (n == 0) return UHS_OK;
+    if (n > (size_t)(1LL << 40)) return fail(UHS_E_INVAL, "copy too large");
+    hipError_t e = uhs_launch_copy(src, dst, (long long)n, (hipStream_t)stream);
+    return e == hipSuccess ? UHS_OK : hip_fail(e, "copy");
 }
 
 const char *uhs_last_error(void) { return g_err.c_str(); }

@@ -162,7 +162,7 @@ __device__ __forceinline__ void run_task(const BGrid &g, const BTask &t, double 
         }
     } else {
         const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, g.TX, false};
-        box3d_tile<double, S, 1, false>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+        box3d_tile<double, S, 1, false, 1>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
     }
 }
 

@@ -1,6 +1,6 @@
 // star3d kernel instantiations: the three 3D star benchmark stencils, FP64,
-// FP32 and FP32 compensated, 1/2/4 points per thread along the unit-stride
-// axis, one or two planes of prefetch.
+// FP32 and FP32 compensated, 1/2/4/8 points per thread along the unit-stride
+// axis, one, two or four planes of prefetch.
 #include "sweep_kernels.h"
 
 namespace {
@@ -10,6 +10,7 @@ template <typename T, bool K, class S, int PF> LaunchFn pick_vec(int vec)
     case 1: return launch_star3d<T, S, 1, K, PF>;
     case 2: return launch_star3d<T, S, 2, K, PF>;
     case 4: return launch_star3d<T, S, 4, K, PF>;
+    case 8: return launch_star3d<T, S, 8, K, PF>;
     }
     return nullptr;
 }
@@ -24,7 +25,12 @@ template <class S, int PF> LaunchFn pick_pf(int dtype, int vec)
 }
 template <class S> LaunchFn pick(int dtype, int vec, int prefetch)
 {
-    return prefetch == 2 ? pick_pf<S, 2>(dtype, vec) : pick_pf<S, 1>(dtype, vec);
+    switch (prefetch) {
+    case 1: return pick_pf<S, 1>(dtype, vec);
+    case 2: return pick_pf<S, 2>(dtype, vec);
+    case 4: return pick_pf<S, 4>(dtype, vec);
+    }
+    return nullptr;
 }
 } // namespace
 

@@ -104,7 +104,22 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *partial, int 
     }
 }
 
+// Streaming copy, one element per thread: the reference point for the HBM
+// bandwidth this box reaches (and the calibration target of the traffic
+// counters: it reads and writes exactly n * 8 bytes).
+__global__ void __launch_bounds__(256) k_copy(const double *__restrict__ a, double *__restrict__ b, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 } // namespace
+
+hipError_t uhs_launch_copy(const double *src, double *dst, long long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
 
 hipError_t uhs_launch_fill(void *grid, int is_f32, const GridShape &s, uint64_t seed,
                            hipStream_t st)

@@ -2,9 +2,9 @@
 // FP32 compensated, 1/2/4 points per thread along the unit-stride axis.
 #include "sweep_kernels.h"
 
-LaunchFn uhs_find_box3d_r1(int dtype, int vec);
-LaunchFn uhs_find_box3d_r2(int dtype, int vec);
-LaunchFn uhs_find_box3d_r3(int dtype, int vec);
+LaunchFn uhs_find_box3d_r1(int dtype, int vec, int deep);
+LaunchFn uhs_find_box3d_r2(int dtype, int vec, int deep);
+LaunchFn uhs_find_box3d_r3(int dtype, int vec, int deep);
 
 namespace {
 template <typename T, bool K, class S> LaunchFn pick_vec(int vec)
@@ -42,12 +42,12 @@ LaunchFn uhs_find_win2d(int prog, int dtype, int vec)
     return nullptr;
 }
 
-LaunchFn uhs_find_box3d(int prog, int dtype, int vec)
+LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep)
 {
     switch (prog) {
-    case Prog_3d1r_box::ID: return uhs_find_box3d_r1(dtype, vec);
-    case Prog_3d2r_box::ID: return uhs_find_box3d_r2(dtype, vec);
-    case Prog_3d3r_box::ID: return uhs_find_box3d_r3(dtype, vec);
+    case Prog_3d1r_box::ID: return uhs_find_box3d_r1(dtype, vec, deep);
+    case Prog_3d2r_box::ID: return uhs_find_box3d_r2(dtype, vec, deep);
+    case Prog_3d3r_box::ID: return uhs_find_box3d_r3(dtype, vec, deep);
     }
     return nullptr;
 }

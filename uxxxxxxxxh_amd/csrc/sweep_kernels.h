@@ -235,7 +235,7 @@ template <class S> constexpr int box_pack_index(int t)
 // coefficients in box_pack_index order.  Shared by the per-grid kernel and
 // the batched work-queue kernel.
 
-template <typename T, class S, int VX, bool KAHAN, class CF>
+template <typename T, class S, int VX, bool KAHAN, int PDEEP, class CF>
 __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restrict__ out,
                                            const TileGeom &p, int x0, int y0, int zs, int ze,
                                            T *lds, CF &cf)
@@ -313,7 +313,7 @@ __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restri
                 // every LDS read to the top of the step (which would hold all
                 // G cells live).
                 constexpr int G = P / NJ;
-                constexpr int PD = G < 3 ? G : 3;
+                constexpr int PD = G < PDEEP ? G : PDEEP;
                 T vb[PD], cb[PD][NJ];
                 static_for<PD>([&](auto dc) {
                     constexpr int d = decltype(dc)::value;
@@ -365,7 +365,7 @@ __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restri
     }
 }
 
-template <typename T, class S, int VX, bool KAHAN>
+template <typename T, class S, int VX, bool KAHAN, int PDEEP>
 __global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
     const int zs = chunk * p.ZC;
     const TileGeom g{p.n0, p.n1, p.n2, p.sy, p.sz, p.TX, p.TY, p.BXT, (p.flags & UHS_CFG_NT_STORE) != 0};
     Coefs<T, S::P> cf(p);
-    box3d_tile<T, S, VX, KAHAN>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs,
+    box3d_tile<T, S, VX, KAHAN, PDEEP>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs,
                                 min(zs + p.ZC, p.n0), reinterpret_cast<T *>(smem_raw), cf);
 }
 
@@ -518,14 +518,14 @@ hipError_t launch_star3d(const SweepArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-template <typename T, class S, int VX, bool KAHAN>
+template <typename T, class S, int VX, bool KAHAN, int PDEEP>
 hipError_t launch_box3d(const SweepArgs &a, hipStream_t st)
 {
     auto p = make_params<T, S::P>(a);
     for (int t = 0; t < S::P; t++) p.c[box_pack_index<S>(t)] = (T)a.coef[t];
     const int nt = (a.tx / VX) * a.ty;
     const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R) * (a.ty + 2 * S::R);
-    hipLaunchKernelGGL((k_box3d<T, S, VX, KAHAN>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    hipLaunchKernelGGL((k_box3d<T, S, VX, KAHAN, PDEEP>), dim3(p.nblocks), dim3(nt), lds, st, p);
     return hipGetLastError();
 }
 

@@ -125,15 +125,19 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
     static const int tys3[] = {1, 2, 4, 8, 16};
     static const int tzs[] = {16, 32, 64, 128, 256, 512};
     static const int txs2[] = {64, 128, 256, 512, 1024};
-    static const int vecs[] = {1, 2, 4};
+    const std::vector<int> vecs = pl.family == FAM_STAR3D ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1, 2, 4};
     const bool d3 = pl.args.dims == 3;
-    const int nflags = pl.family == FAM_STAR3D ? 4 : 2;
+    std::vector<int> flagset{0, UHS_CFG_NT_STORE};
+    if (pl.family == FAM_BOX3D) flagset = {0, UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2};
+    if (pl.family == FAM_STAR3D)
+        flagset = {0, UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2, UHS_CFG_PREFETCH4,
+                   UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4};
     char why[160];
     for (int tx : (d3 ? std::vector<int>(txs3, txs3 + 5) : std::vector<int>(txs2, txs2 + 5)))
         for (int ty : (d3 ? std::vector<int>(tys3, tys3 + 5) : std::vector<int>{1}))
             for (int tz : tzs)
                 for (int v : vecs)
-                    for (int fl = 0; fl < nflags; fl++) {
+                    for (int fl : flagset) {
                         uhs_config c{tx, ty, tz, v, fl};
                         if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) continue;
                         // Skip slabs much longer than the streamed axis: they

@@ -39,7 +39,7 @@ typedef hipError_t (*LaunchFn)(const SweepArgs &, hipStream_t);
 // Per-family kernel tables (nullptr when the combination is not compiled).
 LaunchFn uhs_find_star3d(int prog, int dtype, int vec, int prefetch);
 LaunchFn uhs_find_star3d_rt(int prog, int dtype, int rows);
-LaunchFn uhs_find_box3d(int prog, int dtype, int vec);
+LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep);
 LaunchFn uhs_find_win2d(int prog, int dtype, int vec);
 LaunchFn uhs_find_gather(int dtype);
 

@@ -15,3 +15,4 @@ hipError_t uhs_launch_fill(void *grid, int is_f32, const GridShape &s, uint64_t 
 // partial must hold 3*nparts doubles, out3 three doubles (device memory).
 hipError_t uhs_launch_norms(const void *a, int a_is_f32, const double *b, const GridShape &s,
                             double *partial, int nparts, double *out3, hipStream_t st);
+hipError_t uhs_launch_copy(const double *src, double *dst, long long n, hipStream_t st);

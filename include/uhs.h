@@ -80,6 +80,9 @@ typedef struct uhs_stencil {
  *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight,
  *           3D boxes fetch staged cells three ahead instead of one;
  *           UHS_CFG_PREFETCH4: 3D stars keep four planes in flight;
+ *           UHS_CFG_ROWS2: 3D boxes compute two rows per thread (vec must
+ *           be 1, tile_y even; tile_x * tile_y / 2 threads), halving the
+ *           LDS reads per output;
  *           UHS_CFG_REGTILE: 3D stars use the register-tiled kernel (no LDS,
  *           no barriers): tile_x must be 64 (one wavefront across, 64 - 2r
  *           columns computed), vec = rows per lane (2, 4 or 8), tile_y a
@@ -90,6 +93,7 @@ typedef struct uhs_stencil {
 #define UHS_CFG_PREFETCH2 2
 #define UHS_CFG_REGTILE 4
 #define UHS_CFG_PREFETCH4 8
+#define UHS_CFG_ROWS2 16
 typedef struct uhs_config {
     int tile_x, tile_y, tile_z, vec, flags;
 } uhs_config;

@@ -81,7 +81,7 @@ typedef struct uhs_mlp *uhs_mlp_t;
  * points per thread (1), temporal depth one-hot {1,2,4} (3), non-temporal
  * store flag (1), kernel variant (1: 0 LDS-staged, 1 deeper prefetch (two
  * planes for stars, three cells for boxes), 2 four-plane prefetch, 3
- * register-tiled). */
+ * register-tiled, 4 two rows per thread). */
 int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext,
                      const uhs_config *cfg, int depth, float *out20);
 

@@ -106,7 +106,9 @@ CONFIGS = {
                # register-tiled kernel: 64-lane bands, vec rows per lane
                (64, 8, 16, 4, 4), (64, 4, 7, 2, 5), (64, 32, 64, 8, 4), (64, 16, 3, 8, 4), (64, 2, 11, 2, 4)],
     "box3d": [(64, 4, 128, 1), (32, 8, 7, 2), (16, 4, 3, 1), (128, 2, 11, 2), (32, 16, 5, 1, 1),
-              (32, 16, 9, 1, 2), (64, 8, 6, 2, 3)],
+              (32, 16, 9, 1, 2), (64, 8, 6, 2, 3),
+              # two rows per thread
+              (32, 16, 9, 1, 16), (64, 8, 5, 1, 17), (16, 4, 3, 1, 16)],
     "win2d": [(256, 1, 64, 1), (128, 1, 5, 2), (64, 1, 17, 4), (512, 1, 9, 1), (1024, 1, 3, 2, 1)],
 }
 
@@ -145,6 +147,23 @@ def test_register_tiled_star_all_precisions(gpu, orc, case, dtype):
     dt = np.float64 if dtype == "f64" else np.float32
     A0, B0 = A0.astype(dt), B0.astype(dt)
     for cfg in ((64, 8, 5, 4, 4), (64, 16, 64, 2, 5)):
+        gA, gB = run_gpu(u, st, A0, B0, ext, h, 3, dt, dtype == "f32k", cfg=cfg)
+        A, B = A0.copy(), B0.copy()
+        orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, 3, KAHAN if dtype == "f32k" else FUSED)
+        assert np.array_equal(gA, A) and np.array_equal(gB, B), cfg
+
+
+@pytest.mark.parametrize("case", ["3d1r_box", "3d2r_box", "3d3r_box"])
+@pytest.mark.parametrize("dtype", ["f64", "f32", "f32k"])
+def test_box_two_rows_all_precisions(gpu, orc, case, dtype):
+    u = gpu
+    s = SHAPES[case]
+    st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 14))
+    ext, h = [11, 27, 45], s["radius"]
+    A0, B0 = random_grids(ext, h, 15)
+    dt = np.float64 if dtype == "f64" else np.float32
+    A0, B0 = A0.astype(dt), B0.astype(dt)
+    for cfg in ((32, 8, 4, 1, 16), (16, 16, 64, 1, 17)):
         gA, gB = run_gpu(u, st, A0, B0, ext, h, 3, dt, dtype == "f32k", cfg=cfg)
         A, B = A0.copy(), B0.copy()
         orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, 3, KAHAN if dtype == "f32k" else FUSED)

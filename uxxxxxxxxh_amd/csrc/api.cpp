@@ -108,7 +108,9 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
         if (c.flags & UHS_CFG_REGTILE) return uhs_find_star3d_rt(pl.prog, dt, c.vec);
         return uhs_find_star3d(pl.prog, dt, c.vec,
                                (c.flags & UHS_CFG_PREFETCH4) ? 4 : (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
-    case FAM_BOX3D: return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0);
+    case FAM_BOX3D:
+        if (c.flags & UHS_CFG_ROWS2) return c.vec == 1 ? uhs_find_box3d(pl.prog, dt, 0, 0) : nullptr;
+        return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0);
     case FAM_WIN2D: return uhs_find_win2d(pl.prog, dt, c.vec);
     default: return uhs_find_gather(dt);
     }
@@ -128,8 +130,13 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return std::string(buf);
     };
     if (pl.family == FAM_GATHER) return UHS_OK;
-    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4))
+    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4 | UHS_CFG_ROWS2))
         return bad("unknown config flags");
+    if (c.flags & UHS_CFG_ROWS2) {
+        if (pl.family != FAM_BOX3D) return bad("two rows per thread exists for 3D boxes only");
+        if (c.vec != 1 || (c.tile_y & 1) || (c.flags & UHS_CFG_PREFETCH2))
+            return bad("two rows per thread needs vec 1, an even tile_y and no deeper prefetch");
+    }
     if ((c.flags & UHS_CFG_PREFETCH4) && ((c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE)) || pl.family != FAM_STAR3D))
         return bad("four-plane prefetch is a 3D-star variant of its own");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
@@ -149,7 +156,7 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return bad(str("vec must be 1, 2 or 4, or 8 for 3D stars (got %lld)", c.vec));
     if (c.tile_x % c.vec) return bad(str("tile_x %lld not a multiple of vec %lld", c.tile_x, c.vec));
     if (a.dims == 2 && c.tile_y != 1) return bad("2D stencils need tile_y = 1");
-    const long long nt = (long long)(c.tile_x / c.vec) * c.tile_y;
+    const long long nt = (long long)(c.tile_x / c.vec) * c.tile_y / ((c.flags & UHS_CFG_ROWS2) ? 2 : 1);
     if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
     if (nt < 2 * R) return bad(str("%lld threads cannot stage a radius-%lld halo", nt, R));
     const size_t esz = a.dtype == DT_F64 ? 8 : 4;

@@ -6,6 +6,7 @@
 namespace {
 template <typename T, bool K, class S> LaunchFn pick_vec(int vec, int deep)
 {
+    if (vec == 0) return launch_box3d_rows2<T, S, K>; // two rows per thread
     switch (vec * 2 + deep) {
     case 2: return launch_box3d<T, S, 1, K, 1>;
     case 3: return launch_box3d<T, S, 1, K, 3>;

@@ -272,7 +272,8 @@ int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext, const u
     f[14] = (float)c->vec;
     f[15 + (depth >= 4 ? 2 : depth >= 2 ? 1 : 0)] = 1.f;
     f[18] = (c->flags & UHS_CFG_NT_STORE) ? 1.f : 0.f;
-    f[19] = (c->flags & UHS_CFG_REGTILE) ? 3.f : (c->flags & UHS_CFG_PREFETCH4) ? 2.f : (c->flags & UHS_CFG_PREFETCH2) ? 1.f : 0.f;
+    f[19] = (c->flags & UHS_CFG_ROWS2) ? 4.f : (c->flags & UHS_CFG_REGTILE) ? 3.f : (c->flags & UHS_CFG_PREFETCH4) ? 2.f
+          : (c->flags & UHS_CFG_PREFETCH2) ? 1.f : 0.f;
     return UHS_OK;
 }
 

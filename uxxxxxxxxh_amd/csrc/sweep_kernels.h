@@ -365,6 +365,176 @@ __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restri
     }
 }
 
+// True when every plane group lists its cells row by row (ascending
+// axis-1 offset), every row with the same sequence of axis-2 offsets -- the
+// benchmark boxes.  Then a value staged at row y+dy is cell (dy, dx) of
+// output row y and cell (dy-1, dx) of output row y+1, and visiting staged
+// rows in ascending order feeds both outputs their cells in term order.
+template <class S> constexpr bool row_grouped()
+{
+    if (!uniform_planes<S>()) return false;
+    constexpr int NJ = S::OFF[S::P - 1][0] - S::OFF[0][0] + 1;
+    constexpr int G = S::P / NJ, W = 2 * S::R + 1;
+    if (G != W * W) return false;
+    for (int i = 0; i < G; i++) {
+        if (S::OFF[i][1] != -S::R + i / W) return false;
+        if (S::OFF[i][2] != S::OFF[i % W][2]) return false;
+    }
+    return true;
+}
+
+// box3d with two output rows per thread (vec 1): every staged value is read
+// from LDS once and applied to the pending sums of both rows, which halves
+// the LDS reads per output -- the boxes' other bottleneck next to FP64 FMA.
+template <typename T, class S, bool KAHAN, class CF>
+__device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__restrict__ out,
+                                                 const TileGeom &p, int x0, int y0, int zs, int ze,
+                                                 T *lds, CF &cf)
+{
+    static_assert(row_grouped<S>(), "two-row box3d needs row-grouped plane cells");
+    constexpr int R = S::R, P = S::P, NL = kBoxCellsPerThread, W = 2 * R + 1;
+    constexpr int DZMIN = S::OFF[0][0], DZMAX = S::OFF[P - 1][0];
+    constexpr int NJ = DZMAX - DZMIN + 1, G = P / NJ;
+    const int NT = blockDim.x, tid = threadIdx.x;
+    const int TX = p.TX, TY = p.TY;
+    const int LW = TX + 2 * R, LSZ = LW * (TY + 2 * R);
+    const int tx = tid % TX, ty = 2 * (tid / TX);
+    const long long sz = p.sz;
+    const int sy = p.sy;
+
+    const int xg = x0 + tx, yg = y0 + ty;
+    bool act[2];
+    int coff[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        act[r] = xg < p.n2 && yg + r < p.n1;
+        coff[r] = act[r] ? (yg + r) * sy + xg : 0;
+    }
+    const int lc = (ty + R) * LW + tx + R;
+    int goff[NL];
+    bool gv[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int c = tid + k * NT;
+        gv[k] = c < LSZ;
+        const int ly = gv[k] ? c / LW : 0, lx = gv[k] ? c % LW : 0;
+        goff[k] = clampi(y0 - R + ly, -R, p.n1 + R - 1) * sy + clampi(x0 - R + lx, -R, p.n2 + R - 1);
+    }
+    T pl[NL];
+    {
+        const T *pp = in + (long long)(zs + DZMIN) * sz;
+#pragma unroll
+        for (int k = 0; k < NL; k++)
+            if (gv[k]) pl[k] = pp[goff[k]];
+    }
+    Acc<T, KAHAN> acc[2][NJ];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++) acc[r][j].zero();
+    const int steps = (ze - zs) + NJ - 1;
+    for (int s = 0; s < steps; ++s) {
+        const int zp = zs + DZMIN + s;
+        T *L = lds + (s & 1) * LSZ;
+#pragma unroll
+        for (int k = 0; k < NL; k++)
+            if (gv[k]) L[tid + k * NT] = pl[k];
+        {
+            const T *pn = in + (long long)min(zp + 1, p.n0 + R - 1) * sz;
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+                if (gv[k]) pl[k] = pn[goff[k]];
+        }
+        __syncthreads();
+        cf.refresh();
+        const int zo = zp - DZMAX;
+        // Staged rows j = 0..W (offset j - R from row 0), columns k in the
+        // rows' axis-2 order; item u = j * W + k.  The value and coefficients
+        // of item u+1 are fetched while item u is applied.
+        constexpr int NU = (W + 1) * W;
+        auto value = [&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            return L[lc + (u / W - R) * LW + S::OFF[u % W][2]];
+        };
+        auto coefs = [&](auto uc, T (&c)[2][NJ]) {
+            constexpr int u = decltype(uc)::value, j = u / W, k = u % W;
+            static_for<NJ>([&](auto gc) {
+                constexpr int g = decltype(gc)::value;
+                if constexpr (j < W) c[0][g] = cf.template get<box_pack_index<S>(g * G + j * W + k)>();
+                if constexpr (j >= 1) c[1][g] = cf.template get<box_pack_index<S>(g * G + (j - 1) * W + k)>();
+            });
+        };
+        T cur = value(std::integral_constant<int, 0>{});
+        T cc[2][NJ];
+        coefs(std::integral_constant<int, 0>{}, cc);
+        static_for<NU>([&](auto uc) {
+            constexpr int u = decltype(uc)::value, j = u / W, k = u % W;
+            T nxt = cur, nc[2][NJ];
+            if constexpr (u + 1 < NU) {
+                nxt = value(std::integral_constant<int, u + 1>{});
+                coefs(std::integral_constant<int, u + 1>{}, nc);
+            }
+            static_for<NJ>([&](auto gc) {
+                constexpr int g = decltype(gc)::value;
+                if constexpr (j < W) { // cell (j - R, k) of row 0
+                    constexpr int t = g * G + j * W + k;
+                    if constexpr (t == 0) acc[0][NJ - 1 - g].first(cc[0][g], cur);
+                    else acc[0][NJ - 1 - g].add(cc[0][g], cur);
+                }
+                if constexpr (j >= 1) { // cell (j - 1 - R, k) of row 1
+                    constexpr int t = g * G + (j - 1) * W + k;
+                    if constexpr (t == 0) acc[1][NJ - 1 - g].first(cc[1][g], cur);
+                    else acc[1][NJ - 1 - g].add(cc[1][g], cur);
+                }
+            });
+            if constexpr (u + 1 < NU) {
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+#pragma unroll
+                    for (int q = 0; q < NJ; q++) acc[r][q].pin();
+                cur = nxt;
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+#pragma unroll
+                    for (int g = 0; g < NJ; g++) cc[r][g] = nc[r][g];
+            }
+        });
+        T *oz = out + (long long)zo * sz;
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            if (zo >= zs && act[r]) store_out(oz + coff[r], acc[r][0].value(), p.nts);
+#pragma unroll
+            for (int j = 0; j + 1 < NJ; j++) acc[r][j] = acc[r][j + 1];
+        }
+    }
+}
+
+template <typename T, class S, bool KAHAN>
+__global__ void __launch_bounds__(kMaxThreads) k_box3d_rows2(KParams<T, S::P> p)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int bid = xcd_remap(blockIdx.x, p.nblocks);
+    const int tile_x = bid % p.tiles_x;
+    bid /= p.tiles_x;
+    const int tile_y = bid % p.tiles_y, chunk = bid / p.tiles_y;
+    const int zs = chunk * p.ZC;
+    const TileGeom g{p.n0, p.n1, p.n2, p.sy, p.sz, p.TX, p.TY, p.TX, (p.flags & UHS_CFG_NT_STORE) != 0};
+    Coefs<T, S::P> cf(p);
+    box3d_tile_rows2<T, S, KAHAN>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs, min(zs + p.ZC, p.n0),
+                                  reinterpret_cast<T *>(smem_raw), cf);
+}
+
+template <typename T, class S, bool KAHAN>
+hipError_t launch_box3d_rows2(const SweepArgs &a, hipStream_t st)
+{
+    auto p = make_params<T, S::P>(a);
+    for (int t = 0; t < S::P; t++) p.c[box_pack_index<S>(t)] = (T)a.coef[t];
+    const int nt = a.tx * (a.ty / 2);
+    const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R) * (a.ty + 2 * S::R);
+    hipLaunchKernelGGL((k_box3d_rows2<T, S, KAHAN>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    return hipGetLastError();
+}
+
 template <typename T, class S, int VX, bool KAHAN, int PDEEP>
 __global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
 {

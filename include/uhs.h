@@ -86,7 +86,15 @@ typedef struct uhs_stencil {
  *           UHS_CFG_REGTILE: 3D stars use the register-tiled kernel (no LDS,
  *           no barriers): tile_x must be 64 (one wavefront across, 64 - 2r
  *           columns computed), vec = rows per lane (2, 4 or 8), tile_y a
- *           multiple of vec (tile_y / vec wavefronts per workgroup)
+ *           multiple of vec (tile_y / vec wavefronts per workgroup);
+ *           UHS_CFG_TEMPORAL2: 3D stars advance two timesteps per launch
+ *           (temporal blocking, each cell read and written once per two
+ *           updates): the workgroup covers the tile_x * tile_y output tile
+ *           plus an r-wide ghost zone, (tile_x + 2r) / vec * (tile_y + 2r)
+ *           threads, vec 1 or 2 dividing tile_x + 2r.  Up to three leading
+ *           timesteps run as single sweeps (default configuration) so the
+ *           result lands in the grid the one-step ping-pong leaves it in;
+ *           results are bitwise those of single sweeps.
  * Threads per workgroup = tile_x / vec * tile_y (64 * tile_y / vec with
  * UHS_CFG_REGTILE). */
 #define UHS_CFG_NT_STORE 1
@@ -94,6 +102,7 @@ typedef struct uhs_stencil {
 #define UHS_CFG_REGTILE 4
 #define UHS_CFG_PREFETCH4 8
 #define UHS_CFG_ROWS2 16
+#define UHS_CFG_TEMPORAL2 32
 typedef struct uhs_config {
     int tile_x, tile_y, tile_z, vec, flags;
 } uhs_config;

@@ -170,6 +170,53 @@ def test_box_two_rows_all_precisions(gpu, orc, case, dtype):
         assert np.array_equal(gA, A) and np.array_equal(gB, B), cfg
 
 
+@pytest.mark.parametrize("case", ["3d1r_star", "3d2r_star", "3d3r_star"])
+@pytest.mark.parametrize("dtype", ["f64", "f32", "f32k"])
+def test_two_steps_per_launch(gpu, orc, case, dtype):
+    """UHS_CFG_TEMPORAL2: the result grid (A for even T, B for odd T) is
+    bitwise the oracle's; the halos of both grids, which differ and are
+    nonzero, are untouched; every state's halo is read from the grid the
+    one-step ping-pong holds it in.  T = 1..6 covers 0-3 leading single
+    steps and 0-2 two-step launches."""
+    u = gpu
+    s = SHAPES[case]
+    R = s["radius"]
+    st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 16))
+    dt = np.float64 if dtype == "f64" else np.float32
+    T2, NT = u.CFG_TEMPORAL2, u.CFG_NT_STORE
+    for ext, h in (([19, 37, 70], R), ([9, 14, 23], R + 1)):
+        A0, B0 = random_grids(ext, h, 17)
+        A0, B0 = A0.astype(dt), B0.astype(dt)
+        mask = np.ones(A0.shape, bool)
+        mask[tuple(slice(h, n - h) for n in A0.shape)] = False
+        for cfg in ((8, 4, 5, 2, T2), (30, 6, 7, 1, T2 | NT), (62, 6, 64, 2, T2)):
+            for T in (1, 2, 3, 4, 5, 6):
+                log(case, dtype, ext, h, cfg, "T", T)
+                gA, gB = run_gpu(u, st, A0, B0, ext, h, T, dt, dtype == "f32k", cfg=cfg)
+                A, B = A0.copy(), B0.copy()
+                orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, T, KAHAN if dtype == "f32k" else FUSED)
+                got, want = (gA, A) if T % 2 == 0 else (gB, B)
+                assert np.array_equal(got, want), (ext, h, cfg, T)
+                assert np.array_equal(gA[mask], A0[mask]) and np.array_equal(gB[mask], B0[mask])
+
+
+def test_two_steps_per_launch_config_checks(gpu):
+    u = gpu
+    st = u.builtin("3d2r_star")
+    T2 = u.CFG_TEMPORAL2
+    ext = [64, 64, 64]
+    assert u.plan_kind(ext, 2, st, cfg=(60, 4, 64, 2, T2)) == "star3d"
+    for bad in ((61, 4, 64, 2, T2),                    # tile_x + 2r odd with vec 2
+                (64, 8, 64, 4, T2),                    # vec 4
+                (124, 8, 64, 2, T2),                   # 64 * 12 threads
+                (60, 4, 64, 2, T2 | u.CFG_PREFETCH2)):  # no other variant flags
+        with pytest.raises(u.UhsError) as e:
+            u.plan_kind(ext, 2, st, cfg=bad)
+        assert e.value.code == -3
+    with pytest.raises(u.UhsError):
+        u.plan_kind(ext, 2, u.builtin("3d2r_box"), cfg=(28, 4, 64, 1, T2))
+
+
 @pytest.mark.parametrize("case", CASES)
 @pytest.mark.parametrize("compensated", [False, True])
 def test_f32_parity(gpu, orc, case, compensated):

@@ -105,6 +105,7 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
     const int dt = pl.args.dtype;
     switch (pl.family) {
     case FAM_STAR3D:
+        if (c.flags & UHS_CFG_TEMPORAL2) return uhs_find_star3d_tb2(pl.prog, dt, c.vec);
         if (c.flags & UHS_CFG_REGTILE) return uhs_find_star3d_rt(pl.prog, dt, c.vec);
         return uhs_find_star3d(pl.prog, dt, c.vec,
                                (c.flags & UHS_CFG_PREFETCH4) ? 4 : (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
@@ -130,8 +131,28 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return std::string(buf);
     };
     if (pl.family == FAM_GATHER) return UHS_OK;
-    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4 | UHS_CFG_ROWS2))
+    if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4 | UHS_CFG_ROWS2 |
+                    UHS_CFG_TEMPORAL2))
         return bad("unknown config flags");
+    if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
+    if (c.flags & UHS_CFG_TEMPORAL2) {
+        if (pl.family != FAM_STAR3D) return bad("two steps per launch exists for 3D star stencils only");
+        if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE))
+            return bad("two steps per launch combines with UHS_CFG_NT_STORE only");
+        const long long bx = c.tile_x + 2LL * R, by = c.tile_y + 2LL * R;
+        if (c.vec != 1 && c.vec != 2) return bad(str("two steps per launch needs vec 1 or 2 (got %lld)", c.vec));
+        if (bx % c.vec) return bad(str("tile_x + 2r = %lld not a multiple of vec %lld", bx, c.vec));
+        const long long nt = bx / c.vec * by;
+        if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
+        if (ceil_div(2LL * R * (bx + by), nt) > kStarHaloPerThread)
+            return bad(str("ring of %lld cells needs more than %lld per thread", 2 * R * (bx + by), kStarHaloPerThread));
+        const size_t esz = a.dtype == DT_F64 ? 8 : 4;
+        const size_t lds = 2 * esz * (size_t)((bx + 2 * R) * (by + 2 * R) + bx * by);
+        if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
+        if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
+        if (!uhs_pick_fn(pl, c)) return bad("two-step kernel not compiled for this stencil");
+        return UHS_OK;
+    }
     if (c.flags & UHS_CFG_ROWS2) {
         if (pl.family != FAM_BOX3D) return bad("two rows per thread exists for 3D boxes only");
         if (c.vec != 1 || (c.tile_y & 1) || (c.flags & UHS_CFG_PREFETCH2))
@@ -139,7 +160,6 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     }
     if ((c.flags & UHS_CFG_PREFETCH4) && ((c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE)) || pl.family != FAM_STAR3D))
         return bad("four-plane prefetch is a 3D-star variant of its own");
-    if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
     if (c.flags & UHS_CFG_REGTILE) {
         if (pl.family != FAM_STAR3D) return bad("the register-tiled kernel exists for 3D star stencils only");
         if (c.flags & UHS_CFG_PREFETCH2) return bad("the register-tiled kernel has no two-plane prefetch variant");
@@ -291,7 +311,31 @@ int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
     pl->cfg = c;
     pl->fn = uhs_pick_fn(*pl, c);
     if (!pl->fn) return fail(UHS_E_UNSUPPORTED, "no kernel for this stencil and precision");
+    if (c.flags & UHS_CFG_TEMPORAL2) {
+        Plan one = *pl;
+        const uhs_config c1 = uhs_heuristic_config(one);
+        one.args.tx = c1.tile_x; one.args.ty = c1.tile_y; one.args.tz = c1.tile_z;
+        one.args.vec = c1.vec; one.args.flags = c1.flags;
+        pl->fn1 = uhs_pick_fn(one, c1);
+        pl->args1 = one.args;
+        if (!pl->fn1) return fail(UHS_E_UNSUPPORTED, "no single-step kernel for the leading steps");
+    }
     return UHS_OK;
+}
+
+// Steps advanced per launch of a UHS_CFG_TEMPORAL2 plan: `ones` single
+// steps, then `twos` two-step launches.  Every launch reads one grid and
+// writes the other, so the result lands where the one-step ping-pong leaves
+// it (A for even T, B for odd T) when the launch count has T's parity, i.e.
+// when `twos` is even.
+static void temporal2_schedule(int T, int *ones, int *twos)
+{
+    *twos = T / 2;
+    *ones = T % 2;
+    if (*twos % 2) {
+        (*twos)--;
+        *ones += 2;
+    }
 }
 
 int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream)
@@ -299,6 +343,29 @@ int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream)
     if (!A || !B) return fail(UHS_E_INVAL, "null grid pointer");
     if (A == B) return fail(UHS_E_INVAL, "A and B must be distinct grids");
     if (timesteps < 0) return fail(UHS_E_INVAL, "negative timestep count");
+    if (pl.cfg.flags & UHS_CFG_TEMPORAL2) {
+        int ones, twos;
+        temporal2_schedule(timesteps, &ones, &twos);
+        for (int k = 1; k <= ones; k++) {
+            pl.args1.in = (k & 1) ? A : B;
+            pl.args1.out = (k & 1) ? B : A;
+            hipError_t e = pl.fn1(pl.args1, stream);
+            if (e != hipSuccess) return hip_fail(e, "sweep launch");
+        }
+        // Launch j reads the grid written by launch j-1; halo cells of each
+        // state come from the grid the one-step ping-pong would hold it in.
+        for (int j = 0; j < twos; j++) {
+            const int l = ones + j; // launches before this one
+            pl.args.in = (l & 1) ? B : A;
+            pl.args.out = (l & 1) ? A : B;
+            pl.args.hA = A;
+            pl.args.hB = B;
+            pl.args.k0 = ones + 2 * j;
+            hipError_t e = pl.fn(pl.args, stream);
+            if (e != hipSuccess) return hip_fail(e, "sweep launch");
+        }
+        return UHS_OK;
+    }
     for (int k = 1; k <= timesteps; k++) {
         pl.args.in = (k & 1) ? A : B;
         pl.args.out = (k & 1) ? B : A;

@@ -14,6 +14,9 @@ struct Plan {
     uhs_config cfg;
     LaunchFn fn;
     SweepArgs args;
+    // UHS_CFG_TEMPORAL2: the single-step sweep that runs the leading steps.
+    LaunchFn fn1;
+    SweepArgs args1;
 };
 
 int uhs_set_error(int code, const char *msg);

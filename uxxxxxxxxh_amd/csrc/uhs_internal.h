@@ -32,6 +32,11 @@ struct SweepArgs {
     const int *off;        // host copy of the offsets (gather path)
     const double *coef;    // host copy of the coefficients
     int dtype;
+    // Multi-step kernels only: the grids the one-step ping-pong calls A and B
+    // (base pointers; their halos hold every state's halo) and the global
+    // step of the input state.
+    const void *hA, *hB;
+    int k0;
 };
 
 typedef hipError_t (*LaunchFn)(const SweepArgs &, hipStream_t);
@@ -39,6 +44,7 @@ typedef hipError_t (*LaunchFn)(const SweepArgs &, hipStream_t);
 // Per-family kernel tables (nullptr when the combination is not compiled).
 LaunchFn uhs_find_star3d(int prog, int dtype, int vec, int prefetch);
 LaunchFn uhs_find_star3d_rt(int prog, int dtype, int rows);
+LaunchFn uhs_find_star3d_tb2(int prog, int dtype, int vec); // two steps per launch
 LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep);
 LaunchFn uhs_find_win2d(int prog, int dtype, int vec);
 LaunchFn uhs_find_gather(int dtype);

@@ -34,8 +34,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_sweeps(u, case, ext, T, steps, warmup, seed=1):
-    """Returns (ms per step, ms per sweep launch, config) for uhs_sweep_f64."""
+def time_sweeps(u, case, ext, T, steps, warmup, seed=1, cfg=None):
+    """Returns (ms per step, ms per sweep, config) for uhs_sweep_f64 (cfg None:
+    the library's default configuration)."""
     st = u.builtin(case)
     st = st.with_coeffs(u.make_coeffs(st, seed))
     h = st.radius
@@ -44,20 +45,45 @@ def time_sweeps(u, case, ext, T, steps, warmup, seed=1):
     B = u.DeviceArray(shape, np.float64)
     u.fill_uniform(A, ext, h, seed)
     u.fill_uniform(B, ext, h, seed + 1)
-    cfg = u.default_config(ext, h, st)
+    run_cfg = cfg
+    cfg = cfg if cfg is not None else u.default_config(ext, h, st)
     for _ in range(warmup):
-        u.sweep_f64(A, B, ext, h, st, T)
+        u.sweep_f64(A, B, ext, h, st, T, run_cfg)
     u.synchronize()
     tm = u.Timer()
     tm.start()
     for _ in range(steps):
-        u.sweep_f64(A, B, ext, h, st, T)
+        u.sweep_f64(A, B, ext, h, st, T, run_cfg)
     tm.stop()
     ms = tm.ms()
     u.synchronize()
     A.close()
     B.close()
     return ms / steps, ms / (steps * T), cfg
+
+
+# Two steps per launch (temporal blocking), the fastest configuration of a
+# timed sweep at 512^3: 62x6 output tile (64x8 with the ghost zone), 2 columns
+# per thread, 128-plane slabs, non-temporal stores.
+TEMPORAL2_CFG = (62, 6, 128, 2, 1 | 32)
+
+
+def temporal2(u, n, T):
+    """The same workload with UHS_CFG_TEMPORAL2: GStencils/s, and the DRAM
+    bytes per two-step launch measured by the traffic counters (committed
+    summary), beside the algorithmic 16 B per update."""
+    ext = [n, n, n]
+    ms_step, ms_sweep, cfg = time_sweeps(u, "3d1r_star", ext, T, 2, 1, cfg=TEMPORAL2_CFG)
+    upd = float(n) ** 3
+    g = upd / (ms_sweep * 1e-3) / 1e9
+    out = {"workload": "3d1r_star FP64 %d^3, T=%d, two steps per launch" % (n, T),
+           "config": list(TEMPORAL2_CFG), "GStencils_s": round(g, 2),
+           "algorithmic_GBps_at_16B": round(16.0 * g, 1), "frac_of_8TBps_at_16B": round(16.0 * g / PEAK_HBM_GBPS, 4)}
+    tr = load_traffic("3d1r_star_tb2", ext)
+    if tr:
+        out["dram_bytes_per_launch"] = tr["bytes_per_launch"]
+        out["dram_GBps"] = round(tr["bytes_per_launch"] / (2 * ms_sweep * 1e-3) / 1e9, 1)
+    return out
 
 
 def copy_bandwidth(u, nbytes=2 << 30, reps=10):
@@ -133,7 +159,7 @@ def batched(u, T=50, probe_T=8):
             "batched_ms": round(ms_batch, 3), "batched_GStencils_s": round(upd / ms_batch / 1e6, 2),
             "speedup": round(ms_plain / ms_batch, 3),
             "config": {"star_depth": best.star_depth, "star_tile": [best.star_tile_x, best.star_tile_y, best.star_tile_z],
-                       "box_tile": [best.box_tile_x, best.box_tile_y, best.box_tile_z]},
+                       "box_tile": [best.box_tile_x, best.box_tile_y, best.box_tile_z], "flags": best.flags},
             "tuning_candidates": len(cands), "tuning_s": round(t_tune, 2), "launches": info["launches"],
             "tasks": info["tasks"]}
 
@@ -203,6 +229,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-temporal", action="store_true")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the copy kernel (traffic-counter calibration: 2 GiB read + written)")
     args = ap.parse_args()
@@ -273,6 +300,9 @@ def main():
                          "tile": [c.tile_x, c.tile_y, c.tile_z, c.vec, c.flags]}
             log("%-10s %8.2f GSt/s  %.3f ms/sweep  roof %.0f  frac %.3f" % (case, g, ms_w, roof, g / roof))
         out["per_shape"] = per
+    if not args.no_temporal:
+        out["temporal2"] = temporal2(u, n, T)
+        log("temporal2", out["temporal2"])
     if not args.no_batch:
         out["batched"] = batched(u)
         log("batched", out["batched"])

@@ -77,9 +77,11 @@ typedef struct uhs_stencil {
  *   vec     points per thread along the unit-stride axis (1, 2 or 4; 8 for
  *           3D stars)
  *   flags   UHS_CFG_NT_STORE: output stores bypass the caches;
- *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight,
- *           3D boxes fetch staged cells three ahead instead of one;
- *           UHS_CFG_PREFETCH4: 3D stars keep four planes in flight;
+ *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight
+ *           (2D stencils two rows), 3D boxes fetch staged cells three ahead
+ *           instead of one;
+ *           UHS_CFG_PREFETCH4: 3D stars keep four planes in flight (2D
+ *           stencils four rows);
  *           UHS_CFG_ROWS2: 3D boxes compute two rows per thread (vec must
  *           be 1, tile_y even; tile_x * tile_y / 2 threads), halving the
  *           LDS reads per output;
@@ -87,6 +89,8 @@ typedef struct uhs_stencil {
  *           no barriers): tile_x must be 64 (one wavefront across, 64 - 2r
  *           columns computed), vec = rows per lane (2, 4 or 8), tile_y a
  *           multiple of vec (tile_y / vec wavefronts per workgroup);
+ *           UHS_CFG_PLANES2: 3D boxes keep two planes of tile loads in
+ *           flight instead of one (not with UHS_CFG_PREFETCH2);
  *           UHS_CFG_TEMPORAL2: 3D stars advance two timesteps per launch
  *           (temporal blocking, each cell read and written once per two
  *           updates): the workgroup covers the tile_x * tile_y output tile
@@ -94,7 +98,9 @@ typedef struct uhs_stencil {
  *           threads, vec 1 or 2 dividing tile_x + 2r.  Up to three leading
  *           timesteps run as single sweeps (default configuration) so the
  *           result lands in the grid the one-step ping-pong leaves it in;
- *           results are bitwise those of single sweeps.
+ *           results are bitwise those of single sweeps.  The other grid's
+ *           interior then holds state T-2, not T-1, when a two-step
+ *           launch ran.
  * Threads per workgroup = tile_x / vec * tile_y (64 * tile_y / vec with
  * UHS_CFG_REGTILE). */
 #define UHS_CFG_NT_STORE 1
@@ -103,6 +109,7 @@ typedef struct uhs_stencil {
 #define UHS_CFG_PREFETCH4 8
 #define UHS_CFG_ROWS2 16
 #define UHS_CFG_TEMPORAL2 32
+#define UHS_CFG_PLANES2 64
 typedef struct uhs_config {
     int tile_x, tile_y, tile_z, vec, flags;
 } uhs_config;
@@ -184,13 +191,24 @@ typedef struct uhs_grid_desc {
     uhs_stencil stencil;
 } uhs_grid_desc;
 
-/* Tiling of the batched launches.  Star stencils are advanced star_depth
- * steps per launch (temporal blocking with ghost zones, 1, 2 or 4); box
- * stencils one step per launch.  box_tile_x * box_tile_y must be 256. */
+/* Tiling of the batched launches (256 threads per workgroup).  Star
+ * stencils are advanced star_depth steps per launch (temporal blocking with
+ * ghost zones, 1, 2 or 4); box stencils one step per launch.
+ *   Stars: a 512-point tile (2 points per thread) runs single steps in
+ *   registers; with star_depth 2, a tile whose extension by the radius r on
+ *   each side, (tile_x + 2r) / 2 * (tile_y + 2r), fits 256 threads (tile_x
+ *   even) runs the two-step register-queue kernel; other tiles use the LDS
+ *   ring form.
+ *   Boxes: box_tile_x * box_tile_y must be 256, or 512 with
+ *   UHS_BATCH_BOX_ROWS2 (two rows per thread, box_tile_y even);
+ *   UHS_BATCH_BOX_PLANES2 keeps two planes of tile loads in flight. */
+#define UHS_BATCH_BOX_ROWS2 1
+#define UHS_BATCH_BOX_PLANES2 2
 typedef struct uhs_batch_config {
     int star_depth;
     int star_tile_x, star_tile_y, star_tile_z;
     int box_tile_x, box_tile_y, box_tile_z;
+    int flags; /* UHS_BATCH_* */
 } uhs_batch_config;
 
 typedef struct uhs_batch *uhs_batch_t;

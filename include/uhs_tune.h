@@ -78,10 +78,13 @@ typedef struct uhs_mlp *uhs_mlp_t;
 /* Feature vector of (problem, configuration): dims one-hot (2), radius
  * one-hot (3), star/box one-hot (2), log2 extents (3, axis 0 first, 0 for a
  * missing axis), log2 tile x/y/z (3), log2 threads per workgroup (1),
- * points per thread (1), temporal depth one-hot {1,2,4} (3), non-temporal
- * store flag (1), kernel variant (1: 0 LDS-staged, 1 deeper prefetch (two
- * planes for stars, three cells for boxes), 2 four-plane prefetch, 3
- * register-tiled, 4 two rows per thread). */
+ * points per thread (1), temporal depth one-hot {1,2,4} (3; UHS_CFG_TEMPORAL2
+ * counts as depth 2), non-temporal store flag (1), kernel variant (1: 0
+ * LDS-staged, 1 deeper prefetch (two planes for stars and two rows for 2D,
+ * three cells for boxes), 2 four-plane prefetch, 3 register-tiled, 4 two
+ * rows per thread, 5 two planes of box loads in flight, 6 both 4 and 5).
+ * Threads per workgroup follow the variant (ghost zone included for
+ * UHS_CFG_TEMPORAL2). */
 int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext,
                      const uhs_config *cfg, int depth, float *out20);
 

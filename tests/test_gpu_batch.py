@@ -48,17 +48,38 @@ def check_batch(u, orc, grids, T, cfg):
         dB.close()
 
 
+# Box tile forms: 256 points one row per thread, 512 points two rows per
+# thread, each with one or two planes of loads in flight.
+BOX_FORMS = {1: (32, 8, 9, 0), 2: (32, 16, 9, 1), 5: (32, 8, 9, 2), 8: (16, 32, 5, 3)}
+
+
 @pytest.mark.parametrize("depth", [1, 2, 4])
 @pytest.mark.parametrize("T", [1, 2, 5, 8])
 def test_batch_mix_star_box(gpu, orc, depth, T):
+    """Star tile 16x8: register-queue two-step tiles at depth 2 (its ghost-
+    zone extension needs fewer threads than the workgroup has)."""
     u = gpu
     kinds = ["3d1r_star", "3d2r_box"] * 4
     grids = make_grids(u, orc, kinds, seed=depth * 10 + T)
     cfg = u.batch_default_config()
     cfg.star_depth = depth
     cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = 16, 8, 7
-    cfg.box_tile_x, cfg.box_tile_y, cfg.box_tile_z = 32, 8, 9
+    cfg.box_tile_x, cfg.box_tile_y, cfg.box_tile_z, cfg.flags = BOX_FORMS[T]
     check_batch(u, orc, grids, T, cfg)
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("star_tile", [(64, 8), (62, 6), (30, 14)])
+def test_batch_star_tile_forms(gpu, orc, depth, star_tile):
+    """64x8: register-queue single steps, LDS-ring two-step tiles; 62x6 and
+    30x14: register-queue two-step tiles that fill the workgroup exactly."""
+    u = gpu
+    kinds = ["3d1r_star", "3d2r_box", "3d1r_star"]
+    grids = make_grids(u, orc, kinds, seed=31 + depth)
+    cfg = u.batch_default_config()
+    cfg.star_depth = depth
+    cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = star_tile + (11,)
+    check_batch(u, orc, grids, 7, cfg)
 
 
 @pytest.mark.parametrize("depth", [1, 2, 4])
@@ -84,6 +105,12 @@ def test_batch_default_config_and_rejections(gpu, orc):
     with pytest.raises(u.UhsError) as e:
         u.Batch([(dA, dB, ext, h, st)], 2, bad)
     assert e.value.code == -3
+    for tile in ((32, 8, 9, 1), (32, 15, 9, 1), (32, 16, 9, 0), (32, 8, 9, 4)):
+        bad = u.batch_default_config()
+        bad.box_tile_x, bad.box_tile_y, bad.box_tile_z, bad.flags = tile
+        with pytest.raises(u.UhsError) as e:
+            u.Batch([(dA, dB, ext, h, st)], 2, bad)
+        assert e.value.code == -3
     two_d = u.builtin("2d1r_star")
     with pytest.raises(u.UhsError):
         u.Batch([(dA, dB, ext, h, two_d)], 2)

@@ -108,8 +108,12 @@ CONFIGS = {
     "box3d": [(64, 4, 128, 1), (32, 8, 7, 2), (16, 4, 3, 1), (128, 2, 11, 2), (32, 16, 5, 1, 1),
               (32, 16, 9, 1, 2), (64, 8, 6, 2, 3),
               # two rows per thread
-              (32, 16, 9, 1, 16), (64, 8, 5, 1, 17), (16, 4, 3, 1, 16)],
-    "win2d": [(256, 1, 64, 1), (128, 1, 5, 2), (64, 1, 17, 4), (512, 1, 9, 1), (1024, 1, 3, 2, 1)],
+              (32, 16, 9, 1, 16), (64, 8, 5, 1, 17), (16, 4, 3, 1, 16),
+              # two planes of tile loads in flight
+              (32, 16, 9, 1, 64), (32, 8, 7, 2, 65), (32, 16, 9, 1, 80), (16, 4, 3, 1, 80)],
+    "win2d": [(256, 1, 64, 1), (128, 1, 5, 2), (64, 1, 17, 4), (512, 1, 9, 1), (1024, 1, 3, 2, 1),
+              # two / four rows of loads in flight
+              (256, 1, 5, 1, 2), (128, 1, 3, 2, 3), (64, 1, 7, 4, 8), (512, 1, 64, 1, 9)],
 }
 
 
@@ -163,7 +167,7 @@ def test_box_two_rows_all_precisions(gpu, orc, case, dtype):
     A0, B0 = random_grids(ext, h, 15)
     dt = np.float64 if dtype == "f64" else np.float32
     A0, B0 = A0.astype(dt), B0.astype(dt)
-    for cfg in ((32, 8, 4, 1, 16), (16, 16, 64, 1, 17)):
+    for cfg in ((32, 16, 4, 1, 16), (16, 16, 64, 1, 17), (64, 8, 5, 1, 80), (64, 8, 3, 2, 64)):
         gA, gB = run_gpu(u, st, A0, B0, ext, h, 3, dt, dtype == "f32k", cfg=cfg)
         A, B = A0.copy(), B0.copy()
         orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, 3, KAHAN if dtype == "f32k" else FUSED)
@@ -189,7 +193,8 @@ def test_two_steps_per_launch(gpu, orc, case, dtype):
         A0, B0 = A0.astype(dt), B0.astype(dt)
         mask = np.ones(A0.shape, bool)
         mask[tuple(slice(h, n - h) for n in A0.shape)] = False
-        for cfg in ((8, 4, 5, 2, T2), (30, 6, 7, 1, T2 | NT), (62, 6, 64, 2, T2)):
+        for cfg in ((10, 4, 5, 2, T2), (30, 6, 7, 1, T2 | NT), (62, 6, 64, 2, T2), (14, 6, 3, 2, T2 | u.CFG_PREFETCH2),
+                    (30, 4, 64, 1, T2 | NT | u.CFG_PREFETCH2)):
             for T in (1, 2, 3, 4, 5, 6):
                 log(case, dtype, ext, h, cfg, "T", T)
                 gA, gB = run_gpu(u, st, A0, B0, ext, h, T, dt, dtype == "f32k", cfg=cfg)
@@ -209,7 +214,7 @@ def test_two_steps_per_launch_config_checks(gpu):
     for bad in ((61, 4, 64, 2, T2),                    # tile_x + 2r odd with vec 2
                 (64, 8, 64, 4, T2),                    # vec 4
                 (124, 8, 64, 2, T2),                   # 64 * 12 threads
-                (60, 4, 64, 2, T2 | u.CFG_PREFETCH2)):  # no other variant flags
+                (60, 4, 64, 2, T2 | u.CFG_ROWS2)):      # no other variant flags
         with pytest.raises(u.UhsError) as e:
             u.plan_kind(ext, 2, st, cfg=bad)
         assert e.value.code == -3

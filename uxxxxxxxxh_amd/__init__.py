@@ -18,7 +18,7 @@ UHS_OK = 0
 ERRORS = {-1: "UHS_E_INVAL", -2: "UHS_E_UNSUPPORTED", -3: "UHS_E_CONFIG", -4: "UHS_E_HIP",
           -5: "UHS_E_NOMEM", -6: "UHS_E_IO", -7: "UHS_E_PARSE"}
 DT_F64, DT_F32, DT_F32K = 0, 1, 2
-CFG_NT_STORE, CFG_PREFETCH2, CFG_REGTILE, CFG_PREFETCH4, CFG_ROWS2, CFG_TEMPORAL2 = 1, 2, 4, 8, 16, 32
+CFG_NT_STORE, CFG_PREFETCH2, CFG_REGTILE, CFG_PREFETCH4, CFG_ROWS2, CFG_TEMPORAL2, CFG_PLANES2 = 1, 2, 4, 8, 16, 32, 64
 
 
 class UhsError(RuntimeError):
@@ -60,12 +60,12 @@ class GridDesc(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     _fields_ = [("star_depth", ctypes.c_int), ("star_tile_x", ctypes.c_int), ("star_tile_y", ctypes.c_int),
                 ("star_tile_z", ctypes.c_int), ("box_tile_x", ctypes.c_int), ("box_tile_y", ctypes.c_int),
-                ("box_tile_z", ctypes.c_int)]
+                ("box_tile_z", ctypes.c_int), ("flags", ctypes.c_int)]
 
     def __repr__(self):
-        return "BatchConfig(depth=%d star=%dx%dx%d box=%dx%dx%d)" % (
+        return "BatchConfig(depth=%d star=%dx%dx%d box=%dx%dx%d flags=%d)" % (
             self.star_depth, self.star_tile_x, self.star_tile_y, self.star_tile_z,
-            self.box_tile_x, self.box_tile_y, self.box_tile_z)
+            self.box_tile_x, self.box_tile_y, self.box_tile_z, self.flags)
 
 
 class TuneResult(ctypes.Structure):

@@ -105,14 +105,20 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
     const int dt = pl.args.dtype;
     switch (pl.family) {
     case FAM_STAR3D:
-        if (c.flags & UHS_CFG_TEMPORAL2) return uhs_find_star3d_tb2(pl.prog, dt, c.vec);
+        if (c.flags & UHS_CFG_TEMPORAL2)
+            return uhs_find_star3d_tb2(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
         if (c.flags & UHS_CFG_REGTILE) return uhs_find_star3d_rt(pl.prog, dt, c.vec);
         return uhs_find_star3d(pl.prog, dt, c.vec,
                                (c.flags & UHS_CFG_PREFETCH4) ? 4 : (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
     case FAM_BOX3D:
-        if (c.flags & UHS_CFG_ROWS2) return c.vec == 1 ? uhs_find_box3d(pl.prog, dt, 0, 0) : nullptr;
-        return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0);
-    case FAM_WIN2D: return uhs_find_win2d(pl.prog, dt, c.vec);
+    {
+        const int planes = (c.flags & UHS_CFG_PLANES2) ? 2 : 1;
+        if (c.flags & UHS_CFG_ROWS2) return c.vec == 1 ? uhs_find_box3d(pl.prog, dt, 0, 0, planes) : nullptr;
+        return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0, planes);
+    }
+    case FAM_WIN2D:
+        return uhs_find_win2d(pl.prog, dt, c.vec,
+                              (c.flags & UHS_CFG_PREFETCH4) ? 4 : (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
     default: return uhs_find_gather(dt);
     }
 }
@@ -132,20 +138,23 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     };
     if (pl.family == FAM_GATHER) return UHS_OK;
     if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4 | UHS_CFG_ROWS2 |
-                    UHS_CFG_TEMPORAL2))
+                    UHS_CFG_TEMPORAL2 | UHS_CFG_PLANES2))
         return bad("unknown config flags");
+    if ((c.flags & UHS_CFG_PLANES2) &&
+        (pl.family != FAM_BOX3D || (c.flags & UHS_CFG_PREFETCH2)))
+        return bad("two planes of tile loads in flight exist for 3D boxes without deeper cell prefetch");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
     if (c.flags & UHS_CFG_TEMPORAL2) {
         if (pl.family != FAM_STAR3D) return bad("two steps per launch exists for 3D star stencils only");
-        if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE))
-            return bad("two steps per launch combines with UHS_CFG_NT_STORE only");
+        if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2))
+            return bad("two steps per launch combines with UHS_CFG_NT_STORE and UHS_CFG_PREFETCH2 only");
         const long long bx = c.tile_x + 2LL * R, by = c.tile_y + 2LL * R;
         if (c.vec != 1 && c.vec != 2) return bad(str("two steps per launch needs vec 1 or 2 (got %lld)", c.vec));
         if (bx % c.vec) return bad(str("tile_x + 2r = %lld not a multiple of vec %lld", bx, c.vec));
         const long long nt = bx / c.vec * by;
         if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
-        if (ceil_div(2LL * R * (bx + by), nt) > kStarHaloPerThread)
-            return bad(str("ring of %lld cells needs more than %lld per thread", 2 * R * (bx + by), kStarHaloPerThread));
+        if (ceil_div(2LL * R * (bx + by), nt) > kTb2RingCells)
+            return bad(str("ring of %lld cells needs more than %lld per thread", 2 * R * (bx + by), kTb2RingCells));
         const size_t esz = a.dtype == DT_F64 ? 8 : 4;
         const size_t lds = 2 * esz * (size_t)((bx + 2 * R) * (by + 2 * R) + bx * by);
         if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
@@ -158,8 +167,9 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         if (c.vec != 1 || (c.tile_y & 1) || (c.flags & UHS_CFG_PREFETCH2))
             return bad("two rows per thread needs vec 1, an even tile_y and no deeper prefetch");
     }
-    if ((c.flags & UHS_CFG_PREFETCH4) && ((c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE)) || pl.family != FAM_STAR3D))
-        return bad("four-plane prefetch is a 3D-star variant of its own");
+    if ((c.flags & UHS_CFG_PREFETCH4) &&
+        ((c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE)) || (pl.family != FAM_STAR3D && pl.family != FAM_WIN2D)))
+        return bad("four-plane prefetch is a 3D-star and 2D variant of its own");
     if (c.flags & UHS_CFG_REGTILE) {
         if (pl.family != FAM_STAR3D) return bad("the register-tiled kernel exists for 3D star stencils only");
         if (c.flags & UHS_CFG_PREFETCH2) return bad("the register-tiled kernel has no two-plane prefetch variant");
@@ -190,14 +200,14 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
             return bad(str("halo ring of %lld cells needs more than %lld per thread",
                            2 * R * (c.tile_x + c.tile_y), kStarHaloPerThread));
         if (pl.family == FAM_BOX3D &&
-            ceil_div((long long)(c.tile_x + 2 * R) * (c.tile_y + 2 * R), nt) > kBoxCellsPerThread)
+            ceil_div((long long)(c.tile_x + 2 * R) * (c.tile_y + 2 * R), nt) > box_cells_per_thread(R))
             return bad(str("plane tile of %lld cells needs more than %lld per thread",
-                           (c.tile_x + 2 * R) * (c.tile_y + 2 * R), kBoxCellsPerThread));
+                           (c.tile_x + 2 * R) * (c.tile_y + 2 * R), box_cells_per_thread(R)));
     }
     if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
     if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
-    if ((c.flags & UHS_CFG_PREFETCH2) && pl.family != FAM_STAR3D && pl.family != FAM_BOX3D)
-        return bad("deeper prefetch exists for 3D stencils only");
+    if ((c.flags & (UHS_CFG_ROWS2 | UHS_CFG_PLANES2 | UHS_CFG_REGTILE | UHS_CFG_TEMPORAL2)) && pl.family == FAM_WIN2D)
+        return bad("2D stencils take UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2 and UHS_CFG_PREFETCH4 only");
     if (!uhs_pick_fn(pl, c)) return bad(std::string("no ") + family_name(pl.family) + str(" kernel compiled for vec %lld", c.vec));
     return UHS_OK;
 }
@@ -208,15 +218,15 @@ static const uhs_config kBuiltinDefaults[12] = {
     {512, 1, 32, 1, UHS_CFG_NT_STORE},                       // 2d1r_star
     {1024, 1, 128, 2, 0},                                     // 2d2r_star
     {256, 1, 32, 1, 0},                                       // 2d3r_star
-    {512, 1, 32, 1, 0},                                       // 2d1r_box
+    {512, 1, 32, 1, UHS_CFG_NT_STORE},                       // 2d1r_box
     {1024, 1, 128, 2, 0},                                     // 2d2r_box
-    {64, 1, 256, 1, 0},                                       // 2d3r_box
+    {512, 1, 256, 1, UHS_CFG_NT_STORE},                      // 2d3r_box
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d1r_star
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d2r_star
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d3r_star
-    {32, 16, 64, 2, 0},                                       // 3d1r_box
-    {32, 16, 128, 1, 0},                                      // 3d2r_box
-    {64, 4, 128, 1, 0},                                       // 3d3r_box
+    {64, 16, 128, 1, UHS_CFG_ROWS2 | UHS_CFG_PLANES2},       // 3d1r_box
+    {32, 16, 64, 1, UHS_CFG_ROWS2 | UHS_CFG_PLANES2 | UHS_CFG_NT_STORE}, // 3d2r_box
+    {64, 16, 128, 1, UHS_CFG_ROWS2 | UHS_CFG_PLANES2 | UHS_CFG_NT_STORE}, // 3d3r_box
 };
 
 uhs_config uhs_heuristic_config(const Plan &pl)
@@ -307,18 +317,25 @@ int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
     if (uhs_check_config(*pl, c, why, sizeof why) != UHS_OK)
         return fail(UHS_E_CONFIG, "config {%d,%d,%d,%d,%d} rejected: %s", c.tile_x, c.tile_y,
                     c.tile_z, c.vec, c.flags, why);
+    return uhs_plan_apply(*pl, c);
+}
+
+int uhs_plan_apply(Plan &pl, const uhs_config &c)
+{
+    SweepArgs &a = pl.args;
     a.tx = c.tile_x; a.ty = c.tile_y; a.tz = c.tile_z; a.vec = c.vec; a.flags = c.flags;
-    pl->cfg = c;
-    pl->fn = uhs_pick_fn(*pl, c);
-    if (!pl->fn) return fail(UHS_E_UNSUPPORTED, "no kernel for this stencil and precision");
+    pl.cfg = c;
+    pl.fn = uhs_pick_fn(pl, c);
+    pl.fn1 = nullptr;
+    if (!pl.fn) return fail(UHS_E_UNSUPPORTED, "no kernel for this stencil and precision");
     if (c.flags & UHS_CFG_TEMPORAL2) {
-        Plan one = *pl;
+        Plan one = pl;
         const uhs_config c1 = uhs_heuristic_config(one);
         one.args.tx = c1.tile_x; one.args.ty = c1.tile_y; one.args.tz = c1.tile_z;
         one.args.vec = c1.vec; one.args.flags = c1.flags;
-        pl->fn1 = uhs_pick_fn(one, c1);
-        pl->args1 = one.args;
-        if (!pl->fn1) return fail(UHS_E_UNSUPPORTED, "no single-step kernel for the leading steps");
+        pl.fn1 = uhs_pick_fn(one, c1);
+        pl.args1 = one.args;
+        if (!pl.fn1) return fail(UHS_E_UNSUPPORTED, "no single-step kernel for the leading steps");
     }
     return UHS_OK;
 }

@@ -16,21 +16,52 @@ namespace {
 
 using MixFn = void (*)(BLaunch);
 
-// The benchmark mix (7-point star + 125-point box) gets its own kernel so
-// its register footprint is the larger of the two, not of all six.
-template <int K> MixFn mix_kernel() { return k_batch<K, Prog_3d1r_star, Prog_3d2r_box>; }
+// The benchmark mix (7-point star + 125-point box) gets its own kernels,
+// one per combination of the tile forms a launch can hold (star: none, the
+// register-queue one- or two-step tile, or the LDS-ring tile; box: one or
+// two rows per thread, one or two planes in flight), so each launch carries
+// the register footprint of its own forms only.  Other stencil sets use one
+// kernel with every form per depth.
+template <int K, int SF> MixFn mix_box(int bf)
+{
+    switch (bf) {
+    case kFormBoxStd | kFormPlanes1: return k_batch<K, SF | kFormBoxStd | kFormPlanes1, Prog_3d1r_star, Prog_3d2r_box>;
+    case kFormBoxStd | kFormPlanes2: return k_batch<K, SF | kFormBoxStd | kFormPlanes2, Prog_3d1r_star, Prog_3d2r_box>;
+    case kFormRows2 | kFormPlanes1: return k_batch<K, SF | kFormRows2 | kFormPlanes1, Prog_3d1r_star, Prog_3d2r_box>;
+    case kFormRows2 | kFormPlanes2: return k_batch<K, SF | kFormRows2 | kFormPlanes2, Prog_3d1r_star, Prog_3d2r_box>;
+    }
+    return nullptr;
+}
 template <int K> MixFn all_kernel()
 {
-    return k_batch<K, Prog_3d1r_star, Prog_3d2r_star, Prog_3d3r_star, Prog_3d1r_box, Prog_3d2r_box,
+    return k_batch<K, kFormAll, Prog_3d1r_star, Prog_3d2r_star, Prog_3d3r_star, Prog_3d1r_box, Prog_3d2r_box,
                    Prog_3d3r_box>;
 }
 
-MixFn pick_kernel(bool mix, int K)
+MixFn pick_kernel(bool mix, int K, int forms)
 {
+    if (mix) {
+        const int sf = forms & (kFormRq1 | kFormRq2 | kFormStarLds);
+        const int bf = forms & (kFormRows2 | kFormBoxStd | kFormPlanes1 | kFormPlanes2);
+        MixFn f = nullptr;
+        // Box-only launches use the kernel that also has the (light)
+        // register-queue star tile: an instantiation without any star form
+        // does not compile (the backend rejects it).
+        if (K == 1 && (sf == 0 || sf == kFormRq1)) f = mix_box<1, kFormRq1>(bf);
+        if (K == 1 && sf == kFormStarLds) f = mix_box<1, kFormStarLds>(bf);
+        if (K == 2 && sf == kFormRq2) f = mix_box<2, kFormRq2>(bf);
+        if (K == 2 && sf == kFormStarLds) f = mix_box<2, kFormStarLds>(bf);
+        if (K == 4 && sf == kFormStarLds) f = mix_box<4, kFormStarLds>(bf);
+        if (f) return f;
+        if (K == 1) return k_batch<1, kFormAll, Prog_3d1r_star, Prog_3d2r_box>;
+        if (K == 2) return k_batch<2, kFormAll, Prog_3d1r_star, Prog_3d2r_box>;
+        if (K == 4) return k_batch<4, kFormAll, Prog_3d1r_star, Prog_3d2r_box>;
+        return nullptr;
+    }
     switch (K) {
-    case 1: return mix ? mix_kernel<1>() : all_kernel<1>();
-    case 2: return mix ? mix_kernel<2>() : all_kernel<2>();
-    case 4: return mix ? mix_kernel<4>() : all_kernel<4>();
+    case 1: return all_kernel<1>();
+    case 2: return all_kernel<2>();
+    case 4: return all_kernel<4>();
     }
     return nullptr;
 }
@@ -87,7 +118,7 @@ extern "C" {
 int uhs_batch_default_config(uhs_batch_config *c)
 {
     if (!c) return berr(UHS_E_INVAL, "null config");
-    *c = uhs_batch_config{2, 64, 8, 64, 64, 4, 512};
+    *c = uhs_batch_config{2, 30, 14, 64, 32, 16, 128, UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2};
     return UHS_OK;
 }
 
@@ -101,8 +132,12 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
     else uhs_batch_default_config(&cfg);
     const int K = cfg.star_depth;
     if (K != 1 && K != 2 && K != 4) return berr(UHS_E_CONFIG, "star_depth must be 1, 2 or 4");
-    if (cfg.box_tile_x * cfg.box_tile_y != kBatchThreads || cfg.box_tile_x < 1)
-        return berr(UHS_E_CONFIG, "box tiles must hold exactly 256 points (tile_x * tile_y)");
+    if (cfg.flags & ~(UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2)) return berr(UHS_E_CONFIG, "unknown batch flags");
+    const bool rows2 = cfg.flags & UHS_BATCH_BOX_ROWS2;
+    if (cfg.box_tile_x < 1 || cfg.box_tile_y < 1 ||
+        cfg.box_tile_x * cfg.box_tile_y != (rows2 ? 2 : 1) * kBatchThreads || (rows2 && cfg.box_tile_y % 2))
+        return berr(UHS_E_CONFIG, "box tiles must hold exactly 256 points (tile_x * tile_y), or 512 with an even "
+                                  "tile_y for two rows per thread");
     if (cfg.star_tile_x < 1 || cfg.star_tile_y < 1 || cfg.star_tile_z < 1 || cfg.box_tile_z < 1)
         return berr(UHS_E_CONFIG, "tile sizes must be positive");
 
@@ -148,11 +183,18 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         g.kind = prog;
         if (b.star) {
             g.TX = cfg.star_tile_x; g.TY = cfg.star_tile_y;
-            g.rq = g.TX * g.TY == 2 * kBatchThreads && g.TX % 2 == 0 &&
-                   2 * b.radius * (g.TX + g.TY) <= kStarHaloPerThread * kBatchThreads;
+            const int R = b.radius, bx = g.TX + 2 * R, by = g.TY + 2 * R;
+            if (g.TX * g.TY == 2 * kBatchThreads && g.TX % 2 == 0 &&
+                2 * R * (g.TX + g.TY) <= kStarHaloPerThread * kBatchThreads)
+                g.mode |= kModeRq1;
+            if (K == 2 && bx % 2 == 0 && bx / 2 * by <= kBatchThreads &&
+                2 * R * (bx + by) <= kBatchRingCells * kBatchThreads)
+                g.mode |= kModeRq2;
             for (int t = 0; t < b.npoints; t++) g.coef[t] = d.stencil.coeffs[t];
         } else {
             g.TX = cfg.box_tile_x; g.TY = cfg.box_tile_y;
+            if (rows2) g.mode |= kModeRows2;
+            if (cfg.flags & UHS_BATCH_BOX_PLANES2) g.mode |= kModePlanes2;
             // box3d reads the coefficients cell-major (box_pack_index).
             for (int t = 0; t < b.npoints; t++) {
                 int idx = t;
@@ -163,7 +205,7 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
                 }
                 g.coef[idx] = d.stencil.coeffs[t];
             }
-            if ((long long)(g.TX + 2 * b.radius) * (g.TY + 2 * b.radius) > kBoxCellsPerThread * kBatchThreads)
+            if ((long long)(g.TX + 2 * b.radius) * (g.TY + 2 * b.radius) > kBatchBoxCells * kBatchThreads)
                 return berr(UHS_E_CONFIG, "box tile plus halo exceeds the per-thread staging");
         }
         if (prog != Prog_3d1r_star::ID && prog != Prog_3d2r_box::ID) mix = false;
@@ -171,7 +213,16 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
 
     // Schedules and tasks.
     const std::vector<int> ssched = star_schedule(timesteps, K);
-    const int L = std::max<int>((int)ssched.size(), timesteps);
+    const int nls = (int)ssched.size();
+    const int L = std::max<int>(nls, timesteps);
+    // Star launch j runs in global launch slot[j]: spread evenly over the
+    // box grids' one-step launches, so every part of the run has HBM-bound
+    // star tiles beside the VALU-bound box tiles (each grid's own launches
+    // stay in order).
+    std::vector<int> slot(nls);
+    for (int j = 0; j < nls; j++) slot[j] = (int)((long long)j * L / nls);
+    std::vector<int> ldepth(L, 1);
+    for (int j = 0; j < nls; j++) ldepth[slot[j]] = ssched[j];
     std::vector<std::vector<BTask>> per(L);
     std::vector<std::vector<double>> cost(L);
     for (int i = 0; i < n; i++) {
@@ -195,10 +246,11 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
                         t.x0 = x; t.y0 = y; t.zs = z; t.ze = std::min(z + tz, g.n0);
                         t.depth = depth;
                         t.k0 = k0;
-                        per[l].push_back(t);
+                        const int gl = b.star ? slot[l] : l;
+                        per[gl].push_back(t);
                         const double pts = (double)std::min(g.TX, g.n2 - x) * std::min(g.TY, g.n1 - y) *
                                            (t.ze - t.zs);
-                        cost[l].push_back(pts * (b.star ? 1.0 : b.npoints / 8.0) * depth);
+                        cost[gl].push_back(pts * (b.star ? 1.0 : b.npoints / 8.0) * depth);
                     }
             k0 += depth;
         }
@@ -234,21 +286,27 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
             ord.push_back(k);
         }
         LaunchRec r{};
-        r.depth = l < (int)ssched.size() ? ssched[l] : 1;
+        r.depth = ldepth[l];
         r.first = all.size();
         r.count = ord.size();
         size_t lds = 0;
+        int forms = 0;
         for (size_t k : ord) {
             const BTask &t = per[l][k];
             const BGrid &g = hg[t.grid];
             const BuiltinStencil &b = kBuiltins[g.kind];
+            forms |= task_forms(g, t, b.star);
             const int R = b.radius;
             size_t need;
             if (b.star) {
                 const size_t wc = (size_t)(g.TX + 2 * t.depth * R) * (g.TY + 2 * t.depth * R);
                 need = sizeof(double) * t.depth * (2 * R + 1) * wc;
-                if (t.depth == 1 && g.rq) // star3d_tile: two staged planes
+                if (t.depth == 1 && (g.mode & kModeRq1)) // star3d_tile: two staged planes
                     need = 2 * sizeof(double) * (size_t)(g.TX + 2 * R) * (g.TY + 2 * R);
+                if (t.depth == 2 && (g.mode & kModeRq2)) { // star3d_tb2_tile
+                    const size_t bx = g.TX + 2 * R, by = g.TY + 2 * R;
+                    need = 2 * sizeof(double) * ((bx + 2 * R) * (by + 2 * R) + bx * by);
+                }
             } else {
                 need = 2 * sizeof(double) * (size_t)(g.TX + 2 * R) * (g.TY + 2 * R);
             }
@@ -260,7 +318,7 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
             return berr(UHS_E_CONFIG, "tile and depth need more LDS than a workgroup may use");
         }
         r.lds = lds;
-        r.fn = pick_kernel(mix, r.depth);
+        r.fn = pick_kernel(mix, r.depth, forms);
         B->launches.push_back(r);
     }
     B->ntasks = (long long)all.size();
@@ -360,17 +418,33 @@ int uhs_batch_tune(const uhs_grid_desc *grids, int n, int timesteps, uhs_batch_c
                    uhs_batch_config *cands_out, double *ms_out, int ncand_cap, int *ncand, uhs_stream_t stream)
 {
     if (!grids || n < 1 || !best) return berr(UHS_E_INVAL, "batch tuning needs grids and an output");
-    static const int depths[] = {1, 2, 4};
-    static const int stiles[][2] = {{32, 8}, {64, 4}, {64, 8}, {128, 4}, {32, 16}};
+    // Star tiles per depth: 512-point tiles (register-queue single steps),
+    // tiles whose ghost-zone extension fits 256 threads (register-queue two
+    // steps), and for depth 4 the LDS-ring form.  Box tiles: 256 points, or
+    // 512 with two rows per thread; with one or two planes in flight.
+    static const int s1[][2] = {{64, 8}, {128, 4}, {32, 16}};
+    static const int s2[][2] = {{62, 6}, {30, 14}, {30, 6}, {64, 4}};
+    static const int s4[][2] = {{32, 8}};
     static const int stz[] = {64, 512};
-    static const int btiles[][2] = {{32, 8}, {64, 4}, {16, 16}};
-    static const int btz[] = {128, 512};
+    static const int btiles[][3] = {{32, 8, 0},
+                                    {64, 4, 0},
+                                    {32, 16, UHS_BATCH_BOX_ROWS2},
+                                    {64, 8, UHS_BATCH_BOX_ROWS2},
+                                    {16, 32, UHS_BATCH_BOX_ROWS2}};
+    static const int btz[] = {64, 128};
+    static const int bpl[] = {0, UHS_BATCH_BOX_PLANES2};
     std::vector<uhs_batch_config> cand;
-    for (int d : depths)
-        for (auto &s : stiles)
+    auto add = [&](int d, const int (*tiles)[2], int nt) {
+        for (int i = 0; i < nt; i++)
             for (int sz : stz)
                 for (auto &b : btiles)
-                    for (int bz : btz) cand.push_back(uhs_batch_config{d, s[0], s[1], sz, b[0], b[1], bz});
+                    for (int bz : btz)
+                        for (int pl : bpl)
+                            cand.push_back(uhs_batch_config{d, tiles[i][0], tiles[i][1], sz, b[0], b[1], bz, b[2] | pl});
+    };
+    add(1, s1, 3);
+    add(2, s2, 4);
+    add(4, s4, 1);
     double bms = 1e30;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)

@@ -17,6 +17,9 @@
 #include "sweep_kernels.h"
 
 constexpr int kBatchThreads = 256;
+// Staging registers per thread of the batch's tiles (box: plane-tile cells,
+// two-step star: ring cells); the host rejects tiles that need more.
+constexpr int kBatchBoxCells = 4, kBatchRingCells = 2;
 
 // Per-grid record, in device memory.
 struct BGrid {
@@ -27,7 +30,7 @@ struct BGrid {
     long long sz;
     int kind;             // built-in stencil id (kBuiltins index)
     int TX, TY;           // tile
-    int rq;               // stars: single steps use the register-queue tile (star3d_tile, 2 points/thread)
+    int mode;             // kMode* bits: which tile code runs this grid's tasks
     int pad;
     double coef[UHS_MAX_POINTS]; // term order (stars) or box_pack_index order (boxes)
 };
@@ -43,6 +46,23 @@ struct BTask {
     int k0;
     int pad;
 };
+
+// BGrid::mode bits.  Stars: kModeRq1 = single steps use the register-queue
+// tile (star3d_tile, 2 points per thread, tile = 512 points); kModeRq2 = two-
+// step tasks use the register-queue two-step tile (star3d_tb2_tile, 2 columns
+// per thread over the tile plus its ghost zone).  Boxes: kModeRows2 = two
+// rows per thread (tile = 512 points), kModePlanes2 = two planes of tile
+// loads in flight.
+constexpr int kModeRq1 = 1, kModeRq2 = 2, kModeRows2 = 4, kModePlanes2 = 8;
+
+// Tile forms a k_batch instantiation compiles in (template argument F): the
+// forms of kMode* plus the LDS-ring star tile, the one-row box tile and one
+// plane of box loads in flight.  A launch uses an instantiation with just
+// the forms its tasks need, so its register footprint (and occupancy) is
+// that of those forms, not of every form.
+constexpr int kFormRq1 = 1, kFormRq2 = 2, kFormRows2 = 4, kFormPlanes2 = 8, kFormStarLds = 16, kFormBoxStd = 32,
+              kFormPlanes1 = 64;
+constexpr int kFormAll = 127;
 
 struct BLaunch {
     const BTask *tasks;
@@ -139,38 +159,89 @@ __device__ __forceinline__ void star_tb_tile(const BGrid &g, const BTask &t, dou
     }
 }
 
-// Runs task `t` of grid `g` with program S (star: depth K; box: one step).
-template <class S, int K>
+// Runs task `t` of grid `g` with program S (star: depth t.depth <= K; box:
+// one step), with the tile forms F allows (the host picks F to cover every
+// task of the launch).
+template <class S, int K, int F>
 __device__ __forceinline__ void run_task(const BGrid &g, const BTask &t, double *lds)
 {
     typedef const __attribute__((address_space(4))) double *kptr;
     Coefs<double, S::P> cf((kptr)g.coef);
     if constexpr (S::STAR) {
         if constexpr (K == 1) {
-            // One step: the register-queue star tile when the tile holds
-            // two points per thread, else the LDS-ring tile.  Single-step
-            // launches come first in a grid's schedule, so src always holds
-            // the halo of its own state (src is the ping-pong buffer).
-            if (g.rq) {
-                const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, g.TX / 2, false};
-                star3d_tile<double, S, 2, false, 1>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
-            } else {
-                star_tb_tile<S, 1>(g, t, lds, cf);
+            // Single-step launches come first in a grid's schedule, so src
+            // always holds the halo of its own state (src is the ping-pong
+            // buffer) and the one-step tiles may read halos from it.
+            if constexpr ((F & kFormRq1) != 0) {
+                if (g.mode & kModeRq1) {
+                    const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, g.TX / 2, false};
+                    star3d_tile<double, S, 2, false, 1>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+                    return;
+                }
             }
+            if constexpr ((F & kFormStarLds) != 0) star_tb_tile<S, 1>(g, t, lds, cf);
+        } else if constexpr (K == 2) {
+            if constexpr ((F & kFormRq2) != 0) {
+                if (g.mode & kModeRq2) {
+                    constexpr int R = S::R;
+                    const long long origin = (long long)g.halo * g.sz + (long long)g.halo * g.sy + g.halo;
+                    const double *hA = g.A + origin, *hB = g.B + origin;
+                    const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, (g.TX + 2 * R) / 2, false};
+                    star3d_tb2_tile<double, S, 2, false, 1, kBatchRingCells>(t.src, t.dst, (t.k0 & 1) ? hB : hA,
+                                                             (t.k0 & 1) ? hA : hB, geo, t.x0, t.y0, t.zs, t.ze,
+                                                             lds, cf);
+                    return;
+                }
+            }
+            if constexpr ((F & kFormStarLds) != 0) star_tb_tile<S, 2>(g, t, lds, cf);
         } else {
-            star_tb_tile<S, K>(g, t, lds, cf);
+            if constexpr ((F & kFormStarLds) != 0) star_tb_tile<S, K>(g, t, lds, cf);
         }
     } else {
         const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, g.TX, false};
-        box3d_tile<double, S, 1, false, 1>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+        const bool pl2 = g.mode & kModePlanes2;
+        if (g.mode & kModeRows2) {
+            if constexpr ((F & kFormRows2) != 0) {
+                if constexpr ((F & kFormPlanes2) != 0) {
+                    if (pl2) {
+                        box3d_tile_rows2<double, S, false, 2, kBatchBoxCells>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+                        return;
+                    }
+                }
+                if constexpr ((F & kFormPlanes1) != 0)
+                    box3d_tile_rows2<double, S, false, 1, kBatchBoxCells>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+            }
+        } else {
+            if constexpr ((F & kFormBoxStd) != 0) {
+                if constexpr ((F & kFormPlanes2) != 0) {
+                    if (pl2) {
+                        box3d_tile<double, S, 1, false, 1, 2, kBatchBoxCells>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+                        return;
+                    }
+                }
+                if constexpr ((F & kFormPlanes1) != 0)
+                    box3d_tile<double, S, 1, false, 1, 1, kBatchBoxCells>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+            }
+        }
     }
+}
+
+// The forms task `t` of grid `g` needs (host and device agree on this).
+__host__ __device__ inline int task_forms(const BGrid &g, const BTask &t, bool star)
+{
+    if (star) {
+        if (t.depth == 1) return (g.mode & kModeRq1) ? kFormRq1 : kFormStarLds;
+        if (t.depth == 2) return (g.mode & kModeRq2) ? kFormRq2 : kFormStarLds;
+        return kFormStarLds;
+    }
+    return ((g.mode & kModeRows2) ? kFormRows2 : kFormBoxStd) | ((g.mode & kModePlanes2) ? kFormPlanes2 : kFormPlanes1);
 }
 
 // The persistent kernel: each workgroup takes the next task index from the
 // launch's counter until the list is exhausted (every workgroup reaches the
 // exit).  Progs lists the stencils this instantiation can run; a grid's kind
 // selects one.
-template <int K, class... Progs>
+template <int K, int F, class... Progs>
 __global__ void __launch_bounds__(kBatchThreads) k_batch(BLaunch L)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -179,13 +250,15 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(BLaunch L)
     for (;;) {
         if (threadIdx.x == 0) next = (int)atomicAdd(L.counter, 1u);
         __syncthreads();
-        const int i = next;
+        // Every lane read the same value: say so, so the task and grid
+        // records (and the coefficient pointer) live in scalar registers.
+        const int i = __builtin_amdgcn_readfirstlane(next);
         __syncthreads();
         if (i >= L.ntasks) break;
         const BTask t = L.tasks[i];
         const BGrid &g = L.grids[t.grid];
         const int kind = g.kind;
-        ((kind == Progs::ID ? (run_task<Progs, K>(g, t, lds), 0) : 0), ...);
+        ((kind == Progs::ID ? (run_task<Progs, K, F>(g, t, lds), 0) : 0), ...);
         __syncthreads();
     }
 }

@@ -268,12 +268,19 @@ int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext, const u
     f[10] = log2f((float)c->tile_x);
     f[11] = log2f((float)c->tile_y);
     f[12] = log2f((float)c->tile_z);
-    f[13] = log2f((float)(c->tile_x / std::max(1, c->vec) * c->tile_y));
+    const int v = std::max(1, c->vec);
+    int threads = c->tile_x / v * c->tile_y;
+    if (c->flags & UHS_CFG_REGTILE) threads = 64 * (c->tile_y / v);
+    else if (c->flags & UHS_CFG_ROWS2) threads = c->tile_x * c->tile_y / 2;
+    else if (c->flags & UHS_CFG_TEMPORAL2) threads = (c->tile_x + 2 * radius) / v * (c->tile_y + 2 * radius);
+    f[13] = log2f((float)std::max(1, threads));
     f[14] = (float)c->vec;
+    if (c->flags & UHS_CFG_TEMPORAL2) depth = std::max(depth, 2);
     f[15 + (depth >= 4 ? 2 : depth >= 2 ? 1 : 0)] = 1.f;
     f[18] = (c->flags & UHS_CFG_NT_STORE) ? 1.f : 0.f;
-    f[19] = (c->flags & UHS_CFG_ROWS2) ? 4.f : (c->flags & UHS_CFG_REGTILE) ? 3.f : (c->flags & UHS_CFG_PREFETCH4) ? 2.f
-          : (c->flags & UHS_CFG_PREFETCH2) ? 1.f : 0.f;
+    const bool rows2 = c->flags & UHS_CFG_ROWS2, planes2 = c->flags & UHS_CFG_PLANES2;
+    f[19] = rows2 && planes2 ? 6.f : planes2 ? 5.f : rows2 ? 4.f : (c->flags & UHS_CFG_REGTILE) ? 3.f
+          : (c->flags & UHS_CFG_PREFETCH4) ? 2.f : (c->flags & UHS_CFG_PREFETCH2) ? 1.f : 0.f;
     return UHS_OK;
 }
 

@@ -187,6 +187,212 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
 }
 
 // --------------------------------------------------------------------------
+// star3d_tb2
+// --------------------------------------------------------------------------
+// star3d_tb2: two timesteps per launch for the 3D star stencils (temporal
+// blocking with an R-cell ghost zone), so each cell crosses HBM once per two
+// updates instead of once per update.
+//
+// A workgroup owns an output tile of tile_x * tile_y columns and covers it
+// extended by R on each side (the ghost zone): BX = tile_x + 2R by
+// BY = tile_y + 2R columns, VX columns per thread along the unit-stride
+// axis.  It streams planes along axis 0.  At iteration i (plane z1 of
+// level 1):
+//   level 0 (the input): each thread holds its columns' 2R+1 planes around z1
+//     in a register queue (plus one plane in flight) and stages plane z1,
+//     with an R-wide ring around the extended tile, in LDS buffer i&1;
+//   level 1 (one step): the thread stages plane z1-1-R of level 1 (the
+//     centre of its level-1 register queue) in LDS buffer i&1 too;
+//   -- one barrier --
+//   level 1 at plane z1 for every extended column: axis-0 neighbours from the
+//     level-0 queue, in-plane neighbours from LDS;
+//   level 2 at plane z1-1-R for the tile's own columns: axis-0 neighbours
+//     from the level-1 queue, in-plane neighbours from LDS.
+// Buffer i&1 is rewritten two iterations later, after the next barrier, so
+// one barrier per plane suffices.
+//
+// Cells of a level outside the interior hold that level's halo: the halo of
+// the buffer the one-step ping-pong would hold that state in (the sweep
+// never writes halos), passed as hA/hB with the global step k0 of the
+// input.  Every update is the product-then-fma chain in term order, so the
+// result is bitwise identical to two single sweeps.
+template <typename T, class S, int VX, bool KAHAN, int PF, int NH = kStarHaloPerThread, class CF>
+__device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__restrict__ out,
+                                                const T *__restrict__ h0, const T *__restrict__ h1,
+                                                const TileGeom &g, int x0, int y0, int zs, int ze,
+                                                T *lds, CF &cf)
+{
+    constexpr int R = S::R, P = S::P, Q = 2 * R + 1;
+    const int NT = blockDim.x, tid = threadIdx.x;
+    const int BXT = g.BXT, BX = g.TX + 2 * R, BY = g.TY + 2 * R;
+    const int LW = BX + 2 * R, L0SZ = LW * (BY + 2 * R), L1SZ = BX * BY;
+    T *L0 = lds;            // 2 x L0SZ: level-0 plane + ring
+    T *L1 = lds + 2 * L0SZ; // 2 x L1SZ: level-1 plane over the extended tile
+    const int ox = x0 - R, oy = y0 - R; // interior coordinates of extended cell (0, 0)
+    const int tx = tid % BXT, ty = tid / BXT;
+    // Threads beyond the extended tile (a workgroup larger than it, as in
+    // the batched kernel) only stage ring cells and join the barriers.
+    const bool col = ty < BY;
+    const long long sz = g.sz;
+    const int sy = g.sy, n0 = g.n0, n1 = g.n1, n2 = g.n2;
+
+    int coff[VX], l0c[VX], l1c[VX];
+    bool xyin[VX], core[VX];
+    const int yg = oy + ty;
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int ex = tx + v * BXT, xg = ox + ex;
+        xyin[v] = col && xg >= 0 && xg < n2 && yg >= 0 && yg < n1;
+        core[v] = xyin[v] && ex >= R && ex < BX - R && ty >= R && ty < BY - R;
+        coff[v] = clampi(yg, -R, n1 + R - 1) * sy + clampi(xg, -R, n2 + R - 1);
+        l0c[v] = (ty + R) * LW + ex + R;
+        l1c[v] = ty * BX + ex;
+    }
+    // Level-0 ring around the extended tile: R rows above and below, R
+    // columns left and right (a star needs no corners).
+    const int H = 2 * R * (BX + BY);
+    int hoff[NH], hl[NH];
+    bool hin[NH];
+#pragma unroll
+    for (int k = 0; k < NH; k++) {
+        const int c = tid + k * NT;
+        hl[k] = -1;
+        hoff[k] = 0;
+        hin[k] = false;
+        if (c < H) {
+            int ly, lx;
+            if (c < 2 * R * BX) {
+                const int row = c / BX;
+                ly = row < R ? row : BY + row;
+                lx = R + c % BX;
+            } else {
+                const int cc = c - 2 * R * BX, col = cc % (2 * R);
+                ly = R + cc / (2 * R);
+                lx = col < R ? col : BX + col;
+            }
+            const int gy = oy - R + ly, gx = ox - R + lx;
+            hl[k] = ly * LW + lx;
+            hin[k] = gy >= 0 && gy < n1 && gx >= 0 && gx < n2;
+            hoff[k] = clampi(gy, -R, n1 + R - 1) * sy + clampi(gx, -R, n2 + R - 1);
+        }
+    }
+    // Level-0 value of a cell at plane z: the input inside the interior, the
+    // level-0 halo outside it.  Planes beyond the allocation are clamped;
+    // their values feed only level-1 cells outside the interior, which take
+    // the level-1 halo instead.
+    auto lev0 = [&](bool cin, int z, int off) -> T {
+        const long long a = (long long)clampi(z, -R, n0 + R - 1) * sz + off;
+        return (cin && z >= 0 && z < n0) ? in[a] : h0[a];
+    };
+
+    const int z1s = zs - R;             // first level-1 plane
+    const int N = (ze - zs) + 2 * R;    // level-1 planes
+    // nq0[f]: plane z1+R+f of the own columns; nh[f]: plane z1+f of the
+    // ring (PF planes in flight).
+    T q0[VX][Q], q1[VX][Q], nq0[PF][VX], nh[PF][NH];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+#pragma unroll
+        for (int j = 0; j < 2 * R; j++) q0[v][j] = lev0(xyin[v], z1s - R + j, coff[v]);
+#pragma unroll
+        for (int f = 0; f < PF; f++) nq0[f][v] = lev0(xyin[v], z1s + R + f, coff[v]);
+#pragma unroll
+        for (int j = 0; j < Q; j++) q1[v][j] = T(0);
+    }
+#pragma unroll
+    for (int f = 0; f < PF; f++)
+#pragma unroll
+        for (int k = 0; k < NH; k++)
+            if (hl[k] >= 0) nh[f][k] = lev0(hin[k], z1s + f, hoff[k]);
+
+    for (int i = 0; i <= N; ++i) {
+        const int z1 = z1s + i;
+        T *S0 = L0 + (i & 1) * L0SZ;
+        T *S1 = L1 + (i & 1) * L1SZ;
+        if (i < N) {
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                q0[v][2 * R] = nq0[0][v];
+                if (col) S0[l0c[v]] = q0[v][R];
+            }
+#pragma unroll
+            for (int k = 0; k < NH; k++)
+                if (hl[k] >= 0) S0[hl[k]] = nh[0][k];
+#pragma unroll
+            for (int f = 0; f + 1 < PF; f++) {
+#pragma unroll
+                for (int v = 0; v < VX; v++) nq0[f][v] = nq0[f + 1][v];
+#pragma unroll
+                for (int k = 0; k < NH; k++) nh[f][k] = nh[f + 1][k];
+            }
+            if (i + PF < N) {
+#pragma unroll
+                for (int v = 0; v < VX; v++) nq0[PF - 1][v] = lev0(xyin[v], z1 + R + PF, coff[v]);
+#pragma unroll
+                for (int k = 0; k < NH; k++)
+                    if (hl[k] >= 0) nh[PF - 1][k] = lev0(hin[k], z1 + PF, hoff[k]);
+            }
+        }
+        if (col) {
+#pragma unroll
+            for (int v = 0; v < VX; v++) S1[l1c[v]] = q1[v][R];
+        }
+        __syncthreads();
+        cf.refresh();
+        // Level 2 at plane z2 = z1 - 1 - R: the level-1 queue holds planes
+        // z1-1-2R .. z1-1, the LDS buffer plane z2.
+        const int z2 = z1 - 1 - R;
+        if (z2 >= zs) {
+            T *oz = out + (long long)z2 * sz;
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                if (!core[v]) continue; // ghost columns: their neighbours leave the tile
+                Acc<T, KAHAN> a;
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
+                    T val;
+                    if constexpr (dy == 0 && dx == 0) val = q1[v][R + dz];
+                    else val = S1[l1c[v] + dy * BX + dx];
+                    if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                    else a.add(cf.template get<t>(), val);
+                });
+                store_out(oz + coff[v], a.value(), g.nts);
+            }
+        }
+        if (i < N && col) {
+            // Level 1 at plane z1 for every extended column.
+            const bool zin = z1 >= 0 && z1 < n0;
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                T v1;
+                if (zin && xyin[v]) {
+                    Acc<T, KAHAN> a;
+                    static_for<P>([&](auto tc) {
+                        constexpr int t = decltype(tc)::value;
+                        constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
+                        T val;
+                        if constexpr (dy == 0 && dx == 0) val = q0[v][R + dz];
+                        else val = S0[l0c[v] + dy * LW + dx];
+                        if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                        else a.add(cf.template get<t>(), val);
+                    });
+                    v1 = a.value();
+                } else {
+                    v1 = h1[(long long)z1 * sz + coff[v]];
+                }
+#pragma unroll
+                for (int j = 0; j < 2 * R; j++) q1[v][j] = q1[v][j + 1];
+                q1[v][2 * R] = v1;
+#pragma unroll
+                for (int j = 0; j < 2 * R; j++) q0[v][j] = q0[v][j + 1];
+            }
+        }
+    }
+}
+
+
+// --------------------------------------------------------------------------
 // box3d (terms grouped by ascending axis-0 offset)
 // --------------------------------------------------------------------------
 template <class S> constexpr bool plane_major()
@@ -235,13 +441,14 @@ template <class S> constexpr int box_pack_index(int t)
 // coefficients in box_pack_index order.  Shared by the per-grid kernel and
 // the batched work-queue kernel.
 
-template <typename T, class S, int VX, bool KAHAN, int PDEEP, class CF>
+template <typename T, class S, int VX, bool KAHAN, int PDEEP, int GPF = 1, int NL = box_cells_per_thread(S::R),
+          class CF>
 __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restrict__ out,
                                            const TileGeom &p, int x0, int y0, int zs, int ze,
                                            T *lds, CF &cf)
 {
     static_assert(plane_major<S>(), "box3d needs terms in ascending axis-0 offset");
-    constexpr int R = S::R, P = S::P, NL = kBoxCellsPerThread;
+    constexpr int R = S::R, P = S::P;
     constexpr int DZMIN = S::OFF[0][0], DZMAX = S::OFF[P - 1][0];
     constexpr int NJ = DZMAX - DZMIN + 1; // pending output planes per point
     const int NT = blockDim.x, tid = threadIdx.x;
@@ -273,12 +480,14 @@ __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restri
                   clampi(x0 - R + lx, -R, p.n2 + R - 1);
     }
 
-    T pl[NL];
-    {
-        const T *pp = in + (long long)(zs + DZMIN) * sz;
+    // pl[f] holds plane zp + f of the staged tile (GPF planes in flight).
+    T pl[GPF][NL];
+#pragma unroll
+    for (int f = 0; f < GPF; f++) {
+        const T *pp = in + (long long)min(zs + DZMIN + f, p.n0 + R - 1) * sz;
 #pragma unroll
         for (int k = 0; k < NL; k++)
-            if (gv[k]) pl[k] = pp[goff[k]];
+            if (gv[k]) pl[f][k] = pp[goff[k]];
     }
     // acc[v][j] is the running sum of output plane zp - DZMAX + j.
     Acc<T, KAHAN> acc[VX][NJ];
@@ -292,12 +501,16 @@ __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restri
         T *L = lds + (s & 1) * LSZ;
 #pragma unroll
         for (int k = 0; k < NL; k++)
-            if (gv[k]) L[tid + k * NT] = pl[k];
+            if (gv[k]) L[tid + k * NT] = pl[0][k];
         {
-            const T *pn = in + (long long)min(zp + 1, p.n0 + R - 1) * sz;
+            const T *pn = in + (long long)min(zp + GPF, p.n0 + R - 1) * sz;
+#pragma unroll
+            for (int f = 0; f + 1 < GPF; f++)
+#pragma unroll
+                for (int k = 0; k < NL; k++) pl[f][k] = pl[f + 1][k];
 #pragma unroll
             for (int k = 0; k < NL; k++)
-                if (gv[k]) pl[k] = pn[goff[k]];
+                if (gv[k]) pl[GPF - 1][k] = pn[goff[k]];
         }
         __syncthreads();
         cf.refresh();
@@ -386,13 +599,13 @@ template <class S> constexpr bool row_grouped()
 // box3d with two output rows per thread (vec 1): every staged value is read
 // from LDS once and applied to the pending sums of both rows, which halves
 // the LDS reads per output -- the boxes' other bottleneck next to FP64 FMA.
-template <typename T, class S, bool KAHAN, class CF>
+template <typename T, class S, bool KAHAN, int GPF, int NL = box_cells_per_thread(S::R), class CF>
 __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__restrict__ out,
                                                  const TileGeom &p, int x0, int y0, int zs, int ze,
                                                  T *lds, CF &cf)
 {
     static_assert(row_grouped<S>(), "two-row box3d needs row-grouped plane cells");
-    constexpr int R = S::R, P = S::P, NL = kBoxCellsPerThread, W = 2 * R + 1;
+    constexpr int R = S::R, P = S::P, W = 2 * R + 1;
     constexpr int DZMIN = S::OFF[0][0], DZMAX = S::OFF[P - 1][0];
     constexpr int NJ = DZMAX - DZMIN + 1, G = P / NJ;
     const int NT = blockDim.x, tid = threadIdx.x;
@@ -420,12 +633,13 @@ __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__
         const int ly = gv[k] ? c / LW : 0, lx = gv[k] ? c % LW : 0;
         goff[k] = clampi(y0 - R + ly, -R, p.n1 + R - 1) * sy + clampi(x0 - R + lx, -R, p.n2 + R - 1);
     }
-    T pl[NL];
-    {
-        const T *pp = in + (long long)(zs + DZMIN) * sz;
+    T pl[GPF][NL];
+#pragma unroll
+    for (int f = 0; f < GPF; f++) {
+        const T *pp = in + (long long)min(zs + DZMIN + f, p.n0 + R - 1) * sz;
 #pragma unroll
         for (int k = 0; k < NL; k++)
-            if (gv[k]) pl[k] = pp[goff[k]];
+            if (gv[k]) pl[f][k] = pp[goff[k]];
     }
     Acc<T, KAHAN> acc[2][NJ];
 #pragma unroll
@@ -438,12 +652,16 @@ __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__
         T *L = lds + (s & 1) * LSZ;
 #pragma unroll
         for (int k = 0; k < NL; k++)
-            if (gv[k]) L[tid + k * NT] = pl[k];
+            if (gv[k]) L[tid + k * NT] = pl[0][k];
         {
-            const T *pn = in + (long long)min(zp + 1, p.n0 + R - 1) * sz;
+            const T *pn = in + (long long)min(zp + GPF, p.n0 + R - 1) * sz;
+#pragma unroll
+            for (int f = 0; f + 1 < GPF; f++)
+#pragma unroll
+                for (int k = 0; k < NL; k++) pl[f][k] = pl[f + 1][k];
 #pragma unroll
             for (int k = 0; k < NL; k++)
-                if (gv[k]) pl[k] = pn[goff[k]];
+                if (gv[k]) pl[GPF - 1][k] = pn[goff[k]];
         }
         __syncthreads();
         cf.refresh();
@@ -509,7 +727,7 @@ __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__
     }
 }
 
-template <typename T, class S, bool KAHAN>
+template <typename T, class S, bool KAHAN, int GPF>
 __global__ void __launch_bounds__(kMaxThreads) k_box3d_rows2(KParams<T, S::P> p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -520,22 +738,22 @@ __global__ void __launch_bounds__(kMaxThreads) k_box3d_rows2(KParams<T, S::P> p)
     const int zs = chunk * p.ZC;
     const TileGeom g{p.n0, p.n1, p.n2, p.sy, p.sz, p.TX, p.TY, p.TX, (p.flags & UHS_CFG_NT_STORE) != 0};
     Coefs<T, S::P> cf(p);
-    box3d_tile_rows2<T, S, KAHAN>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs, min(zs + p.ZC, p.n0),
+    box3d_tile_rows2<T, S, KAHAN, GPF>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs, min(zs + p.ZC, p.n0),
                                   reinterpret_cast<T *>(smem_raw), cf);
 }
 
-template <typename T, class S, bool KAHAN>
+template <typename T, class S, bool KAHAN, int GPF>
 hipError_t launch_box3d_rows2(const SweepArgs &a, hipStream_t st)
 {
     auto p = make_params<T, S::P>(a);
     for (int t = 0; t < S::P; t++) p.c[box_pack_index<S>(t)] = (T)a.coef[t];
     const int nt = a.tx * (a.ty / 2);
     const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R) * (a.ty + 2 * S::R);
-    hipLaunchKernelGGL((k_box3d_rows2<T, S, KAHAN>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    hipLaunchKernelGGL((k_box3d_rows2<T, S, KAHAN, GPF>), dim3(p.nblocks), dim3(nt), lds, st, p);
     return hipGetLastError();
 }
 
-template <typename T, class S, int VX, bool KAHAN, int PDEEP>
+template <typename T, class S, int VX, bool KAHAN, int PDEEP, int GPF>
 __global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -546,14 +764,14 @@ __global__ void __launch_bounds__(kMaxThreads) k_box3d(KParams<T, S::P> p)
     const int zs = chunk * p.ZC;
     const TileGeom g{p.n0, p.n1, p.n2, p.sy, p.sz, p.TX, p.TY, p.BXT, (p.flags & UHS_CFG_NT_STORE) != 0};
     Coefs<T, S::P> cf(p);
-    box3d_tile<T, S, VX, KAHAN, PDEEP>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs,
+    box3d_tile<T, S, VX, KAHAN, PDEEP, GPF>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs,
                                 min(zs + p.ZC, p.n0), reinterpret_cast<T *>(smem_raw), cf);
 }
 
 // --------------------------------------------------------------------------
 // win2d
 // --------------------------------------------------------------------------
-template <typename T, class S, int VX, bool KAHAN>
+template <typename T, class S, int VX, bool KAHAN, int PF>
 __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
 {
     constexpr int R = S::R, P = S::P, W = 2 * R + 1;
@@ -584,12 +802,14 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
     const int hlx = tid < R ? tid : TX + tid;
     const int hoff = clampi(x0 - R + hlx, -R, p.n2 + R - 1);
 
-    T nrow[VX], nh = T(0);
-    {
-        const T *pr = in + (long long)(ys - R) * sy;
+    // nrow[f] / nh[f]: row ys - R + s + f, PF rows in flight.
+    T nrow[PF][VX], nh[PF];
 #pragma unroll
-        for (int v = 0; v < VX; v++) nrow[v] = pr[xcl[v]];
-        if (hthr) nh = pr[hoff];
+    for (int f = 0; f < PF; f++) {
+        const T *pr = in + (long long)min(ys - R + f, p.n0 + R - 1) * sy;
+#pragma unroll
+        for (int v = 0; v < VX; v++) nrow[f][v] = pr[xcl[v]];
+        nh[f] = hthr ? pr[hoff] : T(0);
     }
     Coefs<T, P> cf(p);
     // w[v][a][b] = in[row - R + a][x_v - R + b] for the output row being built.
@@ -605,13 +825,19 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
         const int yr = ys - R + s;
         T *L = lds + (s & 1) * LW;
 #pragma unroll
-        for (int v = 0; v < VX; v++) L[R + tid + v * BXT] = nrow[v];
-        if (hthr) L[hlx] = nh;
+        for (int v = 0; v < VX; v++) L[R + tid + v * BXT] = nrow[0][v];
+        if (hthr) L[hlx] = nh[0];
         {
-            const T *pr = in + (long long)min(yr + 1, p.n0 + R - 1) * sy;
 #pragma unroll
-            for (int v = 0; v < VX; v++) nrow[v] = pr[xcl[v]];
-            if (hthr) nh = pr[hoff];
+            for (int f = 0; f + 1 < PF; f++) {
+#pragma unroll
+                for (int v = 0; v < VX; v++) nrow[f][v] = nrow[f + 1][v];
+                nh[f] = nh[f + 1];
+            }
+            const T *pr = in + (long long)min(yr + PF, p.n0 + R - 1) * sy;
+#pragma unroll
+            for (int v = 0; v < VX; v++) nrow[PF - 1][v] = pr[xcl[v]];
+            if (hthr) nh[PF - 1] = pr[hoff];
         }
         __syncthreads();
         cf.refresh();
@@ -688,23 +914,23 @@ hipError_t launch_star3d(const SweepArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-template <typename T, class S, int VX, bool KAHAN, int PDEEP>
+template <typename T, class S, int VX, bool KAHAN, int PDEEP, int GPF>
 hipError_t launch_box3d(const SweepArgs &a, hipStream_t st)
 {
     auto p = make_params<T, S::P>(a);
     for (int t = 0; t < S::P; t++) p.c[box_pack_index<S>(t)] = (T)a.coef[t];
     const int nt = (a.tx / VX) * a.ty;
     const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R) * (a.ty + 2 * S::R);
-    hipLaunchKernelGGL((k_box3d<T, S, VX, KAHAN, PDEEP>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    hipLaunchKernelGGL((k_box3d<T, S, VX, KAHAN, PDEEP, GPF>), dim3(p.nblocks), dim3(nt), lds, st, p);
     return hipGetLastError();
 }
 
-template <typename T, class S, int VX, bool KAHAN>
+template <typename T, class S, int VX, bool KAHAN, int PF>
 hipError_t launch_win2d(const SweepArgs &a, hipStream_t st)
 {
     auto p = make_params<T, S::P>(a);
     const int nt = a.tx / VX;
     const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R);
-    hipLaunchKernelGGL((k_win2d<T, S, VX, KAHAN>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    hipLaunchKernelGGL((k_win2d<T, S, VX, KAHAN, PF>), dim3(p.nblocks), dim3(nt), lds, st, p);
     return hipGetLastError();
 }

@@ -127,10 +127,13 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
     static const int txs2[] = {64, 128, 256, 512, 1024};
     const std::vector<int> vecs = pl.family == FAM_STAR3D ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1, 2, 4};
     const bool d3 = pl.args.dims == 3;
-    std::vector<int> flagset{0, UHS_CFG_NT_STORE};
-    if (pl.family == FAM_BOX3D)
-        flagset = {0, UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2, UHS_CFG_ROWS2,
-                   UHS_CFG_NT_STORE | UHS_CFG_ROWS2};
+    std::vector<int> flagset{0, UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2,
+                             UHS_CFG_PREFETCH4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4};
+    if (pl.family == FAM_BOX3D) {
+        flagset.clear();
+        for (int base : {0, UHS_CFG_PREFETCH2, UHS_CFG_ROWS2, UHS_CFG_PLANES2, UHS_CFG_ROWS2 | UHS_CFG_PLANES2})
+            for (int nt : {0, UHS_CFG_NT_STORE}) flagset.push_back(base | nt);
+    }
     if (pl.family == FAM_STAR3D)
         flagset = {0, UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2, UHS_CFG_PREFETCH4,
                    UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4};
@@ -147,6 +150,20 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
                         if (tz > 2 * pl.args.n0 && tz > 16) continue;
                         out.push_back(c);
                     }
+    if (pl.family == FAM_STAR3D) { // two steps per launch: output tile + ghost zone of 32/64/128 columns
+        const int R = pl.radius;
+        for (int bx : {32, 64, 128})
+            for (int by : {6, 8, 10, 12, 16})
+                for (int tz : {64, 128, 256})
+                    for (int v : {1, 2})
+                        for (int fl : {UHS_CFG_TEMPORAL2, UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE}) {
+                            if (by <= 2 * R) continue;
+                            uhs_config c{bx - 2 * R, by - 2 * R, tz, v, fl};
+                            if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) continue;
+                            if (tz > 2 * pl.args.n0 && tz > 16) continue;
+                            out.push_back(c);
+                        }
+    }
     if (pl.family == FAM_STAR3D) // register-tiled variants
         for (int rows : {2, 4, 8})
             for (int waves : {1, 2, 4, 8})
@@ -160,25 +177,25 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
     return out;
 }
 
-// Times `reps` sweeps of one configuration (after one warm-up sweep).
+// Times `reps` sweeps of one configuration (after one warm-up sweep).  A
+// two-steps-per-launch configuration is timed over a multiple of 4 steps
+// (an even number of two-step launches, no leading single steps).
 int uhs_time_config(Plan &pl, void *A, void *B, const uhs_config &c, int reps, hipStream_t st,
                     double *ms)
 {
     Plan p = pl;
     char why[160];
     if (uhs_check_config(p, c, why, sizeof why) != UHS_OK) return UHS_E_CONFIG;
-    p.cfg = c;
-    p.args.tx = c.tile_x; p.args.ty = c.tile_y; p.args.tz = c.tile_z; p.args.vec = c.vec;
-    p.args.flags = c.flags;
-    p.fn = uhs_pick_fn(p, c);
-    if (!p.fn) return UHS_E_CONFIG;
+    if (uhs_plan_apply(p, c) != UHS_OK) return UHS_E_CONFIG;
+    const bool two = c.flags & UHS_CFG_TEMPORAL2;
+    if (two) reps = (reps + 3) / 4 * 4;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return UHS_E_HIP;
     if (hipEventCreate(&e1) != hipSuccess) {
         (void)hipEventDestroy(e0);
         return UHS_E_HIP;
     }
-    int rc = uhs_run_plan(p, A, B, 1, st);
+    int rc = uhs_run_plan(p, A, B, two ? 4 : 1, st);
     if (rc == UHS_OK) {
         (void)hipEventRecord(e0, st);
         rc = uhs_run_plan(p, A, B, reps, st);

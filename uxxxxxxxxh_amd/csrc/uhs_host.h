@@ -24,6 +24,9 @@ int uhs_match_builtin(int dims, int npoints, const int *off);
 int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,
                   const uhs_config *cfg, Plan *pl);
 int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream);
+// Sets a checked configuration on a plan: tiling arguments, kernel, and for
+// UHS_CFG_TEMPORAL2 the single-step kernel of the leading steps.
+int uhs_plan_apply(Plan &pl, const uhs_config &c);
 int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whylen);
 uhs_config uhs_heuristic_config(const Plan &pl);
 // The launcher for a plan under configuration c (nullptr: not compiled).

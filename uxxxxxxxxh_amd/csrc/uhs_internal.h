@@ -44,15 +44,19 @@ typedef hipError_t (*LaunchFn)(const SweepArgs &, hipStream_t);
 // Per-family kernel tables (nullptr when the combination is not compiled).
 LaunchFn uhs_find_star3d(int prog, int dtype, int vec, int prefetch);
 LaunchFn uhs_find_star3d_rt(int prog, int dtype, int rows);
-LaunchFn uhs_find_star3d_tb2(int prog, int dtype, int vec); // two steps per launch
-LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep);
-LaunchFn uhs_find_win2d(int prog, int dtype, int vec);
+LaunchFn uhs_find_star3d_tb2(int prog, int dtype, int vec, int prefetch); // two steps per launch
+LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep, int planes);
+LaunchFn uhs_find_win2d(int prog, int dtype, int vec, int prefetch);
 LaunchFn uhs_find_gather(int dtype);
 
 // Limits baked into the kernels; the host rejects configurations beyond them.
 constexpr int kMaxThreads = 512;      // __launch_bounds__ of every sweep kernel
 constexpr int kStarHaloPerThread = 4; // star3d: halo cells staged per thread per plane
-constexpr int kBoxCellsPerThread = 8; // box3d: plane-tile cells staged per thread
+// box3d: plane-tile cells staged per thread.  Radius 1 keeps 8 (its
+// best tiles run faster at the occupancy that register count gives);
+// radius 2 and 3 keep 4, which frees 16-32 VGPRs for more waves.
+constexpr int box_cells_per_thread(int radius) { return radius == 1 ? 8 : 4; }
+constexpr int kTb2RingCells = 2;       // star3d_tb2: ring cells staged per thread per plane
 constexpr int kGatherMaxPoints = 256; // gather: terms passed in kernel arguments
 constexpr size_t kMaxLdsBytes = 64 * 1024;
 

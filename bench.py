@@ -230,6 +230,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-temporal", action="store_true")
+    ap.add_argument("--temporal-only", action="store_true",
+                    help="only the two-steps-per-launch sweep (its traffic-counter run)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the copy kernel (traffic-counter calibration: 2 GiB read + written)")
     args = ap.parse_args()
@@ -242,6 +244,9 @@ def main():
     u.lib()
     if args.copy_only:
         print(json.dumps({"copy_GBps": round(copy_bandwidth(u), 1)}))
+        return
+    if args.temporal_only:
+        print(json.dumps(temporal2(u, args.size, args.timesteps)))
         return
     name = __import__("ctypes").create_string_buffer(256)
     u.check(u.lib().uhs_device_name(name, 256))

@@ -80,8 +80,7 @@ typedef struct uhs_stencil {
  *           UHS_CFG_PREFETCH2: 3D stars keep two planes of loads in flight
  *           (2D stencils two rows), 3D boxes fetch staged cells three ahead
  *           instead of one;
- *           UHS_CFG_PREFETCH4: 3D stars keep four planes in flight (2D
- *           stencils four rows);
+ *           UHS_CFG_PREFETCH4: 2D stencils keep four rows in flight;
  *           UHS_CFG_ROWS2: 3D boxes compute two rows per thread (vec must
  *           be 1, tile_y even; tile_x * tile_y / 2 threads), halving the
  *           LDS reads per output;

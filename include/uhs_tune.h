@@ -81,7 +81,7 @@ typedef struct uhs_mlp *uhs_mlp_t;
  * points per thread (1), temporal depth one-hot {1,2,4} (3; UHS_CFG_TEMPORAL2
  * counts as depth 2), non-temporal store flag (1), kernel variant (1: 0
  * LDS-staged, 1 deeper prefetch (two planes for stars and two rows for 2D,
- * three cells for boxes), 2 four-plane prefetch, 3 register-tiled, 4 two
+ * three cells for boxes), 2 four rows of prefetch (2D), 3 register-tiled, 4 two
  * rows per thread, 5 two planes of box loads in flight, 6 both 4 and 5).
  * Threads per workgroup follow the variant (ghost zone included for
  * UHS_CFG_TEMPORAL2). */

@@ -193,8 +193,8 @@ def test_two_steps_per_launch(gpu, orc, case, dtype):
         A0, B0 = A0.astype(dt), B0.astype(dt)
         mask = np.ones(A0.shape, bool)
         mask[tuple(slice(h, n - h) for n in A0.shape)] = False
-        for cfg in ((10, 4, 5, 2, T2), (30, 6, 7, 1, T2 | NT), (62, 6, 64, 2, T2), (14, 6, 3, 2, T2 | u.CFG_PREFETCH2),
-                    (30, 4, 64, 1, T2 | NT | u.CFG_PREFETCH2)):
+        for cfg in ((10, 4, 5, 2, T2), (30, 6, 7, 1, T2 | NT), (62, 6, 64, 2, T2), (14, 6, 3, 2, T2),
+                    (30, 4, 64, 1, T2 | NT)):
             for T in (1, 2, 3, 4, 5, 6):
                 log(case, dtype, ext, h, cfg, "T", T)
                 gA, gB = run_gpu(u, st, A0, B0, ext, h, T, dt, dtype == "f32k", cfg=cfg)
@@ -214,7 +214,8 @@ def test_two_steps_per_launch_config_checks(gpu):
     for bad in ((61, 4, 64, 2, T2),                    # tile_x + 2r odd with vec 2
                 (64, 8, 64, 4, T2),                    # vec 4
                 (124, 8, 64, 2, T2),                   # 64 * 12 threads
-                (60, 4, 64, 2, T2 | u.CFG_ROWS2)):      # no other variant flags
+                (60, 4, 64, 2, T2 | u.CFG_ROWS2),       # no other variant flags
+                (60, 4, 64, 2, T2 | u.CFG_PREFETCH2)):
         with pytest.raises(u.UhsError) as e:
             u.plan_kind(ext, 2, st, cfg=bad)
         assert e.value.code == -3

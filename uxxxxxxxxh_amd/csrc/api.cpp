@@ -146,8 +146,8 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
     if (c.flags & UHS_CFG_TEMPORAL2) {
         if (pl.family != FAM_STAR3D) return bad("two steps per launch exists for 3D star stencils only");
-        if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2))
-            return bad("two steps per launch combines with UHS_CFG_NT_STORE and UHS_CFG_PREFETCH2 only");
+        if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE))
+            return bad("two steps per launch combines with UHS_CFG_NT_STORE only");
         const long long bx = c.tile_x + 2LL * R, by = c.tile_y + 2LL * R;
         if (c.vec != 1 && c.vec != 2) return bad(str("two steps per launch needs vec 1 or 2 (got %lld)", c.vec));
         if (bx % c.vec) return bad(str("tile_x + 2r = %lld not a multiple of vec %lld", bx, c.vec));
@@ -167,9 +167,8 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         if (c.vec != 1 || (c.tile_y & 1) || (c.flags & UHS_CFG_PREFETCH2))
             return bad("two rows per thread needs vec 1, an even tile_y and no deeper prefetch");
     }
-    if ((c.flags & UHS_CFG_PREFETCH4) &&
-        ((c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE)) || (pl.family != FAM_STAR3D && pl.family != FAM_WIN2D)))
-        return bad("four-plane prefetch is a 3D-star and 2D variant of its own");
+    if ((c.flags & UHS_CFG_PREFETCH4) && ((c.flags & UHS_CFG_PREFETCH2) || pl.family != FAM_WIN2D))
+        return bad("four rows of prefetch is a 2D variant of its own");
     if (c.flags & UHS_CFG_REGTILE) {
         if (pl.family != FAM_STAR3D) return bad("the register-tiled kernel exists for 3D star stencils only");
         if (c.flags & UHS_CFG_PREFETCH2) return bad("the register-tiled kernel has no two-plane prefetch variant");

@@ -118,7 +118,7 @@ extern "C" {
 int uhs_batch_default_config(uhs_batch_config *c)
 {
     if (!c) return berr(UHS_E_INVAL, "null config");
-    *c = uhs_batch_config{2, 30, 14, 64, 32, 16, 128, UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2};
+    *c = uhs_batch_config{2, 32, 12, 64, 32, 16, 128, UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2};
     return UHS_OK;
 }
 
@@ -423,7 +423,7 @@ int uhs_batch_tune(const uhs_grid_desc *grids, int n, int timesteps, uhs_batch_c
     // steps), and for depth 4 the LDS-ring form.  Box tiles: 256 points, or
     // 512 with two rows per thread; with one or two planes in flight.
     static const int s1[][2] = {{64, 8}, {128, 4}, {32, 16}};
-    static const int s2[][2] = {{62, 6}, {30, 14}, {30, 6}, {64, 4}};
+    static const int s2[][2] = {{62, 6}, {30, 14}, {32, 12}, {32, 8}, {64, 4}};
     static const int s4[][2] = {{32, 8}};
     static const int stz[] = {64, 512};
     static const int btiles[][3] = {{32, 8, 0},
@@ -443,7 +443,7 @@ int uhs_batch_tune(const uhs_grid_desc *grids, int n, int timesteps, uhs_batch_c
                             cand.push_back(uhs_batch_config{d, tiles[i][0], tiles[i][1], sz, b[0], b[1], bz, b[2] | pl});
     };
     add(1, s1, 3);
-    add(2, s2, 4);
+    add(2, s2, 5);
     add(4, s4, 1);
     double bms = 1e30;
     hipEvent_t e0, e1;

@@ -1,6 +1,6 @@
 // star3d kernel instantiations: the three 3D star benchmark stencils, FP64,
 // FP32 and FP32 compensated, 1/2/4/8 points per thread along the unit-stride
-// axis, one, two or four planes of prefetch.
+// axis, one or two planes of prefetch.
 #include "sweep_kernels.h"
 
 namespace {
@@ -28,7 +28,6 @@ template <class S> LaunchFn pick(int dtype, int vec, int prefetch)
     switch (prefetch) {
     case 1: return pick_pf<S, 1>(dtype, vec);
     case 2: return pick_pf<S, 2>(dtype, vec);
-    case 4: return pick_pf<S, 4>(dtype, vec);
     }
     return nullptr;
 }

@@ -1,6 +1,5 @@
 // star3d_tb2 kernel: two timesteps per launch for the 3D star stencils
-// (star3d_tb2_tile), FP64, FP32 and FP32 compensated, 1/2 columns per thread,
-// one or two planes of loads in flight.
+// (star3d_tb2_tile), FP64, FP32 and FP32 compensated, 1/2 columns per thread.
 #include "sweep_kernels.h"
 
 namespace {
@@ -49,11 +48,10 @@ hipError_t launch_star3d_tb2(const SweepArgs &a, hipStream_t st)
 
 template <typename T, bool K, class S> LaunchFn pick_vec(int vec, int pf)
 {
-    switch (vec * 4 + pf) {
-    case 5: return launch_star3d_tb2<T, S, 1, K, 1>;
-    case 6: return launch_star3d_tb2<T, S, 1, K, 2>;
-    case 9: return launch_star3d_tb2<T, S, 2, K, 1>;
-    case 10: return launch_star3d_tb2<T, S, 2, K, 2>;
+    if (pf != 1) return nullptr;
+    switch (vec) {
+    case 1: return launch_star3d_tb2<T, S, 1, K, 1>;
+    case 2: return launch_star3d_tb2<T, S, 2, K, 1>;
     }
     return nullptr;
 }

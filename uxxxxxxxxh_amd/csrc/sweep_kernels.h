@@ -58,7 +58,7 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
                                             const TileGeom &p, int x0, int y0, int zs, int ze,
                                             T *lds, CF &cf)
 {
-    constexpr int R = S::R, P = S::P, Q = 2 * R + 1, NH = kStarHaloPerThread;
+    constexpr int R = S::R, P = S::P, NH = kStarHaloPerThread;
     const int NT = blockDim.x, tid = threadIdx.x;
     const int BXT = p.BXT, TX = p.TX, TY = p.TY;
     const int LW = TX + 2 * R, LSZ = LW * (TY + 2 * R);
@@ -103,45 +103,44 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
         }
     }
 
-    // Register queue: q[v][j] = plane z-R+j at this thread's point v; nq[f]
-    // holds plane z+R+f, in flight (PF planes of prefetch).
-    T q[VX][Q], nq[PF][VX], nh[NH];
+    // Register ring: plane p of this thread's point v lives in slot
+    // (p - zs + R) mod M -- 2R+1 planes in use, PF in flight.  The halo ring
+    // of plane p lives in slot (p - zs) mod 2.  The z loop is unrolled by hand
+    // by U (a multiple of M and of 2; the compiler does not unroll a loop
+    // with a barrier), so every slot index is a constant and no value still
+    // in flight is ever copied (a copy would make the loop wait for it).
+    constexpr int M = 2 * R + 1 + PF, U = (M % 2 == 0) ? M : 2 * M;
+    T rq[VX][M], rh[2][NH];
     const bool nts = p.nts;
 #pragma unroll
-    for (int v = 0; v < VX; v++) {
+    for (int v = 0; v < VX; v++)
 #pragma unroll
-        for (int j = 0; j < 2 * R; j++) q[v][j] = in[(long long)(zs - R + j) * sz + coff[v]];
-#pragma unroll
-        for (int f = 0; f < PF; f++)
-            nq[f][v] = in[(long long)min(zs + R + f, p.n0 + R - 1) * sz + coff[v]];
-    }
+        for (int j = 0; j < 2 * R + PF; j++)
+            rq[v][j] = in[(long long)min(zs - R + j, p.n0 + R - 1) * sz + coff[v]];
 #pragma unroll
     for (int k = 0; k < NH; k++)
-        if (hl[k] >= 0) nh[k] = in[(long long)zs * sz + hoff[k]];
+        if (hl[k] >= 0) rh[0][k] = in[(long long)zs * sz + hoff[k]];
 
-    for (int z = zs; z < ze; ++z) {
-        T *L = lds + (z & 1) * LSZ;
+    auto step = [&](auto phc, int z) {
+        constexpr int PH = decltype(phc)::value; // (z - zs) mod U
+        constexpr int C = (PH + R) % M;           // slot of plane z
+        T *L = lds + (PH & 1) * LSZ;
 #pragma unroll
-        for (int v = 0; v < VX; v++) {
-            q[v][2 * R] = nq[0][v];
-            L[lc[v]] = q[v][R];
-        }
+        for (int v = 0; v < VX; v++) L[lc[v]] = rq[v][C];
 #pragma unroll
         for (int k = 0; k < NH; k++)
-            if (hl[k] >= 0) L[hl[k]] = nh[k];
-        // Prefetch plane z+R+PF (own column) and the halo of plane z+1.
+            if (hl[k] >= 0) L[hl[k]] = rh[PH & 1][k];
+        // Halo of plane z+1 first (exec-masked), then plane z+R+PF of the own
+        // points into the slot of plane z-R-1: loads issued on every path
+        // come last, so later waits can count them.
         {
-            const T *pn = in + (long long)min(z + R + PF, p.n0 + R - 1) * sz;
             const T *ph = in + (long long)(z + 1) * sz;
 #pragma unroll
-            for (int f = 0; f + 1 < PF; f++)
-#pragma unroll
-                for (int v = 0; v < VX; v++) nq[f][v] = nq[f + 1][v];
-#pragma unroll
-            for (int v = 0; v < VX; v++) nq[PF - 1][v] = pn[coff[v]];
-#pragma unroll
             for (int k = 0; k < NH; k++)
-                if (hl[k] >= 0) nh[k] = ph[hoff[k]];
+                if (hl[k] >= 0) rh[(PH + 1) & 1][k] = ph[hoff[k]];
+            const T *pn = in + (long long)min(z + R + PF, p.n0 + R - 1) * sz;
+#pragma unroll
+            for (int v = 0; v < VX; v++) rq[v][(PH + 2 * R + PF) % M] = pn[coff[v]];
         }
         __syncthreads();
         cf.refresh();
@@ -154,7 +153,7 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
                 constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
                 T val;
                 if constexpr (dy == 0 && dx == 0) {
-                    val = q[v][R + dz];
+                    val = rq[v][(PH + R + dz) % M];
                 } else {
                     static_assert(dz == 0, "star3d needs axis-aligned terms");
                     val = L[lc[v] + dy * LW + dx];
@@ -164,11 +163,12 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
             });
             if (act[v]) store_out(oz + coff[v], a.value(), nts);
         }
-#pragma unroll
-        for (int v = 0; v < VX; v++)
-#pragma unroll
-            for (int j = 0; j < 2 * R; j++) q[v][j] = q[v][j + 1];
-    }
+    };
+    for (int z0 = zs; z0 < ze; z0 += U)
+        static_for<U>([&](auto phc) {
+            const int z = z0 + decltype(phc)::value;
+            if (z < ze) step(phc, z);
+        });
 }
 
 template <typename T, class S, int VX, bool KAHAN, int PF>
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
 // axis.  It streams planes along axis 0.  At iteration i (plane z1 of
 // level 1):
 //   level 0 (the input): each thread holds its columns' 2R+1 planes around z1
-//     in a register queue (plus one plane in flight) and stages plane z1,
+//     in a register ring (plus one plane in flight) and stages plane z1,
 //     with an R-wide ring around the extended tile, in LDS buffer i&1;
 //   level 1 (one step): the thread stages plane z1-1-R of level 1 (the
 //     centre of its level-1 register queue) in LDS buffer i&1 too;
@@ -287,59 +287,68 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
 
     const int z1s = zs - R;             // first level-1 plane
     const int N = (ze - zs) + 2 * R;    // level-1 planes
-    // nq0[f]: plane z1+R+f of the own columns; nh[f]: plane z1+f of the
-    // ring (PF planes in flight).
-    T q0[VX][Q], q1[VX][Q], nq0[PF][VX], nh[PF][NH];
+    // Register rings instead of shifting queues: level-0 plane p of the own
+    // columns lives in slot (p - z1s + R) mod M0 (2R+1 planes in use, one in
+    // flight), the ring cells of plane p in slot (p - z1s) mod 2.  The loop
+    // below is unrolled by M0 by hand (the compiler does not unroll a loop
+    // with a barrier), so every slot index is a constant: a loaded value stays
+    // in its register until its last use, and no copy makes the loop wait for
+    // a load still in flight.  Level-1 plane p of the own columns lives in
+    // slot (p - z1s) mod M0 (it may be a halo value loaded from memory, so it
+    // must not be copied either).
+    static_assert(PF == 1, "one plane in flight");
+    constexpr int M0 = Q + 1;
+    // rh1[v][i & 1]: the level-1 halo value of plane z1 for a column that
+    // takes it (outside the interior, or z1 outside it), loaded one
+    // iteration ahead into a register of its own.
+    T r0[VX][M0], rh[2][NH], r1[VX][M0], rh1[VX][2];
+    auto takes_halo1 = [&](int v, int z) { return !(xyin[v] && z >= 0 && z < n0); };
+    auto halo1 = [&](int v, int z) { return h1[(long long)clampi(z, -R, n0 + R - 1) * sz + coff[v]]; };
 #pragma unroll
     for (int v = 0; v < VX; v++) {
 #pragma unroll
-        for (int j = 0; j < 2 * R; j++) q0[v][j] = lev0(xyin[v], z1s - R + j, coff[v]);
+        for (int j = 0; j < Q; j++) r0[v][j] = lev0(xyin[v], z1s - R + j, coff[v]);
 #pragma unroll
-        for (int f = 0; f < PF; f++) nq0[f][v] = lev0(xyin[v], z1s + R + f, coff[v]);
-#pragma unroll
-        for (int j = 0; j < Q; j++) q1[v][j] = T(0);
+        for (int j = 0; j < M0; j++) r1[v][j] = T(0);
+        rh1[v][0] = rh1[v][1] = T(0);
+        if (col && takes_halo1(v, z1s)) rh1[v][0] = halo1(v, z1s);
     }
 #pragma unroll
-    for (int f = 0; f < PF; f++)
+    for (int k = 0; k < NH; k++) rh[0][k] = lev0(hin[k], z1s, hoff[k]);
+
+    // One iteration, i = z1 - z1s with i % M0 == PH (and i % 2 == PH % 2,
+    // M0 being even).
+    auto step = [&](auto phc, int i) {
+        constexpr int PH = decltype(phc)::value;
+        const int z1 = z1s + i;
+        T *S0 = L0 + (PH & 1) * L0SZ;
+        T *S1 = L1 + (PH & 1) * L1SZ;
+        // Staging and prefetch run on every iteration, also the last (i == N,
+        // whose staged plane nobody reads): loads issued on every path let
+        // the compiler count them when it waits for older loads, instead of
+        // waiting for all of them.
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+            if (col) S0[l0c[v]] = r0[v][(PH + R) % M0];
 #pragma unroll
         for (int k = 0; k < NH; k++)
-            if (hl[k] >= 0) nh[f][k] = lev0(hin[k], z1s + f, hoff[k]);
-
-    for (int i = 0; i <= N; ++i) {
-        const int z1 = z1s + i;
-        T *S0 = L0 + (i & 1) * L0SZ;
-        T *S1 = L1 + (i & 1) * L1SZ;
-        if (i < N) {
+            if (hl[k] >= 0) S0[hl[k]] = rh[PH & 1][k];
+        // Plane z1+R+1 into the slot of plane z1-R-1 (no longer used), the
+        // ring of plane z1+1 into the other ring slot.  Addresses are clamped,
+        // so the loads past the slab end are harmless.
 #pragma unroll
-            for (int v = 0; v < VX; v++) {
-                q0[v][2 * R] = nq0[0][v];
-                if (col) S0[l0c[v]] = q0[v][R];
-            }
+        for (int v = 0; v < VX; v++) r0[v][(PH + M0 - 1) % M0] = lev0(xyin[v], z1 + R + 1, coff[v]);
 #pragma unroll
-            for (int k = 0; k < NH; k++)
-                if (hl[k] >= 0) S0[hl[k]] = nh[0][k];
-#pragma unroll
-            for (int f = 0; f + 1 < PF; f++) {
-#pragma unroll
-                for (int v = 0; v < VX; v++) nq0[f][v] = nq0[f + 1][v];
-#pragma unroll
-                for (int k = 0; k < NH; k++) nh[f][k] = nh[f + 1][k];
-            }
-            if (i + PF < N) {
-#pragma unroll
-                for (int v = 0; v < VX; v++) nq0[PF - 1][v] = lev0(xyin[v], z1 + R + PF, coff[v]);
-#pragma unroll
-                for (int k = 0; k < NH; k++)
-                    if (hl[k] >= 0) nh[PF - 1][k] = lev0(hin[k], z1 + PF, hoff[k]);
-            }
-        }
+        for (int k = 0; k < NH; k++) rh[(PH + 1) & 1][k] = lev0(hin[k], z1 + 1, hoff[k]);
+        // Level-1 plane z1-1+d (d <= 0) is in slot (PH - 1 + d) mod M0.
+        constexpr int L1S = PH - 1 - R + 2 * M0; // slot of plane z2 = z1-1-R, plus 2*M0
         if (col) {
 #pragma unroll
-            for (int v = 0; v < VX; v++) S1[l1c[v]] = q1[v][R];
+            for (int v = 0; v < VX; v++) S1[l1c[v]] = r1[v][L1S % M0];
         }
         __syncthreads();
         cf.refresh();
-        // Level 2 at plane z2 = z1 - 1 - R: the level-1 queue holds planes
+        // Level 2 at plane z2 = z1 - 1 - R: the level-1 ring holds planes
         // z1-1-2R .. z1-1, the LDS buffer plane z2.
         const int z2 = z1 - 1 - R;
         if (z2 >= zs) {
@@ -352,7 +361,7 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
                     constexpr int t = decltype(tc)::value;
                     constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
                     T val;
-                    if constexpr (dy == 0 && dx == 0) val = q1[v][R + dz];
+                    if constexpr (dy == 0 && dx == 0) val = r1[v][(L1S + dz) % M0];
                     else val = S1[l1c[v] + dy * BX + dx];
                     if constexpr (t == 0) a.first(cf.template get<t>(), val);
                     else a.add(cf.template get<t>(), val);
@@ -361,7 +370,11 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
             }
         }
         if (i < N && col) {
-            // Level 1 at plane z1 for every extended column.
+            // Level 1 at plane z1 for every extended column; first the halo
+            // values the columns outside the interior take at plane z1+1.
+#pragma unroll
+            for (int v = 0; v < VX; v++)
+                if (takes_halo1(v, z1 + 1)) rh1[v][(PH + 1) & 1] = halo1(v, z1 + 1);
             const bool zin = z1 >= 0 && z1 < n0;
 #pragma unroll
             for (int v = 0; v < VX; v++) {
@@ -372,23 +385,24 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
                         constexpr int t = decltype(tc)::value;
                         constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
                         T val;
-                        if constexpr (dy == 0 && dx == 0) val = q0[v][R + dz];
+                        if constexpr (dy == 0 && dx == 0) val = r0[v][(PH + R + dz) % M0];
                         else val = S0[l0c[v] + dy * LW + dx];
                         if constexpr (t == 0) a.first(cf.template get<t>(), val);
                         else a.add(cf.template get<t>(), val);
                     });
                     v1 = a.value();
                 } else {
-                    v1 = h1[(long long)z1 * sz + coff[v]];
+                    v1 = rh1[v][PH & 1];
                 }
-#pragma unroll
-                for (int j = 0; j < 2 * R; j++) q1[v][j] = q1[v][j + 1];
-                q1[v][2 * R] = v1;
-#pragma unroll
-                for (int j = 0; j < 2 * R; j++) q0[v][j] = q0[v][j + 1];
+                r1[v][PH] = v1; // the slot of plane z1 - M0, last used above
             }
         }
-    }
+    };
+    for (int i0 = 0; i0 <= N; i0 += M0)
+        static_for<M0>([&](auto phc) {
+            const int i = i0 + decltype(phc)::value;
+            if (i <= N) step(phc, i);
+        });
 }
 
 

@@ -135,8 +135,7 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
             for (int nt : {0, UHS_CFG_NT_STORE}) flagset.push_back(base | nt);
     }
     if (pl.family == FAM_STAR3D)
-        flagset = {0, UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2, UHS_CFG_PREFETCH4,
-                   UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4};
+        flagset = {0, UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2};
     char why[160];
     for (int tx : (d3 ? std::vector<int>(txs3, txs3 + 5) : std::vector<int>(txs2, txs2 + 5)))
         for (int ty : (d3 ? std::vector<int>(tys3, tys3 + 5) : std::vector<int>{1}))

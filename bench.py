@@ -64,8 +64,8 @@ def time_sweeps(u, case, ext, T, steps, warmup, seed=1, cfg=None):
 
 # Two steps per launch (temporal blocking), the fastest configuration of a
 # timed sweep at 512^3: 62x6 output tile (64x8 with the ghost zone), 2 columns
-# per thread, 128-plane slabs, non-temporal stores.
-TEMPORAL2_CFG = (62, 6, 128, 2, 1 | 32)
+# per thread, 128-plane slabs.
+TEMPORAL2_CFG = (62, 6, 128, 2, 32)
 
 
 def temporal2(u, n, T):
@@ -207,7 +207,11 @@ def cpu_baseline(ext, case="3d1r_star", budget_s=10.0):
 def kernel_name(cfg):
     if cfg.flags & 4:
         return "k_star3d_rt<double,Prog_3d1r_star,%d,false>" % cfg.vec
-    return "k_star3d<double,Prog_3d1r_star,%d,false,%d>" % (cfg.vec, 2 if cfg.flags & 2 else 1)
+    nt = cfg.tile_x // cfg.vec * cfg.tile_y
+    ring = 2 * (cfg.tile_x + cfg.tile_y)
+    nh = -(-ring // nt)
+    nh = 1 if nh <= 1 else 2 if nh <= 2 else 4
+    return "k_star3d<double,Prog_3d1r_star,%d,false,%d,%d>" % (cfg.vec, 2 if cfg.flags & 2 else 1, nh)
 
 
 def load_traffic(case, ext):

@@ -65,3 +65,54 @@ def test_model_pruned_search(gpu, tmp_path):
     assert r.best_ms <= 1.5 * full.best_ms
     m.close(); A.close(); B.close()
     u.check(u.lib().uhs_tune_clear())
+
+
+def _db_rows(path):
+    rows = [l.split(",") for l in open(path).read().strip().splitlines()]
+    return [r for r in rows if r[-1] == "0" and r[0] != "device"]
+
+
+def test_model_gains_over_choosing_without_it(gpu, tmp_path):
+    """Train the network on the autotuner's timings of three 3d2r_box
+    problems, then on a held-out extent: its top-5 (timed) must beat the
+    median of 25 random 5-configuration picks (the method without the
+    network), and its predicted ranking of the whole space must follow the
+    measured one (rank correlation > 0.6)."""
+    u = gpu
+    u.check(u.lib().uhs_tune_clear())
+    rng = np.random.default_rng(5)
+    X, y = [], []
+    for i, ext in enumerate(([96, 128, 160], [160, 96, 128], [128, 160, 96])):
+        st, h, A, B = grids(u, "3d2r_box", ext, seed=20 + i)
+        db = str(tmp_path / ("train%d.csv" % i))
+        u.tune_sweep(A, B, ext, h, st, reps=3, db=db)
+        for r in _db_rows(db):
+            X.append(u.mlp_features(3, 2, False, ext, tuple(int(v) for v in r[8:13])))
+            y.append(np.log(float(r[13])))
+        A.close(); B.close()
+    m = u.MLP(6)
+    m.train(np.array(X), np.array(y), epochs=200, batch=64, lr=2e-3)
+
+    ext = [128, 128, 128]
+    st, h, A, B = grids(u, "3d2r_box", ext, seed=30)
+    db = str(tmp_path / "held.csv")
+    full = u.tune_sweep(A, B, ext, h, st, reps=3, db=db)
+    held = _db_rows(db)
+    cfgs = [tuple(int(v) for v in r[8:13]) for r in held]
+    ms = np.array([float(r[13]) for r in held])
+    pred = m.predict(np.array([u.mlp_features(3, 2, False, ext, c) for c in cfgs]))
+    rank = lambda a: np.argsort(np.argsort(a))
+    corr = float(np.corrcoef(rank(pred), rank(ms))[0, 1])
+    # Both picks are scored on the same exhaustive timings (no re-timing noise).
+    model_best = float(np.min(ms[np.argsort(pred)[:5]]))
+    random_best = sorted(float(np.min(ms[rng.choice(len(ms), 5, replace=False)])) for _ in range(25))
+    median_random = random_best[len(random_best) // 2]
+    pruned = u.tune_sweep(A, B, ext, h, st, reps=3, db=None, model=m, topk=5)  # the C++ pruned search
+    print("held-out 3d2r_box 128^3 (%d configurations): exhaustive %.4f ms, model top-5 %.4f ms "
+          "(pruned search re-timed %.4f ms), median random-5 %.4f ms, rank corr %.3f"
+          % (len(ms), full.best_ms, model_best, pruned.best_ms, median_random, corr))
+    assert pruned.evaluated == 5
+    assert corr > 0.6
+    assert model_best <= median_random
+    m.close(); A.close(); B.close()
+    u.check(u.lib().uhs_tune_clear())

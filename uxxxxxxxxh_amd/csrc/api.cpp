@@ -108,8 +108,13 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
         if (c.flags & UHS_CFG_TEMPORAL2)
             return uhs_find_star3d_tb2(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
         if (c.flags & UHS_CFG_REGTILE) return uhs_find_star3d_rt(pl.prog, dt, c.vec);
+    {
+        const long long nt = (long long)std::max(1, c.tile_x / std::max(1, c.vec)) * c.tile_y;
+        const long long ring = 2LL * pl.radius * (c.tile_x + c.tile_y);
         return uhs_find_star3d(pl.prog, dt, c.vec,
-                               (c.flags & UHS_CFG_PREFETCH4) ? 4 : (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
+                               (c.flags & UHS_CFG_PREFETCH4) ? 4 : (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1,
+                               (int)ceil_div(ring, nt));
+    }
     case FAM_BOX3D:
     {
         const int planes = (c.flags & UHS_CFG_PLANES2) ? 2 : 1;
@@ -220,7 +225,7 @@ static const uhs_config kBuiltinDefaults[12] = {
     {512, 1, 32, 1, UHS_CFG_NT_STORE},                       // 2d1r_box
     {1024, 1, 128, 2, 0},                                     // 2d2r_box
     {512, 1, 256, 1, UHS_CFG_NT_STORE},                      // 2d3r_box
-    {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d1r_star
+    {128, 16, 256, 8, UHS_CFG_NT_STORE},                     // 3d1r_star
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d2r_star
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d3r_star
     {64, 16, 128, 1, UHS_CFG_ROWS2 | UHS_CFG_PLANES2},       // 3d1r_box

@@ -53,12 +53,19 @@ template <typename T> __device__ __forceinline__ void store_out(T *p, T v, bool 
 // One star3d tile: the tile_x * tile_y column at (x0, y0), output planes
 // [zs, ze).  `in`/`out` point at the interior origin.  Shared by the
 // per-grid kernel and the batched work-queue kernel.
-template <typename T, class S, int VX, bool KAHAN, int PF, class CF>
+// FULL: the tile lies inside the grid (every point stored).  Then the
+// stores and the halo-ring loads are issued on every path (idle ring slots
+// load a harmless in-bounds cell), which lets the compiler count them when it
+// waits for older loads instead of waiting for all loads in flight.  NH: halo
+// ring cells staged per thread (the launch picks the smallest that covers the
+// ring).
+template <typename T, class S, int VX, bool KAHAN, int PF, bool FULL = false, int NH = kStarHaloPerThread,
+          class CF>
 __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restrict__ out,
                                             const TileGeom &p, int x0, int y0, int zs, int ze,
                                             T *lds, CF &cf)
 {
-    constexpr int R = S::R, P = S::P, NH = kStarHaloPerThread;
+    constexpr int R = S::R, P = S::P;
     const int NT = blockDim.x, tid = threadIdx.x;
     const int BXT = p.BXT, TX = p.TX, TY = p.TY;
     const int LW = TX + 2 * R, LSZ = LW * (TY + 2 * R);
@@ -119,7 +126,7 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
             rq[v][j] = in[(long long)min(zs - R + j, p.n0 + R - 1) * sz + coff[v]];
 #pragma unroll
     for (int k = 0; k < NH; k++)
-        if (hl[k] >= 0) rh[0][k] = in[(long long)zs * sz + hoff[k]];
+        if (FULL || hl[k] >= 0) rh[0][k] = in[(long long)zs * sz + hoff[k]];
 
     auto step = [&](auto phc, int z) {
         constexpr int PH = decltype(phc)::value; // (z - zs) mod U
@@ -137,7 +144,7 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
             const T *ph = in + (long long)(z + 1) * sz;
 #pragma unroll
             for (int k = 0; k < NH; k++)
-                if (hl[k] >= 0) rh[(PH + 1) & 1][k] = ph[hoff[k]];
+                if (FULL || hl[k] >= 0) rh[(PH + 1) & 1][k] = ph[hoff[k]];
             const T *pn = in + (long long)min(z + R + PF, p.n0 + R - 1) * sz;
 #pragma unroll
             for (int v = 0; v < VX; v++) rq[v][(PH + 2 * R + PF) % M] = pn[coff[v]];
@@ -161,7 +168,7 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
                 if constexpr (t == 0) a.first(cf.template get<t>(), val);
                 else a.add(cf.template get<t>(), val);
             });
-            if (act[v]) store_out(oz + coff[v], a.value(), nts);
+            if (FULL || act[v]) store_out(oz + coff[v], a.value(), nts);
         }
     };
     for (int z0 = zs; z0 < ze; z0 += U)
@@ -171,7 +178,127 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
         });
 }
 
-template <typename T, class S, int VX, bool KAHAN, int PF>
+// The same tile with shifting register queues (radius 2 and 3, where the
+// ring form's larger unrolled body measured slower).
+template <typename T, class S, int VX, bool KAHAN, int PF, class CF>
+__device__ __forceinline__ void star3d_tile_shift(const T *__restrict__ in, T *__restrict__ out,
+                                            const TileGeom &p, int x0, int y0, int zs, int ze,
+                                            T *lds, CF &cf)
+{
+    constexpr int R = S::R, P = S::P, Q = 2 * R + 1, NH = kStarHaloPerThread;
+    const int NT = blockDim.x, tid = threadIdx.x;
+    const int BXT = p.BXT, TX = p.TX, TY = p.TY;
+    const int LW = TX + 2 * R, LSZ = LW * (TY + 2 * R);
+    const int tx = tid % BXT, ty = tid / BXT;
+    const long long sz = p.sz;
+    const int sy = p.sy;
+
+    // This thread's points (x0 + tx + v*BXT, y0 + ty).
+    int coff[VX], lc[VX];
+    bool act[VX];
+    const int yg = y0 + ty, ycl = clampi(yg, -R, p.n1 + R - 1);
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int xg = x0 + tx + v * BXT;
+        act[v] = xg < p.n2 && yg < p.n1;
+        coff[v] = ycl * sy + clampi(xg, -R, p.n2 + R - 1);
+        lc[v] = (ty + R) * LW + tx + v * BXT + R;
+    }
+    // Halo ring cells this thread stages: R rows above/below the tile, R
+    // columns left/right of it (corners are not needed by a star).
+    const int H = 2 * R * (TX + TY);
+    int hoff[NH], hl[NH];
+#pragma unroll
+    for (int k = 0; k < NH; k++) {
+        const int c = tid + k * NT;
+        hl[k] = -1;
+        hoff[k] = 0;
+        if (c < H) {
+            int ly, lx;
+            if (c < 2 * R * TX) {
+                const int row = c / TX;
+                ly = row < R ? row : TY + row;
+                lx = R + c % TX;
+            } else {
+                const int cc = c - 2 * R * TX, col = cc % (2 * R);
+                ly = R + cc / (2 * R);
+                lx = col < R ? col : TX + col;
+            }
+            hl[k] = ly * LW + lx;
+            hoff[k] = clampi(y0 - R + ly, -R, p.n1 + R - 1) * sy +
+                      clampi(x0 - R + lx, -R, p.n2 + R - 1);
+        }
+    }
+
+    // Register queue: q[v][j] = plane z-R+j at this thread's point v; nq[f]
+    // holds plane z+R+f, in flight (PF planes of prefetch).
+    T q[VX][Q], nq[PF][VX], nh[NH];
+    const bool nts = p.nts;
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+#pragma unroll
+        for (int j = 0; j < 2 * R; j++) q[v][j] = in[(long long)(zs - R + j) * sz + coff[v]];
+#pragma unroll
+        for (int f = 0; f < PF; f++)
+            nq[f][v] = in[(long long)min(zs + R + f, p.n0 + R - 1) * sz + coff[v]];
+    }
+#pragma unroll
+    for (int k = 0; k < NH; k++)
+        if (hl[k] >= 0) nh[k] = in[(long long)zs * sz + hoff[k]];
+
+    for (int z = zs; z < ze; ++z) {
+        T *L = lds + (z & 1) * LSZ;
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+            q[v][2 * R] = nq[0][v];
+            L[lc[v]] = q[v][R];
+        }
+#pragma unroll
+        for (int k = 0; k < NH; k++)
+            if (hl[k] >= 0) L[hl[k]] = nh[k];
+        // Prefetch plane z+R+PF (own column) and the halo of plane z+1.
+        {
+            const T *pn = in + (long long)min(z + R + PF, p.n0 + R - 1) * sz;
+            const T *ph = in + (long long)(z + 1) * sz;
+#pragma unroll
+            for (int f = 0; f + 1 < PF; f++)
+#pragma unroll
+                for (int v = 0; v < VX; v++) nq[f][v] = nq[f + 1][v];
+#pragma unroll
+            for (int v = 0; v < VX; v++) nq[PF - 1][v] = pn[coff[v]];
+#pragma unroll
+            for (int k = 0; k < NH; k++)
+                if (hl[k] >= 0) nh[k] = ph[hoff[k]];
+        }
+        __syncthreads();
+        cf.refresh();
+        T *oz = out + (long long)z * sz;
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+            Acc<T, KAHAN> a;
+            static_for<P>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
+                T val;
+                if constexpr (dy == 0 && dx == 0) {
+                    val = q[v][R + dz];
+                } else {
+                    static_assert(dz == 0, "star3d needs axis-aligned terms");
+                    val = L[lc[v] + dy * LW + dx];
+                }
+                if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                else a.add(cf.template get<t>(), val);
+            });
+            if (act[v]) store_out(oz + coff[v], a.value(), nts);
+        }
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+#pragma unroll
+            for (int j = 0; j < 2 * R; j++) q[v][j] = q[v][j + 1];
+    }
+}
+
+template <typename T, class S, int VX, bool KAHAN, int PF, int NH>
 __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -182,8 +309,16 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
     const int zs = chunk * p.ZC;
     const TileGeom g{p.n0, p.n1, p.n2, p.sy, p.sz, p.TX, p.TY, p.BXT, (p.flags & UHS_CFG_NT_STORE) != 0};
     Coefs<T, S::P> cf(p);
-    star3d_tile<T, S, VX, KAHAN, PF>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs, min(zs + p.ZC, p.n0),
-                                     reinterpret_cast<T *>(smem_raw), cf);
+    if constexpr (S::R == 1) {
+        const int x0 = tile_x * p.TX, y0 = tile_y * p.TY, ze = min(zs + p.ZC, p.n0);
+        T *lds = reinterpret_cast<T *>(smem_raw);
+        if (x0 + p.TX <= p.n2 && y0 + p.TY <= p.n1)
+            star3d_tile<T, S, VX, KAHAN, PF, true, NH>(p.in, p.out, g, x0, y0, zs, ze, lds, cf);
+        else
+            star3d_tile<T, S, VX, KAHAN, PF, false, NH>(p.in, p.out, g, x0, y0, zs, ze, lds, cf);
+    } else
+        star3d_tile_shift<T, S, VX, KAHAN, PF>(p.in, p.out, g, tile_x * p.TX, tile_y * p.TY, zs,
+                                               min(zs + p.ZC, p.n0), reinterpret_cast<T *>(smem_raw), cf);
 }
 
 // --------------------------------------------------------------------------
@@ -918,13 +1053,13 @@ template <typename T> constexpr int dtype_of(bool kahan)
     return sizeof(T) == 8 ? DT_F64 : (kahan ? DT_F32K : DT_F32);
 }
 
-template <typename T, class S, int VX, bool KAHAN, int PF>
+template <typename T, class S, int VX, bool KAHAN, int PF, int NH = kStarHaloPerThread>
 hipError_t launch_star3d(const SweepArgs &a, hipStream_t st)
 {
     auto p = make_params<T, S::P>(a);
     const int nt = (a.tx / VX) * a.ty;
     const size_t lds = 2 * sizeof(T) * (a.tx + 2 * S::R) * (a.ty + 2 * S::R);
-    hipLaunchKernelGGL((k_star3d<T, S, VX, KAHAN, PF>), dim3(p.nblocks), dim3(nt), lds, st, p);
+    hipLaunchKernelGGL((k_star3d<T, S, VX, KAHAN, PF, NH>), dim3(p.nblocks), dim3(nt), lds, st, p);
     return hipGetLastError();
 }
 

@@ -42,7 +42,8 @@ struct SweepArgs {
 typedef hipError_t (*LaunchFn)(const SweepArgs &, hipStream_t);
 
 // Per-family kernel tables (nullptr when the combination is not compiled).
-LaunchFn uhs_find_star3d(int prog, int dtype, int vec, int prefetch);
+// ring_cells: halo-ring cells each thread stages per plane (ceil(ring / threads)).
+LaunchFn uhs_find_star3d(int prog, int dtype, int vec, int prefetch, int ring_cells);
 LaunchFn uhs_find_star3d_rt(int prog, int dtype, int rows);
 LaunchFn uhs_find_star3d_tb2(int prog, int dtype, int vec, int prefetch); // two steps per launch
 LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep, int planes);

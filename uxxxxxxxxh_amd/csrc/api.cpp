@@ -219,11 +219,11 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
 // Defaults per built-in stencil: the fastest configuration of a timed sweep
 // over the configuration space on MI355X at 512^3 / 8192^2 (FP64).
 static const uhs_config kBuiltinDefaults[12] = {
-    {512, 1, 32, 1, UHS_CFG_NT_STORE},                       // 2d1r_star
+    {1024, 1, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4}, // 2d1r_star
     {1024, 1, 128, 2, 0},                                     // 2d2r_star
-    {256, 1, 32, 1, 0},                                       // 2d3r_star
+    {512, 1, 128, 1, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4},  // 2d3r_star
     {512, 1, 32, 1, UHS_CFG_NT_STORE},                       // 2d1r_box
-    {1024, 1, 128, 2, 0},                                     // 2d2r_box
+    {1024, 1, 128, 4, UHS_CFG_NT_STORE},                     // 2d2r_box
     {512, 1, 256, 1, UHS_CFG_NT_STORE},                      // 2d3r_box
     {128, 16, 256, 8, UHS_CFG_NT_STORE},                     // 3d1r_star
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d2r_star

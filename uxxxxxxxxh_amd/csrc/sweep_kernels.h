@@ -961,7 +961,12 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
         nh[f] = hthr ? pr[hoff] : T(0);
     }
     Coefs<T, P> cf(p);
-    // w[v][a][b] = in[row - R + a][x_v - R + b] for the output row being built.
+    // Window ring: w[v][r][b] holds row y of the window around point v in
+    // slot r = (y - ys + R) mod W, column x_v - R + b.  The row loop is
+    // unrolled by hand by W (the compiler does not unroll a loop with a
+    // barrier), so slots are constants and the window never shifts through
+    // register copies (2R*W moves per row otherwise, next to (2R+1)^2 FMAs
+    // for a box).
     T w[VX][W][W];
 #pragma unroll
     for (int v = 0; v < VX; v++)
@@ -970,8 +975,9 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
 #pragma unroll
             for (int b = 0; b < W; b++) w[v][a][b] = T(0);
     const int steps = (ye - ys) + 2 * R;
-    for (int s = 0; s < steps; ++s) {
-        const int yr = ys - R + s;
+    auto step = [&](auto phc, int s) {
+        constexpr int PH = decltype(phc)::value; // s mod W
+        const int yr = ys - R + s;               // row staged at this step (slot PH)
         T *L = lds + (s & 1) * LW;
 #pragma unroll
         for (int v = 0; v < VX; v++) L[R + tid + v * BXT] = nrow[0][v];
@@ -995,24 +1001,26 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
 #pragma unroll
         for (int v = 0; v < VX; v++) {
 #pragma unroll
-            for (int a = 0; a < 2 * R; a++)
-#pragma unroll
-                for (int b = 0; b < W; b++) w[v][a][b] = w[v][a + 1][b];
-#pragma unroll
-            for (int b = 0; b < W; b++) w[v][2 * R][b] = L[tid + v * BXT + b];
+            for (int b = 0; b < W; b++) w[v][PH][b] = L[tid + v * BXT + b];
             if (s >= 2 * R) {
+                // Output row yo: its row yo + dy is in slot (PH - R + dy) mod W.
                 Acc<T, KAHAN> acc;
                 static_for<P>([&](auto tc) {
                     constexpr int t = decltype(tc)::value;
                     constexpr int dy = S::OFF[t][0], dx = S::OFF[t][1];
-                    const T val = w[v][R + dy][R + dx];
+                    const T val = w[v][(PH + W - R + dy) % W][R + dx];
                     if constexpr (t == 0) acc.first(cf.template get<t>(), val);
                     else acc.add(cf.template get<t>(), val);
                 });
                 if (act[v]) store_out(orow + xg[v], acc.value(), nts);
             }
         }
-    }
+    };
+    for (int s0 = 0; s0 < steps; s0 += W)
+        static_for<W>([&](auto phc) {
+            const int s = s0 + decltype(phc)::value;
+            if (s < steps) step(phc, s);
+        });
 }
 
 // --------------------------------------------------------------------------

@@ -164,6 +164,64 @@ def batched(u, T=50, probe_T=8):
             "tasks": info["tasks"]}
 
 
+# The two multi-kernel example programs of the reference
+# (UHStencil/examples/2d5pt_nested/stencil_2d5pt_nested.dsl and
+# 3d7pt9pt_nested/stencil_3d7pt9pt_nested.dsl), at their own grid sizes,
+# domains and iteration counts, as the multi-kernel DSL reader reads them.
+NESTED_PROGRAMS = {
+    "2d5pt_nested": dict(
+        dims=2, shape=[4098, 4098], lo=[17, 17], hi=[4081, 4081], T=20,
+        producers=[([[-1, 0], [0, -1], [0, 0], [0, 1], [1, 0]], [0.2, 0.4, -0.5, 0.4, 0.2])],
+        offsets=[[-1, -1], [-1, 1], [0, 0], [1, -1], [1, 1]], coeffs=[0.1, 0.3, -0.5, 0.3, 0.1],
+        sources=[0, 0, 0, 0, 0]),
+    "3d7pt9pt_nested": dict(
+        dims=3, shape=[258, 258, 258], lo=[5, 5, 5], hi=[253, 253, 253], T=20, producers=[],
+        offsets=[[-1, -1, -1], [-1, -1, 1], [-1, 1, -1], [-1, 1, 1], [0, 0, 0],
+                 [1, -1, -1], [1, -1, 1], [1, 1, -1], [1, 1, 1]],
+        coeffs=[0.1, 0.3, 0.7, 0.9, 0.5, 0.1, 0.3, 0.7, 0.9], sources=[-1] * 9),
+}
+
+
+def programs(u, reps=3, names=None):
+    """Fused multi-kernel programs: the tiled kernel (the library's choice)
+    and the pointwise kernel, GStencils/s over the written domain, against
+    16 algorithmic bytes per written cell and the FP64 rate of the fused
+    formula (each producer re-evaluated at every access)."""
+    out = {}
+    for name, d in NESTED_PROGRAMS.items():
+        if names and name not in names:
+            continue
+        prog = u.ProgramSpec(d["dims"], d["producers"], d["offsets"], d["coeffs"], d["sources"], d["lo"], d["hi"])
+        A = u.DeviceArray(d["shape"], np.float64)
+        B = u.DeviceArray(d["shape"], np.float64)
+        ext = [n - 2 for n in d["shape"]]
+        u.fill_uniform(A, ext, 1, 5)
+        u.fill_uniform(B, ext, 1, 6)
+        cells = float(np.prod([h - l for l, h in zip(d["lo"], d["hi"])]))
+        flops = sum(2 * len(d["producers"][k][1]) - 1 if k >= 0 else 0 for k in d["sources"]) + 2 * len(d["sources"]) - 1
+        roof = min(PEAK_HBM_GBPS / 16.0, PEAK_FP64_TFLOPS * 1e3 / flops)
+        r = {"shape": d["shape"], "domain": [d["lo"], d["hi"]], "timesteps": d["T"],
+             "flops_per_cell": flops, "roof_GStencils_s": round(roof, 1)}
+        for label, kernel in (("tiled", u.PROGRAM_AUTO), ("pointwise", u.PROGRAM_POINTWISE)):
+            u.program_sweep_f64(A, B, d["shape"], prog, 2, kernel=kernel)
+            u.synchronize()
+            tm = u.Timer()
+            tm.start()
+            for _ in range(reps):
+                u.program_sweep_f64(A, B, d["shape"], prog, d["T"], kernel=kernel)
+            tm.stop()
+            ms = tm.ms() / (reps * d["T"])
+            g = cells / (ms * 1e-3) / 1e9
+            r[label] = {"ms_per_sweep": round(ms, 4), "GStencils_s": round(g, 2),
+                        "hbm_frac_at_16B": round(16 * g / PEAK_HBM_GBPS, 4), "roof_frac": round(g / roof, 4)}
+        A.close()
+        B.close()
+        out[name] = r
+        log("program %-16s tiled %.1f GSt/s, pointwise %.1f GSt/s (roof %.0f)"
+            % (name, r["tiled"]["GStencils_s"], r["pointwise"]["GStencils_s"], roof))
+    return out
+
+
 def fp32_kahan(u):
     from uxxxxxxxxh_amd.fp32study import study
     r = study("3d2r_star", [512, 512, 512], [10, 100, 1000])
@@ -236,6 +294,9 @@ def main():
     ap.add_argument("--no-temporal", action="store_true")
     ap.add_argument("--temporal-only", action="store_true",
                     help="only the two-steps-per-launch sweep (its traffic-counter run)")
+    ap.add_argument("--no-programs", action="store_true")
+    ap.add_argument("--programs-only", nargs="*", default=None, metavar="NAME",
+                    help="only the fused multi-kernel programs (all, or those named)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the copy kernel (traffic-counter calibration: 2 GiB read + written)")
     args = ap.parse_args()
@@ -248,6 +309,9 @@ def main():
     u.lib()
     if args.copy_only:
         print(json.dumps({"copy_GBps": round(copy_bandwidth(u), 1)}))
+        return
+    if args.programs_only is not None:
+        print(json.dumps(programs(u, names=args.programs_only)))
         return
     if args.temporal_only:
         print(json.dumps(temporal2(u, args.size, args.timesteps)))
@@ -312,6 +376,8 @@ def main():
     if not args.no_temporal:
         out["temporal2"] = temporal2(u, n, T)
         log("temporal2", out["temporal2"])
+    if not args.no_programs:
+        out["programs"] = programs(u)
     if not args.no_batch:
         out["batched"] = batched(u)
         log("batched", out["batched"])

@@ -230,6 +230,54 @@ int uhs_batch_default_config(uhs_batch_config *out);
 int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps,
                        uhs_stream_t stream);
 
+/* ---- fused multi-kernel programs ---------------------------------------- */
+
+/* A stencil program of several kernels after kernel fusion (the reference's
+ * StencilKernelFusion.cpp:66-180 inlines a producer kernel into its single
+ * consumer): the output kernel's term j reads either the input grid at
+ * p + o_j or producer k evaluated at p + o_j, and every producer is a
+ * stencil over the input grid.  So
+ *     out[p] = c_0*x_0 + c_1*x_1 + ...,   x_j = in[p+o_j]  or
+ *     x_j = d_0*in[p+o_j+e_0] + d_1*in[p+o_j+e_1] + ...   (producer k)
+ * each sum left to right in term order (first term a product, later terms
+ * fused multiply-adds, as in uhs_sweep_f64).  Up to UHS_MAX_PRODUCERS
+ * producers and UHS_MAX_PROGRAM_TERMS terms per kernel. */
+#define UHS_MAX_PRODUCERS 4
+#define UHS_MAX_PROGRAM_TERMS 32
+typedef struct uhs_program {
+    int dims;                      /* 2 or 3 */
+    int nproducers;
+    const uhs_stencil *producers;  /* [nproducers], each over the input grid */
+    uhs_stencil output;            /* the output ("operation") kernel */
+    const int *sources;            /* [output.npoints]: -1 = the input grid, k = producers[k] */
+    int64_t lo[3], hi[3];          /* cells written: [lo, hi) per axis, in halo-inclusive
+                                      grid coordinates (the output kernel's domain) */
+} uhs_program;
+
+/* T sweeps of a program over halo-inclusive FP64 grids of `shape` cells per
+ * axis, ping-ponging A and B as uhs_sweep_f64 does; cells outside [lo, hi)
+ * are never written.  Every read must fall inside the grid (UHS_E_INVAL
+ * otherwise). */
+int uhs_program_sweep_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
+                          int timesteps, uhs_stream_t stream);
+
+/* The same with the kernel and its tile chosen by the caller.  Both kernels
+ * give bitwise the same result: a producer's value at a cell does not
+ * depend on which access evaluates it. */
+#define UHS_PROGRAM_AUTO 0      /* tiled when its LDS frame fits, else pointwise */
+#define UHS_PROGRAM_TILED 1     /* a block of the input staged in LDS once, each producer
+                                   evaluated once per cell of the block */
+#define UHS_PROGRAM_POINTWISE 2 /* one thread per cell, producers re-evaluated per access */
+/* tile (tiled kernel): {tile_x, tile_y, tile_z, planes_per_step}.  3D: a
+ * workgroup owns tile_x x tile_y cells of axes 2 and 1 and walks a slab of
+ * tile_z planes of axis 0; 2D: tile_x columns and a slab of tile_y rows
+ * (tile_z 1).  planes_per_step (0 = chosen) planes are staged and written
+ * between two barriers.  NULL = chosen from the program's reach.  The LDS
+ * rings (staged planes of the tile plus reach, for the input and each
+ * producer) must fit in 64 KiB (UHS_E_CONFIG otherwise). */
+int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
+                             int timesteps, int kernel, const int *tile, uhs_stream_t stream);
+
 /* ---- plumbing: device memory, streams, timing ---------------------------- */
 
 /* Streaming device-to-device copy of n doubles, one element per thread: the

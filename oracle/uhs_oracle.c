@@ -202,3 +202,97 @@ void orc_fill_rand50_f64(double *a, size_t n)
 {
     for (size_t i = 0; i < n; i++) a[i] = rand() % 50;
 }
+
+/* ---- fused two-level programs ------------------------------------------- */
+
+/* The value of producer term list [b, e) at linear cell q. */
+static double prog_producer(const double *in, long long q, int b, int e, const long long *poff,
+                            const double *pcoef, int mode)
+{
+    double s = pcoef[b] * in[q + poff[b]];
+    for (int t = b + 1; t < e; t++) {
+        const double v = in[q + poff[t]];
+        s = mode == ORC_FUSED ? fma(pcoef[t], v, s) : s + pcoef[t] * v;
+    }
+    return s;
+}
+
+int orc_program_f64(double *A, double *B, int dims, const int64_t *shape,
+                    int nprod, const int *prod_np, const int *prod_off, const double *prod_coef,
+                    int np, const int *off, const double *coef, const int *src,
+                    const int64_t *lo, const int64_t *hi, int T, int mode)
+{
+    if (!A || !B || !shape || (dims != 2 && dims != 3) || np < 1 || !off || !coef || !src ||
+        nprod < 0 || (nprod && (!prod_np || !prod_off || !prod_coef)) || !lo || !hi || T < 0 ||
+        (mode != ORC_PLAIN && mode != ORC_FUSED))
+        return -1;
+    /* 3D view: a 2D grid is one plane. */
+    long long shp[3] = {1, 1, 1}, l[3] = {0, 0, 0}, h[3] = {1, 1, 1};
+    for (int k = 0; k < dims; k++) {
+        shp[3 - dims + k] = shape[k];
+        l[3 - dims + k] = lo[k];
+        h[3 - dims + k] = hi[k];
+        if (lo[k] < 0 || hi[k] > shape[k] || lo[k] >= hi[k]) return -1;
+    }
+    const long long sy = shp[2], sz = shp[1] * shp[2];
+    int ntot = 0;
+    for (int k = 0; k < nprod; k++) {
+        if (prod_np[k] < 1) return -1;
+        ntot += prod_np[k];
+    }
+    long long *poff = (long long *)malloc(sizeof(long long) * (ntot + 1));
+    long long *ooff = (long long *)malloc(sizeof(long long) * np);
+    int *pstart = (int *)malloc(sizeof(int) * (nprod + 1));
+    if (!poff || !ooff || !pstart) {
+        free(poff); free(ooff); free(pstart);
+        return -1;
+    }
+    int rc = 0, t = 0;
+    for (int k = 0; k < nprod; k++) {
+        pstart[k] = t;
+        for (int i = 0; i < prod_np[k]; i++, t++) {
+            long long v[3] = {0, 0, 0};
+            for (int a = 0; a < dims; a++) v[3 - dims + a] = prod_off[t * dims + a];
+            poff[t] = v[0] * sz + v[1] * sy + v[2];
+        }
+    }
+    pstart[nprod] = t;
+    /* Reads must stay inside the grid for every written cell. */
+    for (int j = 0; j < np && !rc; j++) {
+        if (src[j] < -1 || src[j] >= nprod) { rc = -1; break; }
+        long long v[3] = {0, 0, 0};
+        for (int a = 0; a < dims; a++) v[3 - dims + a] = off[j * dims + a];
+        ooff[j] = v[0] * sz + v[1] * sy + v[2];
+        const int b = src[j] < 0 ? 0 : pstart[src[j]], e = src[j] < 0 ? 1 : pstart[src[j] + 1];
+        for (int i = b; i < e && !rc; i++) {
+            for (int a = 0; a < 3; a++) {
+                long long w = v[a];
+                if (src[j] >= 0) {
+                    const int ax = a - (3 - dims);
+                    w += ax >= 0 ? prod_off[i * dims + ax] : 0;
+                }
+                if (l[a] + w < 0 || h[a] - 1 + w >= shp[a]) rc = -1;
+            }
+        }
+    }
+    for (int step = 1; step <= T && !rc; step++) {
+        const double *in = (step & 1) ? A : B;
+        double *out = (step & 1) ? B : A;
+        for (long long z = l[0]; z < h[0]; z++)
+            for (long long y = l[1]; y < h[1]; y++)
+                for (long long x = l[2]; x < h[2]; x++) {
+                    const long long c = z * sz + y * sy + x;
+                    double acc = 0.0;
+                    for (int j = 0; j < np; j++) {
+                        const double xv = src[j] < 0 ? in[c + ooff[j]]
+                                                     : prog_producer(in, c + ooff[j], pstart[src[j]],
+                                                                     pstart[src[j] + 1], poff, prod_coef, mode);
+                        if (j == 0) acc = coef[0] * xv;
+                        else acc = mode == ORC_FUSED ? fma(coef[j], xv, acc) : acc + coef[j] * xv;
+                    }
+                    out[c] = acc;
+                }
+    }
+    free(poff); free(ooff); free(pstart);
+    return rc;
+}

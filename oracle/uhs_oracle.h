@@ -61,6 +61,23 @@ int orc_iterate_f32(float *A, float *B, int dims, const int64_t *ext,
                     int halo, int P, const int *off, const float *coef,
                     int T, int mode, int nthreads);
 
+/* Fused two-level programs (the reference's multi-kernel stencils after
+ * kernel fusion, UHStencil/lib/Dialect/Stencil/StencilKernelFusion.cpp:66-180:
+ * each producer is cloned into its consumer at every access offset):
+ *   out[p] = sum_j c_j * x_j,  x_j = in[p+o_j]            (src[j] == -1)
+ *                              x_j = sum_i d_ki * in[p+o_j+e_ki]  (src[j] == k)
+ * each sum left to right in term order, in `mode` (ORC_PLAIN or ORC_FUSED).
+ * Grids are `shape` cells per axis, halo included; cells [lo, hi) are
+ * written, the rest never.  Producers: nprod of them, producer k has
+ * prod_np[k] terms; their offsets and coefficients are concatenated in
+ * producer order (prod_off[(sum of earlier np + i)*dims + a]).  T sweeps
+ * ping-pong A and B as orc_iterate does.  Returns 0, or -1 on bad
+ * arguments (a read outside the grid included). */
+int orc_program_f64(double *A, double *B, int dims, const int64_t *shape,
+                    int nprod, const int *prod_np, const int *prod_off, const double *prod_coef,
+                    int np, const int *off, const double *coef, const int *src,
+                    const int64_t *lo, const int64_t *hi, int T, int mode);
+
 /* Seeded synthetic inputs.  Interior cell with halo-inclusive linear index L
  * gets u01(mix64(seed + (L+1)*0x9E3779B97F4A7C15)), halo cells get 0. */
 void orc_fill_uniform_f64(double *a, int dims, const int64_t *ext, int halo,

@@ -1,0 +1,307 @@
+"""Fused two-level programs on the CPU: the oracle's nested evaluation
+(orc_program_f64) against an independent numpy two-pass evaluation, and the
+multi-kernel DSL reader (uhs_parse_dsl_program).
+
+The two-pass evaluation is the reference's own definition of a multi-kernel
+program: the producer is written to a temporary grid, then the consumer reads
+it (UHStencil/examples/2d5pt_nested/stencil_2d5pt_nested_driver.serial.c:40-60,
+the driver's check loop; stencil_2d5pt_nested_kernel.serial_master.c:18-32,
+the generated code's tmp buffer).  Plain mode evaluates each kernel as a
+left-associated sum of separate products, like that C code, so it must match
+the two-pass evaluation bitwise; fused mode (the GPU kernel's fma chain) is
+held to 1e-12 of the sum of the terms' magnitudes."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.oracle_py import FUSED, PLAIN
+
+REF = "/root/reference/UHStencil/examples"
+
+
+def slab(lo, hi, off):
+    return tuple(slice(l + o, h + o) for l, h, o in zip(lo, hi, off))
+
+
+def two_pass(inp, producers, offsets, coeffs, sources, lo, hi, absolute=False):
+    """One sweep over [lo, hi): every term first materialised as a whole
+    shifted block (a producer as a temporary grid over that block)."""
+    out = None
+    for j, (v, c) in enumerate(zip(offsets, coeffs)):
+        if sources[j] < 0:
+            x = inp[slab(lo, hi, v)]
+        else:
+            po, pc = producers[sources[j]]
+            x = None
+            for w, d in zip(po, pc):
+                t = (abs(d) * np.abs(inp[slab(lo, hi, np.add(v, w))]) if absolute
+                     else d * inp[slab(lo, hi, np.add(v, w))])
+                x = t if x is None else x + t
+        t = abs(c) * np.abs(x) if absolute else c * x
+        out = t if out is None else out + t
+    return out
+
+
+def random_program(rng, dims):
+    nprod = int(rng.integers(0, 3))
+    prods = []
+    for _ in range(nprod):
+        n = int(rng.integers(1, 7))
+        prods.append((rng.integers(-2, 3, size=(n, dims)).tolist(), rng.uniform(-1, 1, n).tolist()))
+    n = int(rng.integers(1, 8))
+    offs = rng.integers(-2, 3, size=(n, dims)).tolist()
+    coefs = rng.uniform(-1, 1, n).tolist()
+    srcs = rng.integers(-1, nprod, size=n).tolist() if nprod else [-1] * n
+    shape = [int(rng.integers(12, 20)) for _ in range(dims)] if dims == 3 else \
+        [int(rng.integers(20, 40)), int(rng.integers(20, 40))]
+    # Reach of every term, for a domain whose reads stay inside the grid.
+    rlo, rhi = [0] * dims, [0] * dims
+    for j in range(n):
+        pts = [offs[j]] if srcs[j] < 0 else [np.add(offs[j], w).tolist() for w in prods[srcs[j]][0]]
+        for p in pts:
+            for a in range(dims):
+                rlo[a], rhi[a] = min(rlo[a], p[a]), max(rhi[a], p[a])
+    lo = [-rlo[a] + int(rng.integers(0, 3)) for a in range(dims)]
+    hi = [shape[a] - rhi[a] - int(rng.integers(0, 3)) for a in range(dims)]
+    return shape, prods, offs, coefs, srcs, lo, hi
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_oracle_plain_is_two_pass(orc, seed):
+    rng = np.random.default_rng(seed)
+    dims = 2 + seed % 2
+    shape, prods, offs, coefs, srcs, lo, hi = random_program(rng, dims)
+    A0, B0 = rng.random(shape), rng.random(shape)
+    for T in (1, 2, 3):
+        A, B = A0.copy(), B0.copy()
+        assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, T, PLAIN) == 0
+        eA, eB = A0.copy(), B0.copy()
+        for s in range(1, T + 1):
+            src, dst = (eA, eB) if s & 1 else (eB, eA)
+            dst[slab(lo, hi, [0] * dims)] = two_pass(src, prods, offs, coefs, srcs, lo, hi)
+        assert np.array_equal(A, eA) and np.array_equal(B, eB), (seed, T)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_fused_within_bound(orc, seed):
+    rng = np.random.default_rng(100 + seed)
+    dims = 2 + seed % 2
+    shape, prods, offs, coefs, srcs, lo, hi = random_program(rng, dims)
+    A0, B0 = rng.random(shape), rng.random(shape)
+    A, B = A0.copy(), B0.copy()
+    assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, 1, FUSED) == 0
+    want = two_pass(A0, prods, offs, coefs, srcs, lo, hi)
+    mag = two_pass(A0, prods, offs, coefs, srcs, lo, hi, absolute=True)
+    got = B[slab(lo, hi, [0] * dims)]
+    assert np.all(np.abs(got - want) <= 1e-12 * np.maximum(mag, 1e-300))
+    # Outside the domain nothing is written.
+    mask = np.ones(shape, bool)
+    mask[slab(lo, hi, [0] * dims)] = False
+    assert np.array_equal(B[mask], B0[mask]) and np.array_equal(A, A0)
+
+
+def test_oracle_detects_planted_index_error(orc):
+    """A consumer that reads its producer at the wrong offset (one cell off)
+    must not match: the two-pass check sees the offsets, not just the sums."""
+    rng = np.random.default_rng(5)
+    shape, lo, hi = [30, 30], [4, 4], [26, 26]
+    prods = [([[-1, 0], [0, 1], [0, 0]], [0.2, 0.4, -0.5])]
+    offs, coefs, srcs = [[-1, -1], [1, 1], [0, 0]], [0.1, 0.3, -0.5], [0, 0, 0]
+    A0, B0 = rng.random(shape), rng.random(shape)
+    want = two_pass(A0, prods, offs, coefs, srcs, lo, hi)
+    A, B = A0.copy(), B0.copy()
+    assert orc.program(A, B, shape, prods, [[-1, -1], [1, 0], [0, 0]], coefs, srcs, lo, hi, 1, PLAIN) == 0
+    assert not np.array_equal(B[4:26, 4:26], want)
+
+
+def test_oracle_rejects_reads_outside_grid(orc):
+    A, B = np.zeros((10, 10)), np.zeros((10, 10))
+    prods = [([[0, -2]], [1.0])]
+    # producer reach -2 plus term offset -1 at lo = 2 reads column -1
+    assert orc.program(A, B, [10, 10], prods, [[0, -1]], [1.0], [0], [2, 2], [8, 8], 1, PLAIN) == -1
+    assert orc.program(A, B, [10, 10], prods, [[0, -1]], [1.0], [0], [2, 3], [8, 8], 1, PLAIN) == 0
+    assert orc.program(A, B, [10, 10], prods, [[0, 0]], [1.0], [1], [2, 3], [8, 8], 1, PLAIN) == -1
+
+
+NESTED_2D = """
+stencil nested_small(double input[98][98]) {
+    iteration(4)
+    operation (outer)
+    kernel inner {
+        domain([1, 97][1, 97])
+        expr {
+            0.2*input[y-1][x]
+            + 0.4*input[y][x-1] - 0.5*input[y][x] + 0.4*input[y][x+1]
+            + 0.2*input[y+1][x]
+        }
+    }
+    kernel outer {
+        domain([17, 81][17, 81])
+        expr {
+            0.1*inner[y-1][x-1] + 0.3*inner[y-1][x+1]
+            - 0.5*inner[y][x]
+            + 0.3*inner[y+1][x-1] + 0.1*inner[y+1][x+1]
+        }
+    }
+}
+"""
+
+
+def in_place_check_loop(inp, T, h2):
+    """The reference driver's own check loop, restated in numpy: each
+    iteration writes tmp over the whole interior, then the consumer back into
+    the input over the inner domain (stencil_2d5pt_nested_driver.serial.c:44-60)."""
+    a = inp.copy()
+    tmp = np.zeros_like(a)
+    M, N = a.shape
+    for _ in range(T):
+        tmp[1:M - 1, 1:N - 1] = (0.2 * a[0:M - 2, 1:N - 1] + 0.4 * a[1:M - 1, 0:N - 2]
+                                 - 0.5 * a[1:M - 1, 1:N - 1] + 0.4 * a[1:M - 1, 2:N]
+                                 + 0.2 * a[2:M, 1:N - 1])
+        lo, hi = 1 + h2, M - 1 - h2
+        a[lo:hi, lo:hi] = (0.1 * tmp[lo - 1:hi - 1, lo - 1:hi - 1] + 0.3 * tmp[lo - 1:hi - 1, lo + 1:hi + 1]
+                           - 0.5 * tmp[lo:hi, lo:hi]
+                           + 0.3 * tmp[lo + 1:hi + 1, lo - 1:hi - 1] + 0.1 * tmp[lo + 1:hi + 1, lo + 1:hi + 1])
+    return a
+
+
+def run_parsed(orc, prog, A, B, T, mode):
+    prods = [(p.offsets.tolist(), p.coeffs.tolist()) for p in prog.producers]
+    return orc.program(A, B, [int(v) for v in A.shape], prods, prog.output.offsets.tolist(),
+                       prog.output.coeffs.tolist(), prog.sources.tolist(), prog.lo, prog.hi, T, mode)
+
+
+def test_parse_nested_program_and_reference_check_loop(uhs, orc):
+    prog, info = uhs.parse_dsl_program(NESTED_2D)
+    assert info.name.decode() == "nested_small" and info.operation.decode() == "outer"
+    assert info.nproducers == 1 and info.producer_names[0].value.decode() == "inner"
+    assert prog.producers[0].offsets.tolist() == [[-1, 0], [0, -1], [0, 0], [0, 1], [1, 0]]
+    assert prog.producers[0].coeffs.tolist() == [0.2, 0.4, -0.5, 0.4, 0.2]
+    assert prog.output.offsets.tolist() == [[-1, -1], [-1, 1], [0, 0], [1, -1], [1, 1]]
+    assert prog.output.coeffs.tolist() == [0.1, 0.3, -0.5, 0.3, 0.1]
+    assert prog.sources.tolist() == [0] * 5 and prog.lo == [17, 17] and prog.hi == [81, 81]
+    assert info.iteration == 4 and list(info.grid[:2]) == [98, 98]
+    # Ping-pong with B a copy of A equals the driver's in-place loop: cells
+    # outside the domain are never written, so both grids keep them.
+    rng = np.random.default_rng(3)
+    A = rng.random((98, 98))
+    B = A.copy()
+    want = in_place_check_loop(A, 4, 16)
+    assert run_parsed(orc, prog, A, B, 4, PLAIN) == 0
+    assert np.array_equal(A, want)
+
+
+def test_default_operation_and_input_terms(uhs):
+    text = """stencil s(double a[12][14]) { iteration(2)
+      kernel k1 { domain([1,11][1,13]) expr { 0.2*a[y-1][x] + 0.4*a[y][x-1] - 0.5*a[y][x] } }
+      kernel k2 { domain([3,9][3,11]) expr { 0.1*k1[y-1][x-1] + 0.3*a[y][x+1] - 0.5*k1[y][x] } } }"""
+    prog, info = uhs.parse_dsl_program(text)
+    assert info.operation.decode() == "k2" and info.nproducers == 1
+    assert prog.sources.tolist() == [0, -1, 0]
+    assert prog.output.offsets.tolist() == [[-1, -1], [0, 1], [0, 0]]
+    assert prog.lo == [3, 3] and prog.hi == [9, 11]
+
+
+def test_single_kernel_program_matches_parse_dsl(uhs):
+    text = """stencil one(double f[10][12][14]) { iteration(4)
+      kernel k { expr { 0.5*f[z-1][y][x] + 0.25*f[z][y][x+1] - f[z+1][y-1][x] } } }"""
+    prog, info = uhs.parse_dsl_program(text)
+    st, sinfo = uhs.parse_dsl(text)
+    assert info.nproducers == 0 and prog.sources.tolist() == [-1, -1, -1]
+    assert prog.output.offsets.tolist() == st.offsets.tolist()
+    assert prog.output.coeffs.tolist() == st.coeffs.tolist()
+    # Default domain: the interior at the program's reach.
+    assert prog.lo == [1, 1, 1] and prog.hi == [9, 11, 13]
+
+
+def test_program_reader_errors(uhs):
+    chain = """stencil c(double a[20][20]) { operation(k3)
+      kernel k1 { expr { a[y][x+1] } }
+      kernel k2 { expr { 0.5*k1[y][x] } }
+      kernel k3 { expr { 0.5*k2[y][x-1] } } }"""
+    with pytest.raises(uhs.UhsError, match="producer"):
+        uhs.parse_dsl_program(chain)
+    unknown = """stencil u(double a[20][20]) { kernel k { expr { 0.5*b[y][x] } } }"""
+    with pytest.raises(uhs.UhsError):
+        uhs.parse_dsl_program(unknown)
+    missing_op = """stencil m(double a[20][20]) { operation(nope) kernel k { expr { a[y][x] } } }"""
+    with pytest.raises(uhs.UhsError):
+        uhs.parse_dsl_program(missing_op)
+
+
+class HostBuf:
+    """Host memory standing in for a device grid: every call below fails its
+    argument checks before any launch, so nothing reads it."""
+
+    def __init__(self, shape):
+        self.a = np.zeros(shape)
+        self.ptr = self.a.ctypes.data
+
+
+def test_program_sweep_rejects_bad_programs(uhs):
+    A, B = HostBuf((16, 16)), HostBuf((16, 16))
+
+    def sweep(prods, offs, coefs, srcs, lo, hi, a=A, b=B, T=1):
+        uhs.program_sweep_f64(a, b, [16, 16], uhs.ProgramSpec(2, prods, offs, coefs, srcs, lo, hi), T)
+
+    with pytest.raises(uhs.UhsError, match="outside the grid"):
+        sweep([([[0, -3]], [1.0])], [[0, -1]], [1.0], [0], [2, 3], [14, 14])
+    with pytest.raises(uhs.UhsError, match="outside the grid"):
+        sweep([], [[2, 0]], [1.0], [-1], [2, 2], [15, 14])
+    with pytest.raises(uhs.UhsError, match="source index"):
+        sweep([([[0, 1]], [1.0])], [[0, 0]], [1.0], [1], [2, 2], [14, 14])
+    with pytest.raises(uhs.UhsError, match="domain"):
+        sweep([], [[0, 0]], [1.0], [-1], [2, 2], [2, 14])
+    with pytest.raises(uhs.UhsError, match="distinct"):
+        sweep([], [[0, 0]], [1.0], [-1], [2, 2], [14, 14], b=A)
+    with pytest.raises(uhs.UhsError, match="producers"):
+        sweep([([[0, 0]], [1.0])] * 5, [[0, 0]], [1.0], [0], [2, 2], [14, 14])
+    with pytest.raises(uhs.UhsError, match="negative"):
+        sweep([], [[0, 0]], [1.0], [-1], [2, 2], [14, 14], T=-1)
+    with pytest.raises(uhs.UhsError, match="1 to 32"):
+        sweep([], [[0, 0]] * 33, [1.0] * 33, [-1] * 33, [2, 2], [14, 14])
+
+
+def test_program_kernel_choice_errors(uhs):
+    """Tile and kernel checks happen before any launch."""
+    A, B = HostBuf((90, 20, 70)), HostBuf((90, 20, 70))
+    # 81 planes x 17 rows x 64 columns even for one-cell blocks: beyond 64 KiB.
+    far = uhs.ProgramSpec(3, [], [[-40, 0, 0], [40, 0, 0], [0, -8, 0], [0, 8, 0]], [0.5, 0.5, 0.1, 0.1],
+                          [-1, -1, -1, -1], [41, 9, 1], [49, 11, 69])
+    near = uhs.ProgramSpec(3, [], [[-1, 0, 0], [1, 0, 0]], [0.5, 0.5], [-1, -1], [1, 1, 1], [89, 19, 69])
+    shape = [90, 20, 70]
+    with pytest.raises(uhs.UhsError, match="LDS") as e:
+        uhs.program_sweep_f64(A, B, shape, far, 1, kernel=uhs.PROGRAM_TILED)
+    assert e.value.code == -3
+    with pytest.raises(uhs.UhsError, match="LDS"):
+        uhs.program_sweep_f64(A, B, shape, near, 1, tile=[4096, 64, 4])
+    with pytest.raises(uhs.UhsError, match="tile extents"):
+        uhs.program_sweep_f64(A, B, shape, near, 1, tile=[0, 8, 4])
+    with pytest.raises(uhs.UhsError, match="unknown program kernel"):
+        uhs.program_sweep_f64(A, B, shape, near, 1, kernel=3)
+    two_d = uhs.ProgramSpec(2, [], [[0, 1]], [1.0], [-1], [1, 1], [19, 69])
+    with pytest.raises(uhs.UhsError, match="tile_z 1"):
+        uhs.program_sweep_f64(A, B, [20, 70], two_d, 1, tile=[60, 8, 2])
+    # Nothing launched for T = 0, whatever the choice.
+    uhs.program_sweep_f64(A, B, shape, near, 0, kernel=uhs.PROGRAM_TILED, tile=[60, 8, 4])
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference example programs not present")
+def test_reference_nested_examples(uhs, orc):
+    prog, info = uhs.parse_dsl_program(path=os.path.join(REF, "2d5pt_nested", "stencil_2d5pt_nested.dsl"))
+    assert info.name.decode() == "stencil_2d5pt_nested" and info.operation.decode() == "stencil_2d9pt_x"
+    assert info.nproducers == 1 and prog.sources.tolist() == [0] * 5
+    assert prog.lo == [17, 17] and prog.hi == [4081, 4081] and info.iteration == 20
+    # Two iterations of the driver's own check loop at the full 4098^2 grid.
+    rng = np.random.default_rng(11)
+    A = rng.random((4098, 4098))
+    B = A.copy()
+    want = in_place_check_loop(A, 2, 16)
+    assert run_parsed(orc, prog, A, B, 2, PLAIN) == 0
+    assert np.array_equal(A, want)
+    del A, B, want
+
+    prog, info = uhs.parse_dsl_program(path=os.path.join(REF, "3d7pt9pt_nested", "stencil_3d7pt9pt_nested.dsl"))
+    assert info.operation.decode() == "stencil_3d9pt_x" and info.nproducers == 0
+    assert prog.output.npoints == 9 and prog.lo == [5, 5, 5] and prog.hi == [253, 253, 253]

@@ -33,6 +33,27 @@ class Stencil(ctypes.Structure):
                 ("coeffs", ctypes.POINTER(ctypes.c_double))]
 
 
+MAX_PRODUCERS, MAX_PROGRAM_TERMS = 4, 32
+
+
+class Program(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int), ("nproducers", ctypes.c_int), ("producers", ctypes.POINTER(Stencil)),
+                ("output", Stencil), ("sources", ctypes.POINTER(ctypes.c_int)),
+                ("lo", ctypes.c_int64 * 3), ("hi", ctypes.c_int64 * 3)]
+
+
+class DslProgram(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("operation", ctypes.c_char * 64), ("dims", ctypes.c_int),
+                ("is_float", ctypes.c_int), ("iteration", ctypes.c_int), ("grid", ctypes.c_int64 * 3),
+                ("domain_lo", ctypes.c_int64 * 3), ("domain_hi", ctypes.c_int64 * 3), ("nproducers", ctypes.c_int),
+                ("producer_names", (ctypes.c_char * 64) * MAX_PRODUCERS),
+                ("producer_npoints", ctypes.c_int * MAX_PRODUCERS),
+                ("producer_offsets", (ctypes.c_int * (MAX_PROGRAM_TERMS * 3)) * MAX_PRODUCERS),
+                ("producer_coeffs", (ctypes.c_double * MAX_PROGRAM_TERMS) * MAX_PRODUCERS),
+                ("npoints", ctypes.c_int), ("offsets", ctypes.c_int * (MAX_PROGRAM_TERMS * 3)),
+                ("coeffs", ctypes.c_double * MAX_PROGRAM_TERMS), ("sources", ctypes.c_int * MAX_PROGRAM_TERMS)]
+
+
 class Config(ctypes.Structure):
     _fields_ = [("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int),
                 ("tile_z", ctypes.c_int), ("vec", ctypes.c_int), ("flags", ctypes.c_int)]
@@ -92,6 +113,10 @@ _SIGS = {
     "uhs_error_norms_f32_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_error_norms_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_parse_dsl": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
+    "uhs_parse_dsl_program": (_I, [ctypes.c_char_p, ctypes.POINTER(DslProgram)]),
+    "uhs_parse_dsl_program_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslProgram)]),
+    "uhs_program_sweep_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _P]),
+    "uhs_program_sweep_ex_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _I, ctypes.POINTER(ctypes.c_int), _P]),
     "uhs_parse_dsl_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
     "uhs_batch_create": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(BatchConfig), ctypes.POINTER(_P)]),
     "uhs_batch_run": (_I, [_P, _P]),
@@ -225,6 +250,55 @@ def builtin(case, coeffs=None):
     s = StencilSpec(dims.value, offs, c, name.value.decode(), rad.value)
     s.symbol = sym.value.decode()
     return s
+
+
+class ProgramSpec:
+    """A fused two-level program (uhs_program) kept alive for ctypes: producers
+    are (offsets, coeffs) over the input grid; the output kernel's term j reads
+    the input (sources[j] == -1) or producer sources[j]; cells [lo, hi) are
+    written (halo-inclusive coordinates, axis 0 first)."""
+
+    def __init__(self, dims, producers, offsets, coeffs, sources, lo, hi):
+        self.dims = int(dims)
+        self.producers = [StencilSpec(dims, o, c) for o, c in producers]
+        self.output = StencilSpec(dims, offsets, coeffs)
+        self.sources = np.ascontiguousarray(np.asarray(sources, np.int32))
+        self.lo, self.hi = [int(v) for v in lo], [int(v) for v in hi]
+        self._parr = (Stencil * max(1, len(self.producers)))(*[p.c for p in self.producers])
+        self.c = Program(self.dims, len(self.producers), self._parr, self.output.c,
+                         self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                         (ctypes.c_int64 * 3)(*(self.lo + [0] * (3 - dims))),
+                         (ctypes.c_int64 * 3)(*(self.hi + [0] * (3 - dims))))
+
+
+PROGRAM_AUTO, PROGRAM_TILED, PROGRAM_POINTWISE = 0, 1, 2
+
+
+def program_sweep_f64(A, B, shape, prog, timesteps, stream=None, kernel=PROGRAM_AUTO, tile=None):
+    if kernel == PROGRAM_AUTO and tile is None:
+        check(lib().uhs_program_sweep_f64(A.ptr, B.ptr, ext_array(shape), ctypes.byref(prog.c), timesteps, stream))
+        return
+    t = (ctypes.c_int * 4)(*(list(tile) + [0] * (4 - len(tile)))) if tile is not None else None
+    check(lib().uhs_program_sweep_ex_f64(A.ptr, B.ptr, ext_array(shape), ctypes.byref(prog.c), timesteps,
+                                         kernel, t, stream))
+
+
+def parse_dsl_program(text=None, path=None):
+    """Read a multi-kernel DSL program into (ProgramSpec, DslProgram)."""
+    info = DslProgram()
+    if path is not None:
+        check(lib().uhs_parse_dsl_program_file(path.encode(), ctypes.byref(info)))
+    else:
+        check(lib().uhs_parse_dsl_program(text.encode(), ctypes.byref(info)))
+    d = info.dims
+    prods = []
+    for k in range(info.nproducers):
+        n = info.producer_npoints[k]
+        prods.append((np.array(info.producer_offsets[k][:n * d], np.int32), np.array(info.producer_coeffs[k][:n])))
+    n = info.npoints
+    prog = ProgramSpec(d, prods, np.array(info.offsets[:n * d], np.int32), np.array(info.coeffs[:n]),
+                       np.array(info.sources[:n], np.int32), list(info.domain_lo[:d]), list(info.domain_hi[:d]))
+    return prog, info
 
 
 def parse_dsl(text=None, path=None):

@@ -118,9 +118,11 @@ struct Parser {
         if (!expect("(")) return false;
         for (;;) {
             long long v;
-            if (is("[")) { // mpiHalo([a,b]...) style
-                adv();
-                if (!integer(&v) || !expect(",") || !integer(&v) || !expect("]")) return false;
+            if (is("[")) { // mpiHalo([a,b][c,d]...) style: adjacent bracket pairs
+                while (is("[")) {
+                    adv();
+                    if (!integer(&v) || !expect(",") || !integer(&v) || !expect("]")) return false;
+                }
             } else if (!integer(&v)) {
                 return false;
             }
@@ -290,7 +292,7 @@ int uhs_parse_dsl(const char *text, uhs_dsl_info *out)
         } else if (ps.isid("kernel")) {
             if (kernels++ > 0)
                 return uhs_set_error(UHS_E_UNSUPPORTED,
-                                     "more than one kernel (fused stencil programs) is not read");
+                                     "more than one kernel: read fused programs with uhs_parse_dsl_program");
             if (!parse_kernel(ps, arr, dims, out, &have_domain)) return bail();
         } else {
             ps.fail("iteration, operation, mpiTile, mpiHalo or kernel");
@@ -335,4 +337,293 @@ int uhs_parse_dsl_file(const char *path, uhs_dsl_info *out)
     while ((n = fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
     fclose(f);
     return uhs_parse_dsl(text.c_str(), out);
+}
+
+// --------------------------------------------------------------------------
+// Multi-kernel programs (uhs_parse_dsl_program): every kernel is read as a
+// sum of coefficient * access terms whose accesses name the array or an
+// earlier kernel (MLIRGen.cpp:356-432 resolves kernel results by name); the
+// operation kernel (MLIRGen.cpp:187-188,259-264: its result is copied to the
+// output) becomes the program's output, the kernels it reads become its
+// producers (inlined, as StencilKernelFusion.cpp:66-180 does), and kernels
+// it does not read are dropped, as they are dead in the reference too.
+// --------------------------------------------------------------------------
+namespace {
+
+struct KTerm {
+    std::string src;
+    int off[3];
+    double c;
+};
+struct KDef {
+    std::string name;
+    bool have_domain = false;
+    long long lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    std::vector<KTerm> terms;
+};
+
+bool prog_access(Parser &ps, int dims, KTerm &t)
+{
+    if (ps.cur.kind != Tok::ID) return ps.fail("an array or kernel name");
+    t.src = ps.cur.text;
+    ps.adv();
+    int k = 0;
+    while (ps.is("[")) {
+        ps.adv();
+        if (ps.cur.kind != Tok::ID) return ps.fail("an index name");
+        ps.adv();
+        int o = 0;
+        if (ps.is("+") || ps.is("-")) {
+            const int sign = ps.is("-") ? -1 : 1;
+            ps.adv();
+            long long v;
+            if (!ps.integer(&v)) return false;
+            o = sign * (int)v;
+        }
+        if (!ps.expect("]")) return false;
+        if (k >= dims) return ps.fail("no more indices than the array has dimensions");
+        t.off[k++] = o;
+    }
+    if (k != dims) return ps.fail("one index per dimension");
+    return true;
+}
+
+bool prog_kernel(Parser &ps, int dims, KDef &kd)
+{
+    ps.adv(); // 'kernel'
+    if (ps.cur.kind != Tok::ID) return ps.fail("a kernel name");
+    kd.name = ps.cur.text;
+    ps.adv();
+    if (!ps.expect("{")) return false;
+    bool have_expr = false;
+    while (!ps.is("}")) {
+        if (ps.isid("tile") || ps.isid("swCacheAt")) {
+            ps.adv();
+            if (!ps.int_list()) return false;
+        } else if (ps.isid("domain")) {
+            ps.adv();
+            if (!ps.expect("(")) return false;
+            int k = 0;
+            while (ps.is("[")) {
+                ps.adv();
+                long long lo, hi;
+                if (!ps.integer(&lo) || !ps.expect(",") || !ps.integer(&hi) || !ps.expect("]")) return false;
+                if (k >= dims) return ps.fail("one domain range per dimension");
+                kd.lo[k] = lo;
+                kd.hi[k] = hi;
+                k++;
+            }
+            if (k != dims) return ps.fail("one domain range per dimension");
+            if (!ps.expect(")")) return false;
+            kd.have_domain = true;
+        } else if (ps.isid("expr")) {
+            ps.adv();
+            if (!ps.expect("{")) return false;
+            double sign = 1.0;
+            if (ps.is("-")) {
+                sign = -1.0;
+                ps.adv();
+            }
+            for (;;) {
+                KTerm t{};
+                double c = 1.0;
+                if (ps.cur.kind == Tok::NUM) {
+                    c = ps.cur.num;
+                    ps.adv();
+                    if (!ps.expect("*") || !prog_access(ps, dims, t)) return false;
+                } else {
+                    if (!prog_access(ps, dims, t)) return false;
+                    if (ps.is("*")) {
+                        ps.adv();
+                        if (ps.cur.kind != Tok::NUM) return ps.fail("a coefficient");
+                        c = ps.cur.num;
+                        ps.adv();
+                    }
+                }
+                t.c = sign * c;
+                kd.terms.push_back(t);
+                if (ps.is("+") || ps.is("-")) {
+                    sign = ps.is("-") ? -1.0 : 1.0;
+                    ps.adv();
+                    continue;
+                }
+                if (ps.is("}")) break;
+                return ps.fail("'+', '-' or the end of expr (only sums of coefficient*access terms are read)");
+            }
+            ps.adv();
+            have_expr = true;
+        } else {
+            return ps.fail("tile, swCacheAt, domain or expr");
+        }
+    }
+    ps.adv();
+    if (!have_expr) return ps.fail("an expr block");
+    return true;
+}
+
+} // namespace
+
+int uhs_parse_dsl_program(const char *text, uhs_dsl_program *out)
+{
+    if (!text || !out) return uhs_set_error(UHS_E_INVAL, "null argument");
+    memset(out, 0, sizeof *out);
+    out->iteration = 1;
+    Parser ps(text);
+    auto bail = [&]() { return uhs_set_error(UHS_E_PARSE, ps.err.c_str()); };
+    if (!ps.isid("stencil")) return uhs_set_error(UHS_E_PARSE, "line 1: expected 'stencil'");
+    ps.adv();
+    if (ps.cur.kind != Tok::ID) { ps.fail("a stencil name"); return bail(); }
+    snprintf(out->name, sizeof out->name, "%s", ps.cur.text.c_str());
+    ps.adv();
+    if (!ps.expect("(")) return bail();
+    if (ps.isid("double")) out->is_float = 0;
+    else if (ps.isid("float")) out->is_float = 1;
+    else { ps.fail("double or float"); return bail(); }
+    ps.adv();
+    if (ps.cur.kind != Tok::ID) { ps.fail("an array name"); return bail(); }
+    const std::string arr = ps.cur.text;
+    ps.adv();
+    int dims = 0;
+    while (ps.is("[")) {
+        ps.adv();
+        long long n;
+        if (!ps.integer(&n) || !ps.expect("]")) return bail();
+        if (dims >= 3) { ps.fail("at most 3 dimensions"); return bail(); }
+        out->grid[dims++] = n;
+    }
+    if (dims < 2) { ps.fail("a 2D or 3D array"); return bail(); }
+    if (!ps.expect(")") || !ps.expect("{")) return bail();
+    out->dims = dims;
+    std::vector<KDef> kernels;
+    std::string operation;
+    while (!ps.is("}")) {
+        if (ps.cur.kind == Tok::END) { ps.fail("'}'"); return bail(); }
+        if (ps.isid("iteration")) {
+            ps.adv();
+            long long t;
+            if (!ps.expect("(") || !ps.integer(&t) || !ps.expect(")")) return bail();
+            out->iteration = (int)t;
+        } else if (ps.isid("operation")) {
+            ps.adv();
+            if (!ps.expect("(")) return bail();
+            if (ps.cur.kind != Tok::ID) { ps.fail("a kernel name"); return bail(); }
+            operation = ps.cur.text;
+            ps.adv();
+            if (!ps.expect(")")) return bail();
+        } else if (ps.isid("mpiTile") || ps.isid("mpiHalo")) {
+            ps.adv();
+            if (!ps.int_list()) return bail();
+        } else if (ps.isid("kernel")) {
+            KDef kd;
+            if (!prog_kernel(ps, dims, kd)) return bail();
+            for (const KDef &o : kernels)
+                if (o.name == kd.name) return uhs_set_error(UHS_E_PARSE, ("kernel '" + kd.name + "' defined twice").c_str());
+            kernels.push_back(std::move(kd));
+        } else {
+            ps.fail("iteration, operation, mpiTile, mpiHalo or kernel");
+            return bail();
+        }
+    }
+    if (kernels.empty()) { ps.fail("a kernel"); return bail(); }
+    if (out->iteration != 1 && out->iteration % 2)
+        return uhs_set_error(UHS_E_PARSE, "iteration must be 1 or a multiple of 2");
+    if (operation.empty()) operation = kernels.back().name;
+    int op = -1;
+    for (int k = 0; k < (int)kernels.size(); k++)
+        if (kernels[k].name == operation) op = k;
+    if (op < 0) return uhs_set_error(UHS_E_PARSE, ("operation names no kernel: '" + operation + "'").c_str());
+    snprintf(out->operation, sizeof out->operation, "%s", operation.c_str());
+    const KDef &ok = kernels[op];
+    if ((int)ok.terms.size() > UHS_MAX_PROGRAM_TERMS)
+        return uhs_set_error(UHS_E_UNSUPPORTED, "more than 32 terms in the operation kernel");
+    // Producers: the earlier kernels the operation reads, in definition order.
+    std::vector<int> prod;
+    for (int k = 0; k < op; k++)
+        for (const KTerm &t : ok.terms)
+            if (t.src == kernels[k].name) {
+                prod.push_back(k);
+                break;
+            }
+    if ((int)prod.size() > UHS_MAX_PRODUCERS)
+        return uhs_set_error(UHS_E_UNSUPPORTED, "the operation reads more than 4 kernels");
+    int reach[3] = {0, 0, 0};
+    for (int i = 0; i < (int)prod.size(); i++) {
+        const KDef &pk = kernels[prod[i]];
+        if ((int)pk.terms.size() > UHS_MAX_PROGRAM_TERMS)
+            return uhs_set_error(UHS_E_UNSUPPORTED, ("more than 32 terms in kernel '" + pk.name + "'").c_str());
+        snprintf(out->producer_names[i], sizeof out->producer_names[i], "%s", pk.name.c_str());
+        out->producer_npoints[i] = (int)pk.terms.size();
+        for (int t = 0; t < (int)pk.terms.size(); t++) {
+            if (pk.terms[t].src != arr)
+                return uhs_set_error(UHS_E_UNSUPPORTED,
+                                     ("kernel '" + pk.name + "' reads another kernel: only one level of producers is fused").c_str());
+            for (int a = 0; a < dims; a++) out->producer_offsets[i][t * dims + a] = pk.terms[t].off[a];
+            out->producer_coeffs[i][t] = pk.terms[t].c;
+        }
+    }
+    out->nproducers = (int)prod.size();
+    out->npoints = (int)ok.terms.size();
+    for (int t = 0; t < out->npoints; t++) {
+        const KTerm &tm = ok.terms[t];
+        int src = -2;
+        if (tm.src == arr) src = -1;
+        for (int i = 0; i < (int)prod.size(); i++)
+            if (tm.src == kernels[prod[i]].name) src = i;
+        if (src == -2)
+            return uhs_set_error(UHS_E_PARSE, ("'" + tm.src + "' is neither the array nor an earlier kernel").c_str());
+        out->sources[t] = src;
+        out->coeffs[t] = tm.c;
+        for (int a = 0; a < dims; a++) {
+            out->offsets[t * dims + a] = tm.off[a];
+            int lo = tm.off[a], hi = tm.off[a];
+            if (src >= 0)
+                for (int u = 0; u < out->producer_npoints[src]; u++) {
+                    const int w = tm.off[a] + out->producer_offsets[src][u * dims + a];
+                    lo = std::min(lo, w);
+                    hi = std::max(hi, w);
+                }
+            reach[a] = std::max(reach[a], std::max(-lo, hi));
+        }
+    }
+    // The written cells: the operation kernel's domain, or the interior at
+    // the program's reach when it has none.
+    for (int a = 0; a < dims; a++) {
+        out->domain_lo[a] = ok.have_domain ? ok.lo[a] : reach[a];
+        out->domain_hi[a] = ok.have_domain ? ok.hi[a] : out->grid[a] - reach[a];
+        if (out->domain_lo[a] < reach[a] || out->domain_hi[a] > out->grid[a] - reach[a] ||
+            out->domain_hi[a] <= out->domain_lo[a])
+            return uhs_set_error(UHS_E_UNSUPPORTED, "the operation's domain reads outside the array");
+    }
+    return UHS_OK;
+}
+
+int uhs_parse_dsl_program_file(const char *path, uhs_dsl_program *out)
+{
+    if (!path) return uhs_set_error(UHS_E_INVAL, "null path");
+    FILE *f = fopen(path, "rb");
+    if (!f) return uhs_set_error(UHS_E_IO, (std::string("cannot open ") + path).c_str());
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+    fclose(f);
+    return uhs_parse_dsl_program(text.c_str(), out);
+}
+
+int uhs_dsl_program_view(const uhs_dsl_program *dp, uhs_stencil *producers, uhs_program *pg)
+{
+    if (!dp || !pg || (dp->nproducers && !producers)) return uhs_set_error(UHS_E_INVAL, "null argument");
+    memset(pg, 0, sizeof *pg);
+    pg->dims = dp->dims;
+    pg->nproducers = dp->nproducers;
+    for (int i = 0; i < dp->nproducers; i++)
+        producers[i] = uhs_stencil{dp->dims, dp->producer_npoints[i], dp->producer_offsets[i], dp->producer_coeffs[i]};
+    pg->producers = producers;
+    pg->output = uhs_stencil{dp->dims, dp->npoints, dp->offsets, dp->coeffs};
+    pg->sources = dp->sources;
+    for (int a = 0; a < dp->dims; a++) {
+        pg->lo[a] = dp->domain_lo[a];
+        pg->hi[a] = dp->domain_hi[a];
+    }
+    return UHS_OK;
 }

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from tests.oracle_py import FUSED, PLAIN
-from tests.test_program import NESTED_2D, random_program, run_parsed
+from tests.test_program import NESTED_2D, max_program, random_program, run_parsed
 
 pytestmark = pytest.mark.gpu
 
@@ -66,6 +66,24 @@ def test_random_programs_bitwise(gpu, orc, seed, kt):
         A, B = A0.copy(), B0.copy()
         assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, T, FUSED) == 0
         assert np.array_equal(gA, A) and np.array_equal(gB, B), (seed, T)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+def test_max_program_bitwise(gpu, orc, dims, kernel):
+    """4 producers of 32 terms, 32 output terms (both producer tables of the
+    tiled kernel in use)."""
+    u = gpu
+    rng = np.random.default_rng(40 + dims)
+    shape, prods, offs, coefs, srcs, lo, hi = max_program(rng, dims)
+    prog = u.ProgramSpec(dims, prods, offs, coefs, srcs, lo, hi)
+    A0, B0 = rng.random(shape), rng.random(shape)
+    log("max program", dims, "D, kernel", kernel)
+    for T in (1, 2):
+        gA, gB = run_gpu(u, prog, A0, B0, T, kernel)
+        A, B = A0.copy(), B0.copy()
+        assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, T, FUSED) == 0
+        assert np.array_equal(gA, A) and np.array_equal(gB, B), T
 
 
 @pytest.mark.parametrize("kernel", [1, 2])

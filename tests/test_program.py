@@ -44,7 +44,7 @@ def two_pass(inp, producers, offsets, coeffs, sources, lo, hi, absolute=False):
 
 
 def random_program(rng, dims):
-    nprod = int(rng.integers(0, 3))
+    nprod = int(rng.integers(0, 5))
     prods = []
     for _ in range(nprod):
         n = int(rng.integers(1, 7))
@@ -81,6 +81,28 @@ def test_oracle_plain_is_two_pass(orc, seed):
             src, dst = (eA, eB) if s & 1 else (eB, eA)
             dst[slab(lo, hi, [0] * dims)] = two_pass(src, prods, offs, coefs, srcs, lo, hi)
         assert np.array_equal(A, eA) and np.array_equal(B, eB), (seed, T)
+
+
+def max_program(rng, dims):
+    """The largest program the interface takes: 4 producers of 32 terms and
+    an output kernel of 32 terms, 8 from each producer (offsets in [-1, 1])."""
+    prods = [(rng.integers(-1, 2, size=(32, dims)).tolist(), rng.uniform(-1, 1, 32).tolist()) for _ in range(4)]
+    offs = rng.integers(-1, 2, size=(32, dims)).tolist()
+    coefs = rng.uniform(-1, 1, 32).tolist()
+    srcs = [j % 4 for j in range(32)]
+    shape = [14, 15, 70] if dims == 3 else [40, 150]
+    return shape, prods, offs, coefs, srcs, [2] * dims, [n - 2 for n in shape]
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_oracle_max_program(orc, dims):
+    rng = np.random.default_rng(40 + dims)
+    shape, prods, offs, coefs, srcs, lo, hi = max_program(rng, dims)
+    A0, B0 = rng.random(shape), rng.random(shape)
+    A, B = A0.copy(), B0.copy()
+    assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, 1, PLAIN) == 0
+    want = two_pass(A0, prods, offs, coefs, srcs, lo, hi)
+    assert np.array_equal(B[slab(lo, hi, [0] * dims)], want)
 
 
 @pytest.mark.parametrize("seed", range(6))

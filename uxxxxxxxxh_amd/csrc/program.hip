@@ -194,22 +194,29 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
         return rr < rows && cc < cols;
     };
 
-    // Term tables in lanes: lane l holds output term l, producer terms l and
-    // 64 + l, and producer l's ring slot.  Each step advances the slots by
+    // Term tables in lanes: lane l holds output term l, producer term l of
+    // each of two tables, and producer l's ring slot and region.  Each step advances the slots by
     // one with one vector operation per table; the term loops read them with
     // readlane (no scalar arithmetic or scalar loads per term).
     const int ol = min(lane, p.nout - 1);
     const int o_N = p.oN[ol], o_fix = p.oring[ol] + p.ooff[ol];
     int o_slot = p.od[ol];
     const double o_c = p.ocoef[ol];
+    // Producer tables: table 0 holds producers 0 and 1 (at most 64 terms),
+    // table 1 producers 2 and 3, each from its first producer's first term.
     const int nptot = p.pstart[p.nprod];
-    const int pl0 = min(lane, max(nptot - 1, 0)), pl1 = min(lane + 64, max(nptot - 1, 0));
+    const int t1 = p.pstart[min(2, p.nprod)];
+    const int pl0 = min(lane, max(t1 - 1, 0)), pl1 = min(t1 + lane, max(nptot - 1, 0));
     const int p_off0 = p.poff[pl0], p_off1 = p.poff[pl1];
     int p_slot0 = p.pd[pl0], p_slot1 = p.pd[pl1];
     const double p_c0 = p.pcoef[pl0], p_c1 = p.pcoef[pl1];
+    // Lane k: producer k's ring slot and length, and its region.
     const int kl = min(lane, kMaxProd - 1);
     const int k_N = lane < p.nprod ? p.pN[kl] : 1;
     int k_slot = lane < p.nprod ? p.cr_a[kl] : 0;
+    const int k_rows = p.cr_b[kl], k_cols = p.cr_c[kl], k_ring = p.pring[kl];
+    const int k_li = (p.clo_b[kl] - p.flo_b) * p.Hc + (p.clo_c[kl] - p.flo_c);
+    const int k_first = p.pstart[kl] - (kl < 2 ? 0 : t1), k_end = p.pstart[kl + 1] - (kl < 2 ? 0 : t1);
     // Wait for the tables here, once: otherwise the first use inside the
     // plane loop waits on a count that also covers the planes in flight.
     __builtin_amdgcn_s_waitcnt(0);
@@ -280,10 +287,13 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
         const int pb0 = sp0 * p.plane + p_off0;
         const int pb1 = sp1 * p.plane + p_off1;
         for (int k = 0; k < p.nprod; k++) {
-            const int rows = eb + p.cr_b[k], cols = ec + p.cr_c[k];
+            const int rows = eb + lane_int(k_rows, k), cols = ec + lane_int(k_cols, k);
             if (rows <= 0) continue;   // a producer the output kernel never reads
-            double *dst = lds + p.pring[k] + lane_int(sk, k) * p.plane;
-            const int b = p.pstart[k], e = p.pstart[k + 1];
+            double *dst = lds + lane_int(k_ring, k) + lane_int(sk, k) * p.plane;
+            const int b = lane_int(k_first, k), e = lane_int(k_end, k), lk = lane_int(k_li, k);
+            // This producer's table (a per-lane select once per producer).
+            const int pb = k < 2 ? pb0 : pb1;
+            const double pc = k < 2 ? p_c0 : p_c1;
             int li[UC];
             bool ok[UC];
             double s[UC];
@@ -291,15 +301,11 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
             for (int u = 0; u < UC; u++) {
                 int r, c;
                 ok[u] = cell(u, rows, cols, r, c);
-                li[u] = (r + p.clo_b[k] - p.flo_b) * p.Hc + (c + p.clo_c[k] - p.flo_c);
+                li[u] = r * p.Hc + c + lk;
             }
-            // Producer term q sits in lane q & 63 of the first or second table.
             auto term = [&](int q, int &off, double &cf) {
-                const int l = q & 63;
-                const int o0 = lane_int(pb0, l), o1 = lane_int(pb1, l);
-                const double c0_ = lane_dbl(p_c0, l), c1_ = lane_dbl(p_c1, l);
-                off = q < 64 ? o0 : o1;
-                cf = q < 64 ? c0_ : c1_;
+                off = lane_int(pb, q);
+                cf = lane_dbl(pc, q);
             };
             {
                 int off;
@@ -602,7 +608,8 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
 
 // Default tiles, first fit: rows of 64 << n cells once the reach is added;
 // 2D 16-row slabs of 512-cell rows, 3D 6 rows of 128 cells x 16-plane slabs
-// (the fastest of a sweep over both reference example programs).
+// (the fastest of a sweep over both reference example programs), then
+// smaller tiles for programs whose rings need more LDS.
 int default_tiles(const Checked &c, int d, int tiles[][3])
 {
     const long long rc = c.in.hi[2] - c.in.lo[2];
@@ -611,11 +618,10 @@ int default_tiles(const Checked &c, int d, int tiles[][3])
         for (int w : {512, 256, 1024})
             if (w - rc >= 64) { tiles[n][0] = (int)(w - rc); tiles[n][1] = 16; tiles[n][2] = 1; n++; }
     } else {
-        for (int w : {128, 64, 256}) {
-            if (w - rc < 32) continue;
-            for (int tb : {6, 4, 2, 1}) { tiles[n][0] = (int)(w - rc); tiles[n][1] = tb; tiles[n][2] = 16; n++; }
-            break;
-        }
+        // Narrower rows only when the wider ones leave no room in the LDS.
+        for (int w : {128, 64})
+            if (w - rc >= 32)
+                for (int tb : {6, 4, 2, 1}) { tiles[n][0] = (int)(w - rc); tiles[n][1] = tb; tiles[n][2] = 16; n++; }
     }
     return n;
 }

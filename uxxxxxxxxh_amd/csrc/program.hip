@@ -22,6 +22,7 @@
 #include "uhs_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -83,8 +84,11 @@ __global__ void __launch_bounds__(256) k_program(ProgParams p)
 // cells, Hc a multiple of 64; every staged plane has that layout, so a
 // term's in-plane LDS offset is the constant db * Hc + dc.  Planes live in
 // LDS rings, exactly as long as they need to be:
-//   input ring    ir_a + 2 planes: the ir_a + 1 a step reads, and the one
-//                 the next step stores before its first barrier;
+//   input ring    the planes one step reads, span_a + 1 (each producer
+//                 reads its newest plane's inputs, the output kernel its own
+//                 input terms: offsets [nlo_a, nlo_a + span_a] from the
+//                 written plane), plus the one the next step stores before
+//                 its first barrier;
 //   producer k    cr_k + 1 planes of its values.
 // Ring index of a plane = its distance from the slab's first plane of that
 // ring; a term at distance d reads ring index s + d at step s.  Each lane
@@ -102,6 +106,7 @@ struct StreamParams {
     int flo_b, flo_c;              // frame in-plane origin relative to the tile (<= 0)
     int ilo_a, ilo_b, ilo_c;       // input reach lower offsets
     int ir_a, ir_b, ir_c;          // input reach widths
+    int nlo_a, span_a;             // input planes one step reads: [nlo_a, nlo_a + span_a]
     int NI;                        // input ring length
     int lead;                      // planes before the slab's first written plane
     int nprod, nout;
@@ -250,24 +255,25 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
             if (okin[u]) s[lin[u]] = v[u];
     };
 
-    const int A = a0 - p.lead;     // first step: planes [A + ilo_a, ...] staged
-    for (int i = 0; i < p.ir_a; i++) {
+    const int A = a0 - p.lead;     // first step: planes [A + nlo_a, ...] staged
+    const int nhi = p.nlo_a + p.span_a;
+    for (int i = 0; i < p.span_a; i++) {
         double v[US];
-        load_plane(A + p.ilo_a + i, v);
+        load_plane(A + p.nlo_a + i, v);
         store_plane(i, v);
     }
     // PF planes in flight in a register ring: step s stores ring slot s % PF
-    // (the plane of ring index s + ir_a) and refills it with the plane PF
+    // (the plane of ring index s + span_a) and refills it with the plane PF
     // steps ahead.  The loop is unrolled by PF so slots are registers; the
     // refill loads are unconditional (clamped planes) and never copied.
     double pf[PF][US];
     // Issued in slot order (no reordering across slots), so the loop entry
     // holds the same loads in flight as the loop's back edge.
     static_for<PF>([&](auto f) {
-        load_plane(A + p.ilo_a + p.ir_a + f, pf[f]);
+        load_plane(A + nhi + f, pf[f]);
         __builtin_amdgcn_sched_barrier(0);
     });
-    int in_slot = p.ir_a;          // slot of the plane stored at the current step
+    int in_slot = p.span_a;        // slot of the plane stored at the current step
     auto step = [](int v, int n) { return v + 1 == n ? 0 : v + 1; };
     for (int i0 = 0; A + i0 < a_end; i0 += PF)
         static_for<PF>([&](auto ph) {
@@ -280,7 +286,7 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
         p_slot1 = step(p_slot1, p.NI);
         k_slot = step(k_slot, k_N);
         if (a < a_end) store_plane(sin, pf[ph]);
-        load_plane(a + PF + p.ilo_a + p.ir_a, pf[ph]);
+        load_plane(a + PF + nhi, pf[ph]);
         if (a >= a_end) return;
         __syncthreads();
         // Producer k's plane a + chi_a.
@@ -550,7 +556,30 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
     p.ir_a = (int)ira;
     p.ir_b = (int)(c.in.hi[ax_b] - c.in.lo[ax_b]);
     p.ir_c = (int)(c.in.hi[ax_c] - c.in.lo[ax_c]);
-    const int ni = p.ir_a + 2;
+    // Input planes one step reads (offsets from the written plane a): the
+    // producers' newest planes a + chi_k read a + chi_k + w; the output
+    // kernel's input terms a + v.
+    long long nlo = LLONG_MAX, nhi = LLONG_MIN;
+    for (int k = 0; k < pg->nproducers; k++) {
+        if (!c.cons[k].any) continue;
+        const uhs_stencil &s = pg->producers[k];
+        for (int t = 0; t < s.npoints; t++) {
+            long long w[3];
+            off3(s.offsets, d, t, w);
+            nlo = std::min(nlo, c.cons[k].hi[ax_a] + w[ax_a]);
+            nhi = std::max(nhi, c.cons[k].hi[ax_a] + w[ax_a]);
+        }
+    }
+    for (int j = 0; j < o.npoints; j++)
+        if (pg->sources[j] < 0) {
+            long long v[3];
+            off3(o.offsets, d, j, v);
+            nlo = std::min(nlo, v[ax_a]);
+            nhi = std::max(nhi, v[ax_a]);
+        }
+    p.nlo_a = (int)nlo;
+    p.span_a = (int)(nhi - nlo);
+    const int ni = p.span_a + 2;
     p.NI = ni;
     p.lead = maxcr;
     p.nprod = pg->nproducers;
@@ -581,7 +610,7 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
             long long w[3];
             off3(s.offsets, d, t, w);
             // Plane a + chi_a + w_a of the input: ring distance from the step.
-            p.pd[nterms] = r.any ? (int)(chi_a + w[ax_a] - c.in.lo[ax_a]) : 0;
+            p.pd[nterms] = r.any ? (int)(chi_a + w[ax_a] - nlo) : 0;
             p.poff[nterms] = (int)(w[ax_b] * p.Hc + w[ax_c]);
             p.pcoef[nterms] = s.coeffs[t];
         }
@@ -594,7 +623,7 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
         if (k < 0) {
             p.oring[j] = 0;
             p.oN[j] = p.NI;
-            p.od[j] = (int)(v[ax_a] - c.in.lo[ax_a]);
+            p.od[j] = (int)(v[ax_a] - nlo);
         } else {
             p.oring[j] = p.pring[k];
             p.oN[j] = p.pN[k];
@@ -607,7 +636,7 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
 }
 
 // Default tiles, first fit: rows of 64 << n cells once the reach is added;
-// 2D 16-row slabs of 512-cell rows, 3D 6 rows of 128 cells x 16-plane slabs
+// 2D 32-row slabs of 1024-cell rows, 3D 6 rows of 128 cells x 8-plane slabs
 // (the fastest of a sweep over both reference example programs), then
 // smaller tiles for programs whose rings need more LDS.
 int default_tiles(const Checked &c, int d, int tiles[][3])
@@ -615,13 +644,13 @@ int default_tiles(const Checked &c, int d, int tiles[][3])
     const long long rc = c.in.hi[2] - c.in.lo[2];
     int n = 0;
     if (d == 2) {
-        for (int w : {512, 256, 1024})
-            if (w - rc >= 64) { tiles[n][0] = (int)(w - rc); tiles[n][1] = 16; tiles[n][2] = 1; n++; }
+        for (int w : {1024, 512, 256})
+            if (w - rc >= 64) { tiles[n][0] = (int)(w - rc); tiles[n][1] = w == 1024 ? 32 : 16; tiles[n][2] = 1; n++; }
     } else {
         // Narrower rows only when the wider ones leave no room in the LDS.
         for (int w : {128, 64})
             if (w - rc >= 32)
-                for (int tb : {6, 4, 2, 1}) { tiles[n][0] = (int)(w - rc); tiles[n][1] = tb; tiles[n][2] = 16; n++; }
+                for (int tb : {6, 4, 2, 1}) { tiles[n][0] = (int)(w - rc); tiles[n][1] = tb; tiles[n][2] = 8; n++; }
     }
     return n;
 }

@@ -164,10 +164,12 @@ def batched(u, T=50, probe_T=8):
             "tasks": info["tasks"]}
 
 
-# The two multi-kernel example programs of the reference
-# (UHStencil/examples/2d5pt_nested/stencil_2d5pt_nested.dsl and
-# 3d7pt9pt_nested/stencil_3d7pt9pt_nested.dsl), at their own grid sizes,
-# domains and iteration counts, as the multi-kernel DSL reader reads them.
+# The reference's multi-kernel and one-sided example programs
+# (UHStencil/examples/2d5pt_nested/stencil_2d5pt_nested.dsl,
+# 3d7pt9pt_nested/stencil_3d7pt9pt_nested.dsl,
+# 2d5pt_arbitrary_shape/stencil_2d5pt_arbitrary_shape.dsl and
+# 3d7pt_arbitrary_shape/stencil_3d7pt_arbitrary_shape.dsl), at their own grid
+# sizes, domains and iteration counts, as the DSL program reader reads them.
 NESTED_PROGRAMS = {
     "2d5pt_nested": dict(
         dims=2, shape=[4098, 4098], lo=[17, 17], hi=[4081, 4081], T=20,
@@ -179,6 +181,13 @@ NESTED_PROGRAMS = {
         offsets=[[-1, -1, -1], [-1, -1, 1], [-1, 1, -1], [-1, 1, 1], [0, 0, 0],
                  [1, -1, -1], [1, -1, 1], [1, 1, -1], [1, 1, 1]],
         coeffs=[0.1, 0.3, 0.7, 0.9, 0.5, 0.1, 0.3, 0.7, 0.9], sources=[-1] * 9),
+    "2d5pt_arbitrary_shape": dict(
+        dims=2, shape=[4098, 4098], lo=[2, 0], hi=[4098, 4096], T=20, producers=[],
+        offsets=[[-2, 0], [-1, 0], [0, 0], [0, 1], [0, 2]], coeffs=[0.1, 0.2, 0.3, 0.4, 0.5], sources=[-1] * 5),
+    "3d7pt_arbitrary_shape": dict(
+        dims=3, shape=[258, 258, 258], lo=[0, 0, 0], hi=[256, 256, 256], T=20, producers=[],
+        offsets=[[2, 0, 0], [1, 0, 0], [0, 0, 0], [0, 0, 1], [0, 0, 2], [0, 1, 0], [0, 2, 0]],
+        coeffs=[0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7], sources=[-1] * 7),
 }
 
 

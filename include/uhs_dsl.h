@@ -42,8 +42,9 @@ int uhs_parse_dsl_file(const char *path, uhs_dsl_info *out);
  * (the last one when `operation` is absent) becomes the program's output;
  * the earlier kernels it reads become its producers, which must read only
  * the array (one level of fusion); kernels it does not read are dropped.
- * The output domain is the operation kernel's domain, or the interior at the
- * program's reach.  Up to UHS_MAX_PRODUCERS producers and
+ * The output domain is the operation kernel's domain, or, without one, the
+ * cells whose reads stay inside the array (one-sided programs keep the
+ * side they do not read).  Up to UHS_MAX_PRODUCERS producers and
  * UHS_MAX_PROGRAM_TERMS terms per kernel (UHS_E_UNSUPPORTED beyond). */
 typedef struct uhs_dsl_program {
     char name[64];

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from tests.oracle_py import FUSED, PLAIN
-from tests.test_program import NESTED_2D, max_program, random_program, run_parsed
+from tests.test_program import NESTED_2D, ONE_SIDED_2D, max_program, random_program, run_parsed
 
 pytestmark = pytest.mark.gpu
 
@@ -138,4 +138,28 @@ def test_far_reach_falls_back_to_pointwise(gpu, orc):
     gA, gB = run_gpu(u, prog, A0, B0, 2)
     A, B = A0.copy(), B0.copy()
     assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, 2, FUSED) == 0
+    assert np.array_equal(gA, A) and np.array_equal(gB, B)
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_one_sided_programs(gpu, orc, kernel):
+    """Domains that touch the grid's edge on the side a program does not
+    read (the reference's arbitrary-shape examples, at small sizes)."""
+    u = gpu
+    prog, info = u.parse_dsl_program(ONE_SIDED_2D)
+    rng = np.random.default_rng(31)
+    A0, B0 = rng.random((66, 66)), np.zeros((66, 66))
+    gA, gB = run_gpu(u, prog, A0, B0, 4, kernel)
+    A, B = A0.copy(), B0.copy()
+    assert run_parsed(orc, prog, A, B, 4, FUSED) == 0
+    assert np.array_equal(gA, A) and np.array_equal(gB, B)
+    offs = [[2, 0, 0], [1, 0, 0], [0, 0, 0], [0, 0, 1], [0, 0, 2], [0, 1, 0], [0, 2, 0]]
+    coefs = [0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7]
+    shape, lo, hi = [34, 36, 70], [0, 0, 0], [32, 34, 68]
+    prog3 = u.ProgramSpec(3, [], offs, coefs, [-1] * 7, lo, hi)
+    A0, B0 = rng.random(shape), np.zeros(shape)
+    log("one-sided 3d, kernel", kernel)
+    gA, gB = run_gpu(u, prog3, A0, B0, 2, kernel)
+    A, B = A0.copy(), B0.copy()
+    assert orc.program(A, B, shape, [], offs, coefs, [-1] * 7, lo, hi, 2, FUSED) == 0
     assert np.array_equal(gA, A) and np.array_equal(gB, B)

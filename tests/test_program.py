@@ -233,8 +233,9 @@ def test_single_kernel_program_matches_parse_dsl(uhs):
     assert info.nproducers == 0 and prog.sources.tolist() == [-1, -1, -1]
     assert prog.output.offsets.tolist() == st.offsets.tolist()
     assert prog.output.coeffs.tolist() == st.coeffs.tolist()
-    # Default domain: the interior at the program's reach.
-    assert prog.lo == [1, 1, 1] and prog.hi == [9, 11, 13]
+    # Default domain: the cells whose reads stay inside, per axis and side
+    # (z reads -1 and +1, y only -1, x only +1).
+    assert prog.lo == [1, 1, 0] and prog.hi == [9, 12, 13]
 
 
 def test_program_reader_errors(uhs):
@@ -307,6 +308,73 @@ def test_program_kernel_choice_errors(uhs):
         uhs.program_sweep_f64(A, B, [20, 70], two_d, 1, tile=[60, 8, 2])
     # Nothing launched for T = 0, whatever the choice.
     uhs.program_sweep_f64(A, B, shape, near, 0, kernel=uhs.PROGRAM_TILED, tile=[60, 8, 4])
+
+
+ONE_SIDED_2D = """
+stencil one_sided_small(double input[66][66]) {
+    iteration(4)
+    operation (k)
+    kernel k {
+        domain([2, 66][0, 64])
+        expr {
+            0.1*input[y-2][x] + 0.2*input[y-1][x] +
+            0.3*input[y][x] + 0.4*input[y][x+1] + 0.5*input[y][x+2]
+        }
+    }
+}
+"""
+
+
+def one_sided_check_loop(inp, T):
+    """The reference driver's check loop for its one-sided example, restated
+    in numpy: pairs of sweeps input -> tmp -> input over rows [2, M) and
+    columns [0, N - 2), tmp starting at zero
+    (UHStencil/examples/2d5pt_arbitrary_shape/stencil_2d5pt_arbitrary_shape_driver.serial.c:37-52)."""
+    a = inp.copy()
+    tmp = np.zeros_like(a)
+    M, N = a.shape
+
+    def sweep(src, dst):
+        dst[2:M, 0:N - 2] = (0.1 * src[0:M - 2, 0:N - 2] + 0.2 * src[1:M - 1, 0:N - 2]
+                             + 0.3 * src[2:M, 0:N - 2] + 0.4 * src[2:M, 1:N - 1] + 0.5 * src[2:M, 2:N])
+
+    for _ in range(T // 2):
+        sweep(a, tmp)
+        sweep(tmp, a)
+    return a
+
+
+def test_one_sided_program_and_reference_check_loop(uhs, orc):
+    prog, info = uhs.parse_dsl_program(ONE_SIDED_2D)
+    assert info.nproducers == 0 and prog.lo == [2, 0] and prog.hi == [66, 64]
+    assert prog.output.offsets.tolist() == [[-2, 0], [-1, 0], [0, 0], [0, 1], [0, 2]]
+    rng = np.random.default_rng(9)
+    A = rng.random((66, 66))
+    B = np.zeros_like(A)
+    want = one_sided_check_loop(A, 4)
+    assert run_parsed(orc, prog, A, B, 4, PLAIN) == 0
+    assert np.array_equal(A, want)
+    # Without a domain, the written cells are those whose reads stay inside:
+    # rows [2, 66), columns [0, 64).
+    prog2, _ = uhs.parse_dsl_program(ONE_SIDED_2D.replace("domain([2, 66][0, 64])", ""))
+    assert prog2.lo == [2, 0] and prog2.hi == [66, 64]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference example programs not present")
+def test_reference_one_sided_examples(uhs, orc):
+    prog, info = uhs.parse_dsl_program(path=os.path.join(REF, "2d5pt_arbitrary_shape", "stencil_2d5pt_arbitrary_shape.dsl"))
+    assert info.iteration == 20 and list(info.grid[:2]) == [4098, 4098]
+    assert prog.lo == [2, 0] and prog.hi == [4098, 4096] and info.nproducers == 0
+    rng = np.random.default_rng(12)
+    A = rng.random((4098, 4098))
+    B = np.zeros_like(A)
+    want = one_sided_check_loop(A, 2)
+    assert run_parsed(orc, prog, A, B, 2, PLAIN) == 0
+    assert np.array_equal(A, want)
+    del A, B, want
+    prog, info = uhs.parse_dsl_program(path=os.path.join(REF, "3d7pt_arbitrary_shape", "stencil_3d7pt_arbitrary_shape.dsl"))
+    assert prog.lo == [0, 0, 0] and prog.hi == [256, 256, 256] and list(info.grid[:3]) == [258, 258, 258]
+    assert prog.output.offsets.tolist() == [[2, 0, 0], [1, 0, 0], [0, 0, 0], [0, 0, 1], [0, 0, 2], [0, 1, 0], [0, 2, 0]]
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference example programs not present")

@@ -546,7 +546,9 @@ int uhs_parse_dsl_program(const char *text, uhs_dsl_program *out)
             }
     if ((int)prod.size() > UHS_MAX_PRODUCERS)
         return uhs_set_error(UHS_E_UNSUPPORTED, "the operation reads more than 4 kernels");
-    int reach[3] = {0, 0, 0};
+    // Reach of the program per axis, below and above the written cell (one-
+    // sided programs read only one side).
+    int rlo[3] = {0, 0, 0}, rhi[3] = {0, 0, 0};
     for (int i = 0; i < (int)prod.size(); i++) {
         const KDef &pk = kernels[prod[i]];
         if ((int)pk.terms.size() > UHS_MAX_PROGRAM_TERMS)
@@ -582,15 +584,16 @@ int uhs_parse_dsl_program(const char *text, uhs_dsl_program *out)
                     lo = std::min(lo, w);
                     hi = std::max(hi, w);
                 }
-            reach[a] = std::max(reach[a], std::max(-lo, hi));
+            rlo[a] = std::max(rlo[a], -lo);
+            rhi[a] = std::max(rhi[a], hi);
         }
     }
-    // The written cells: the operation kernel's domain, or the interior at
-    // the program's reach when it has none.
+    // The written cells: the operation kernel's domain, or, when it has
+    // none, the cells whose reads (below and above) stay inside the array.
     for (int a = 0; a < dims; a++) {
-        out->domain_lo[a] = ok.have_domain ? ok.lo[a] : reach[a];
-        out->domain_hi[a] = ok.have_domain ? ok.hi[a] : out->grid[a] - reach[a];
-        if (out->domain_lo[a] < reach[a] || out->domain_hi[a] > out->grid[a] - reach[a] ||
+        out->domain_lo[a] = ok.have_domain ? ok.lo[a] : rlo[a];
+        out->domain_hi[a] = ok.have_domain ? ok.hi[a] : out->grid[a] - rhi[a];
+        if (out->domain_lo[a] < rlo[a] || out->domain_hi[a] > out->grid[a] - rhi[a] ||
             out->domain_hi[a] <= out->domain_lo[a])
             return uhs_set_error(UHS_E_UNSUPPORTED, "the operation's domain reads outside the array");
     }

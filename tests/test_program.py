@@ -225,6 +225,24 @@ def test_default_operation_and_input_terms(uhs):
     assert prog.lo == [3, 3] and prog.hi == [9, 11]
 
 
+def test_program_view_points_into_the_parsed_program(uhs):
+    """uhs_dsl_program_view: the uhs_program a C caller hands to
+    uhs_program_sweep_f64, built from the parsed struct."""
+    import ctypes
+    _, info = uhs.parse_dsl_program(NESTED_2D)
+    prods = (uhs.Stencil * uhs.MAX_PRODUCERS)()
+    pg = uhs.Program()
+    uhs.check(uhs.lib().uhs_dsl_program_view(ctypes.byref(info), prods, ctypes.byref(pg)))
+    assert pg.dims == 2 and pg.nproducers == 1 and list(pg.lo[:2]) == [17, 17] and list(pg.hi[:2]) == [81, 81]
+    assert pg.output.npoints == 5 and [pg.output.coeffs[i] for i in range(5)] == [0.1, 0.3, -0.5, 0.3, 0.1]
+    assert [pg.sources[i] for i in range(5)] == [0] * 5
+    p0 = pg.producers[0]
+    assert p0.npoints == 5 and [p0.offsets[i] for i in range(10)] == [-1, 0, 0, -1, 0, 0, 0, 1, 1, 0]
+    assert ctypes.addressof(p0.coeffs.contents) == ctypes.addressof(info.producer_coeffs[0])
+    with pytest.raises(uhs.UhsError):
+        uhs.check(uhs.lib().uhs_dsl_program_view(ctypes.byref(info), None, ctypes.byref(pg)))
+
+
 def test_single_kernel_program_matches_parse_dsl(uhs):
     text = """stencil one(double f[10][12][14]) { iteration(4)
       kernel k { expr { 0.5*f[z-1][y][x] + 0.25*f[z][y][x+1] - f[z+1][y-1][x] } } }"""

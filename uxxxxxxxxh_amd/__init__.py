@@ -116,6 +116,7 @@ _SIGS = {
     "uhs_parse_dsl_program": (_I, [ctypes.c_char_p, ctypes.POINTER(DslProgram)]),
     "uhs_parse_dsl_program_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslProgram)]),
     "uhs_program_sweep_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _P]),
+    "uhs_dsl_program_view": (_I, [ctypes.POINTER(DslProgram), ctypes.POINTER(Stencil), ctypes.POINTER(Program)]),
     "uhs_program_sweep_ex_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _I, ctypes.POINTER(ctypes.c_int), _P]),
     "uhs_parse_dsl_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
     "uhs_batch_create": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(BatchConfig), ctypes.POINTER(_P)]),

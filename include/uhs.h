@@ -234,11 +234,12 @@ int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps,
 
 /* A stencil program of several kernels after kernel fusion (the reference's
  * StencilKernelFusion.cpp:66-180 inlines a producer kernel into its single
- * consumer): the output kernel's term j reads either the input grid at
- * p + o_j or producer k evaluated at p + o_j, and every producer is a
- * stencil over the input grid.  So
- *     out[p] = c_0*x_0 + c_1*x_1 + ...,   x_j = in[p+o_j]  or
- *     x_j = d_0*in[p+o_j+e_0] + d_1*in[p+o_j+e_1] + ...   (producer k)
+ * consumer at every access offset, repeatedly, so chains fuse completely):
+ * the output kernel's term j reads either the input grid at p + o_j or
+ * producer k evaluated at p + o_j, and producer k's terms read the input
+ * grid or an earlier producer (index < k) the same way.  So
+ *     out[p] = c_0*x_0 + c_1*x_1 + ...,   x_j = in[p+o_j]  or  P_k(p+o_j)
+ *     P_k(q) = d_0*y_0 + d_1*y_1 + ...,   y_i = in[q+e_i]  or  P_s(q+e_i)
  * each sum left to right in term order (first term a product, later terms
  * fused multiply-adds, as in uhs_sweep_f64).  Up to UHS_MAX_PRODUCERS
  * producers and UHS_MAX_PROGRAM_TERMS terms per kernel. */
@@ -247,11 +248,14 @@ int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps,
 typedef struct uhs_program {
     int dims;                      /* 2 or 3 */
     int nproducers;
-    const uhs_stencil *producers;  /* [nproducers], each over the input grid */
+    const uhs_stencil *producers;  /* [nproducers], in dependency order */
     uhs_stencil output;            /* the output ("operation") kernel */
     const int *sources;            /* [output.npoints]: -1 = the input grid, k = producers[k] */
     int64_t lo[3], hi[3];          /* cells written: [lo, hi) per axis, in halo-inclusive
                                       grid coordinates (the output kernel's domain) */
+    const int *producer_sources;   /* [nproducers][UHS_MAX_PROGRAM_TERMS]: term i of
+                                      producer k reads the input (-1) or producer s < k;
+                                      NULL = every producer reads only the input grid */
 } uhs_program;
 
 /* T sweeps of a program over halo-inclusive FP64 grids of `shape` cells per

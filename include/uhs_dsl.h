@@ -40,9 +40,10 @@ int uhs_parse_dsl_file(const char *path, uhs_dsl_info *out);
 /* Programs of several kernels (the fused "nested" examples,
  * UHStencil/examples/{2d5pt_nested,3d7pt9pt_nested}/).  The operation kernel
  * (the last one when `operation` is absent) becomes the program's output;
- * the earlier kernels it reads become its producers, which must read only
- * the array (one level of fusion); kernels it does not read are dropped.
- * The output domain is the operation kernel's domain, or, without one, the
+ * every kernel it reads, directly or through other kernels, becomes a
+ * producer, in definition order (a kernel may read the array and kernels
+ * defined before it); kernels it does not read are dropped.  The output
+ * domain is the operation kernel's domain, or, without one, the
  * cells whose reads stay inside the array (one-sided programs keep the
  * side they do not read).  Up to UHS_MAX_PRODUCERS producers and
  * UHS_MAX_PROGRAM_TERMS terms per kernel (UHS_E_UNSUPPORTED beyond). */
@@ -57,6 +58,7 @@ typedef struct uhs_dsl_program {
     int producer_npoints[UHS_MAX_PRODUCERS];
     int producer_offsets[UHS_MAX_PRODUCERS][UHS_MAX_PROGRAM_TERMS * 3];
     double producer_coeffs[UHS_MAX_PRODUCERS][UHS_MAX_PROGRAM_TERMS];
+    int producer_sources[UHS_MAX_PRODUCERS][UHS_MAX_PROGRAM_TERMS]; /* -1 the array, s producer s */
     int npoints;                     /* output kernel terms */
     int offsets[UHS_MAX_PROGRAM_TERMS * 3];
     double coeffs[UHS_MAX_PROGRAM_TERMS];

@@ -10,6 +10,7 @@
  * point is bitwise independent of the thread count.
  */
 #include "uhs_oracle.h"
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -203,22 +204,34 @@ void orc_fill_rand50_f64(double *a, size_t n)
     for (size_t i = 0; i < n; i++) a[i] = rand() % 50;
 }
 
-/* ---- fused two-level programs ------------------------------------------- */
+/* ---- fused multi-kernel programs ---------------------------------------- */
 
-/* The value of producer term list [b, e) at linear cell q. */
-static double prog_producer(const double *in, long long q, int b, int e, const long long *poff,
-                            const double *pcoef, int mode)
+struct prog_ctx {
+    const double *in;
+    const int *pstart, *psrc;      /* producer k's terms [pstart[k], pstart[k+1]); psrc NULL: all input */
+    const long long *poff;
+    const double *pcoef;
+    int mode;
+};
+
+/* The value of producer k at linear cell q: its terms left to right, each
+ * reading the input grid or an earlier producer evaluated at the shifted
+ * cell (the reference's inlining at every access offset). */
+static double prog_value(const struct prog_ctx *c, int k, long long q)
 {
-    double s = pcoef[b] * in[q + poff[b]];
-    for (int t = b + 1; t < e; t++) {
-        const double v = in[q + poff[t]];
-        s = mode == ORC_FUSED ? fma(pcoef[t], v, s) : s + pcoef[t] * v;
+    double s = 0.0;
+    for (int t = c->pstart[k]; t < c->pstart[k + 1]; t++) {
+        const int src = c->psrc ? c->psrc[t] : -1;
+        const double v = src < 0 ? c->in[q + c->poff[t]] : prog_value(c, src, q + c->poff[t]);
+        if (t == c->pstart[k]) s = c->pcoef[t] * v;
+        else s = c->mode == ORC_FUSED ? fma(c->pcoef[t], v, s) : s + c->pcoef[t] * v;
     }
     return s;
 }
 
 int orc_program_f64(double *A, double *B, int dims, const int64_t *shape,
                     int nprod, const int *prod_np, const int *prod_off, const double *prod_coef,
+                    const int *prod_src,
                     int np, const int *off, const double *coef, const int *src,
                     const int64_t *lo, const int64_t *hi, int T, int mode)
 {
@@ -243,17 +256,30 @@ int orc_program_f64(double *A, double *B, int dims, const int64_t *shape,
     long long *poff = (long long *)malloc(sizeof(long long) * (ntot + 1));
     long long *ooff = (long long *)malloc(sizeof(long long) * np);
     int *pstart = (int *)malloc(sizeof(int) * (nprod + 1));
-    if (!poff || !ooff || !pstart) {
-        free(poff); free(ooff); free(pstart);
+    long long *reach = (long long *)malloc(sizeof(long long) * 6 * (nprod + 1));
+    if (!poff || !ooff || !pstart || !reach) {
+        free(poff); free(ooff); free(pstart); free(reach);
         return -1;
     }
     int rc = 0, t = 0;
-    for (int k = 0; k < nprod; k++) {
+    /* reach[k]: input offsets producer k reads, per axis [lo, hi]; a
+     * producer may read only earlier producers. */
+    for (int k = 0; k < nprod && !rc; k++) {
         pstart[k] = t;
+        long long *r = reach + 6 * k;
+        for (int a = 0; a < 3; a++) { r[a] = LLONG_MAX; r[3 + a] = LLONG_MIN; }
         for (int i = 0; i < prod_np[k]; i++, t++) {
             long long v[3] = {0, 0, 0};
             for (int a = 0; a < dims; a++) v[3 - dims + a] = prod_off[t * dims + a];
             poff[t] = v[0] * sz + v[1] * sy + v[2];
+            const int s = prod_src ? prod_src[t] : -1;
+            if (s < -1 || s >= k) { rc = -1; break; }
+            for (int a = 0; a < 3; a++) {
+                const long long rl = s < 0 ? v[a] : v[a] + reach[6 * s + a];
+                const long long rh = s < 0 ? v[a] : v[a] + reach[6 * s + 3 + a];
+                if (rl < r[a]) r[a] = rl;
+                if (rh > r[3 + a]) r[3 + a] = rh;
+            }
         }
     }
     pstart[nprod] = t;
@@ -263,36 +289,29 @@ int orc_program_f64(double *A, double *B, int dims, const int64_t *shape,
         long long v[3] = {0, 0, 0};
         for (int a = 0; a < dims; a++) v[3 - dims + a] = off[j * dims + a];
         ooff[j] = v[0] * sz + v[1] * sy + v[2];
-        const int b = src[j] < 0 ? 0 : pstart[src[j]], e = src[j] < 0 ? 1 : pstart[src[j] + 1];
-        for (int i = b; i < e && !rc; i++) {
-            for (int a = 0; a < 3; a++) {
-                long long w = v[a];
-                if (src[j] >= 0) {
-                    const int ax = a - (3 - dims);
-                    w += ax >= 0 ? prod_off[i * dims + ax] : 0;
-                }
-                if (l[a] + w < 0 || h[a] - 1 + w >= shp[a]) rc = -1;
-            }
+        for (int a = 0; a < 3; a++) {
+            const long long rl = src[j] < 0 ? v[a] : v[a] + reach[6 * src[j] + a];
+            const long long rh = src[j] < 0 ? v[a] : v[a] + reach[6 * src[j] + 3 + a];
+            if (l[a] + rl < 0 || h[a] - 1 + rh >= shp[a]) rc = -1;
         }
     }
     for (int step = 1; step <= T && !rc; step++) {
         const double *in = (step & 1) ? A : B;
         double *out = (step & 1) ? B : A;
+        struct prog_ctx c = {in, pstart, prod_src, poff, prod_coef, mode};
         for (long long z = l[0]; z < h[0]; z++)
             for (long long y = l[1]; y < h[1]; y++)
                 for (long long x = l[2]; x < h[2]; x++) {
-                    const long long c = z * sz + y * sy + x;
+                    const long long q = z * sz + y * sy + x;
                     double acc = 0.0;
                     for (int j = 0; j < np; j++) {
-                        const double xv = src[j] < 0 ? in[c + ooff[j]]
-                                                     : prog_producer(in, c + ooff[j], pstart[src[j]],
-                                                                     pstart[src[j] + 1], poff, prod_coef, mode);
+                        const double xv = src[j] < 0 ? in[q + ooff[j]] : prog_value(&c, src[j], q + ooff[j]);
                         if (j == 0) acc = coef[0] * xv;
                         else acc = mode == ORC_FUSED ? fma(coef[j], xv, acc) : acc + coef[j] * xv;
                     }
-                    out[c] = acc;
+                    out[q] = acc;
                 }
     }
-    free(poff); free(ooff); free(pstart);
+    free(poff); free(ooff); free(pstart); free(reach);
     return rc;
 }

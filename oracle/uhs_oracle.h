@@ -61,20 +61,25 @@ int orc_iterate_f32(float *A, float *B, int dims, const int64_t *ext,
                     int halo, int P, const int *off, const float *coef,
                     int T, int mode, int nthreads);
 
-/* Fused two-level programs (the reference's multi-kernel stencils after
+/* Fused multi-kernel programs (the reference's multi-kernel stencils after
  * kernel fusion, UHStencil/lib/Dialect/Stencil/StencilKernelFusion.cpp:66-180:
- * each producer is cloned into its consumer at every access offset):
- *   out[p] = sum_j c_j * x_j,  x_j = in[p+o_j]            (src[j] == -1)
- *                              x_j = sum_i d_ki * in[p+o_j+e_ki]  (src[j] == k)
+ * a producer with a single consumer is cloned into it at every access
+ * offset, repeatedly, so chains fuse completely):
+ *   out[p] = sum_j c_j * x_j,  x_j = in[p+o_j]         (src[j] == -1)
+ *                              x_j = P_k(p+o_j)         (src[j] == k)
+ *   P_k(q) = sum_i d_ki * y_ki, y_ki = in[q+e_ki] or P_s(q+e_ki) (s = prod_src[i] < k)
  * each sum left to right in term order, in `mode` (ORC_PLAIN or ORC_FUSED).
  * Grids are `shape` cells per axis, halo included; cells [lo, hi) are
  * written, the rest never.  Producers: nprod of them, producer k has
- * prod_np[k] terms; their offsets and coefficients are concatenated in
- * producer order (prod_off[(sum of earlier np + i)*dims + a]).  T sweeps
+ * prod_np[k] terms; their offsets, coefficients and sources are
+ * concatenated in producer order (prod_off[(sum of earlier np + i)*dims +
+ * a]); prod_src NULL = every producer term reads the input.  T sweeps
  * ping-pong A and B as orc_iterate does.  Returns 0, or -1 on bad
- * arguments (a read outside the grid included). */
+ * arguments (a read outside the grid or a producer reading itself or a
+ * later producer included). */
 int orc_program_f64(double *A, double *B, int dims, const int64_t *shape,
                     int nprod, const int *prod_np, const int *prod_off, const double *prod_coef,
+                    const int *prod_src,
                     int np, const int *off, const double *coef, const int *src,
                     const int64_t *lo, const int64_t *hi, int T, int mode);
 

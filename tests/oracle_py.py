@@ -30,7 +30,7 @@ class Oracle:
         L.orc_fill_uniform_f32.argtypes = [_fp, I, _i64, I, ctypes.c_uint64]
         L.orc_make_coeffs.argtypes = [I, I, _ip, ctypes.c_uint64, _dp]
         L.orc_fill_rand50_f64.argtypes = [_dp, ctypes.c_size_t]
-        L.orc_program_f64.argtypes = [_dp, _dp, I, _i64, I, _ip, _ip, _dp, I, _ip, _dp, _ip, _i64, _i64, I, I]
+        L.orc_program_f64.argtypes = [_dp, _dp, I, _i64, I, _ip, _ip, _dp, _ip, I, _ip, _dp, _ip, _i64, _i64, I, I]
         L.orc_program_f64.restype = I
         L.orc_mix64.argtypes = [ctypes.c_uint64]
         L.orc_mix64.restype = ctypes.c_uint64
@@ -68,18 +68,22 @@ class Oracle:
         assert rc == 0
 
     def program(self, A, B, shape, producers, offsets, coeffs, sources, lo, hi, T, mode=FUSED):
-        """T sweeps of a fused two-level program (A, B modified in place);
-        producers = [(offsets, coeffs), ...]."""
+        """T sweeps of a fused program (A, B modified in place); producers =
+        [(offsets, coeffs) or (offsets, coeffs, sources), ...]."""
         dims = len(shape)
-        pnp = np.array([len(np.asarray(c)) for _, c in producers] or [0], np.int32)
-        poff = np.ascontiguousarray(np.concatenate([self._st(o, dims).reshape(-1) for o, _ in producers])
+        pnp = np.array([len(np.asarray(pr[1])) for pr in producers] or [0], np.int32)
+        poff = np.ascontiguousarray(np.concatenate([self._st(pr[0], dims).reshape(-1) for pr in producers])
                                     if producers else np.zeros(1, np.int32), dtype=np.int32)
-        pcoef = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float64) for _, c in producers])
+        pcoef = np.ascontiguousarray(np.concatenate([np.asarray(pr[1], np.float64) for pr in producers])
                                      if producers else np.zeros(1))
+        psrc = np.ascontiguousarray(np.concatenate(
+            [np.asarray(pr[2] if len(pr) > 2 else [-1] * len(pr[1]), np.int32) for pr in producers])
+            if producers else np.full(1, -1, np.int32), dtype=np.int32)
         off = self._st(offsets, dims)
         rc = self.L.orc_program_f64(A, B, dims, np.asarray(shape, np.int64), len(producers), pnp, poff, pcoef,
-                                    len(off), off, np.asarray(coeffs, np.float64), np.asarray(sources, np.int32),
-                                    np.asarray(lo, np.int64), np.asarray(hi, np.int64), T, mode)
+                                    psrc, len(off), off, np.asarray(coeffs, np.float64),
+                                    np.asarray(sources, np.int32), np.asarray(lo, np.int64),
+                                    np.asarray(hi, np.int64), T, mode)
         return rc
 
     def fill(self, ext, halo, seed, dtype=np.float64):

@@ -163,3 +163,35 @@ def test_one_sided_programs(gpu, orc, kernel):
     A, B = A0.copy(), B0.copy()
     assert orc.program(A, B, shape, [], offs, coefs, [-1] * 7, lo, hi, 2, FUSED) == 0
     assert np.array_equal(gA, A) and np.array_equal(gB, B)
+
+
+def chain_program(dims):
+    """A 4-deep chain: producer 0 reads the input, 1 reads 0 and the input,
+    2 reads 1 and 0, 3 reads 2; the output reads 3, 1 and the input."""
+    def o(*v):
+        return list(v) if dims == 3 else list(v[1:])
+    prods = [
+        ([o(0, 0, 1), o(0, -1, 0), o(1, 0, 0)], [0.5, 0.25, -0.125], [-1, -1, -1]),
+        ([o(0, 0, 0), o(0, 0, -1), o(-1, 1, 0)], [0.75, 0.5, 0.25], [0, -1, 0]),
+        ([o(0, 1, 0), o(0, 0, 1), o(1, 0, -1)], [-0.5, 1.5, 0.25], [1, 0, 1]),
+        ([o(0, 0, -1), o(-1, 0, 0)], [0.5, 0.5], [2, 2]),
+    ]
+    offs = [o(0, 0, 0), o(0, 1, -1), o(1, 0, 1), o(0, 0, 2)]
+    shape = [18, 20, 90] if dims == 3 else [40, 150]
+    return shape, prods, offs, [0.5, -0.25, 0.125, 1.0], [3, 1, 3, -1], [5] * dims, [n - 5 for n in shape]
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+def test_chain_program_bitwise(gpu, orc, dims, kernel):
+    u = gpu
+    shape, prods, offs, coefs, srcs, lo, hi = chain_program(dims)
+    prog = u.ProgramSpec(dims, prods, offs, coefs, srcs, lo, hi)
+    rng = np.random.default_rng(77 + dims)
+    A0, B0 = rng.random(shape), rng.random(shape)
+    log("chain program", dims, "D, kernel", kernel)
+    for T in (1, 2, 3):
+        gA, gB = run_gpu(u, prog, A0, B0, T, kernel)
+        A, B = A0.copy(), B0.copy()
+        assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, T, FUSED) == 0
+        assert np.array_equal(gA, A) and np.array_equal(gB, B), T

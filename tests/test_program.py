@@ -24,46 +24,94 @@ def slab(lo, hi, off):
     return tuple(slice(l + o, h + o) for l, h, o in zip(lo, hi, off))
 
 
+MARGIN = 12
+
+
+def materialise(inp, producers, absolute=False):
+    """Every producer written to a temporary grid, in dependency order (the
+    reference's unfused lowering).  The temporaries extend MARGIN cells past
+    the input grid on every side: a fused producer may be evaluated at a
+    cell outside the grid whose own reads are inside.  Cells whose reads
+    leave the grid hold NaN.  absolute: the same sums of |terms|."""
+    dims, M = inp.ndim, MARGIN
+    src_in = np.abs(inp) if absolute else inp
+    P, reach = [], []
+    for pr in producers:
+        offs, coefs = pr[0], pr[1]
+        srcs = pr[2] if len(pr) > 2 else [-1] * len(coefs)
+        rlo = [min(w[a] + (reach[s][0][a] if s >= 0 else 0) for w, s in zip(offs, srcs)) for a in range(dims)]
+        rhi = [max(w[a] + (reach[s][1][a] if s >= 0 else 0) for w, s in zip(offs, srcs)) for a in range(dims)]
+        reach.append((rlo, rhi))
+        # Valid cells in grid coordinates, limited to the margin.
+        lo = [max(-M, -r) for r in rlo]
+        hi = [min(n + M, n - r) for n, r in zip(inp.shape, rhi)]
+        arr = np.full([n + 2 * M for n in inp.shape], np.nan)
+        if all(h > l for l, h in zip(lo, hi)):
+            acc = None
+            for w, c, s in zip(offs, coefs, srcs):
+                if s < 0:
+                    v = src_in[slab(lo, hi, w)]
+                else:
+                    v = P[s][slab([l + M for l in lo], [h + M for h in hi], w)]
+                t = (abs(c) if absolute else c) * v
+                acc = t if acc is None else acc + t
+            arr[slab([l + M for l in lo], [h + M for h in hi], [0] * dims)] = acc
+        P.append(arr)
+    return P
+
+
 def two_pass(inp, producers, offsets, coeffs, sources, lo, hi, absolute=False):
-    """One sweep over [lo, hi): every term first materialised as a whole
-    shifted block (a producer as a temporary grid over that block)."""
+    """One sweep over [lo, hi) the unfused way: producers materialised as
+    temporary grids, then the output kernel reads them."""
+    P = materialise(inp, producers, absolute)
+    M = MARGIN
     out = None
-    for j, (v, c) in enumerate(zip(offsets, coeffs)):
-        if sources[j] < 0:
-            x = inp[slab(lo, hi, v)]
+    for v, c, s in zip(offsets, coeffs, sources):
+        if s < 0:
+            x = (np.abs(inp) if absolute else inp)[slab(lo, hi, v)]
         else:
-            po, pc = producers[sources[j]]
-            x = None
-            for w, d in zip(po, pc):
-                t = (abs(d) * np.abs(inp[slab(lo, hi, np.add(v, w))]) if absolute
-                     else d * inp[slab(lo, hi, np.add(v, w))])
-                x = t if x is None else x + t
-        t = abs(c) * np.abs(x) if absolute else c * x
+            x = P[s][slab([l + M for l in lo], [h + M for h in hi], v)]
+        t = (abs(c) if absolute else c) * x
         out = t if out is None else out + t
+    assert not np.isnan(out).any()
     return out
 
 
-def random_program(rng, dims):
+def random_program(rng, dims, chains=True):
+    """A random program: 0-4 producers; with chains, a producer's terms read
+    the input or earlier producers (offsets in [-1, 1] then, so the reach
+    stays small)."""
     nprod = int(rng.integers(0, 5))
     prods = []
-    for _ in range(nprod):
+    for k in range(nprod):
         n = int(rng.integers(1, 7))
-        prods.append((rng.integers(-2, 3, size=(n, dims)).tolist(), rng.uniform(-1, 1, n).tolist()))
+        chained = chains and k > 0 and rng.random() < 0.6
+        srcs = [int(rng.integers(-1, k)) if chained and rng.random() < 0.7 else -1 for _ in range(n)]
+        span = 2 if chained else 3
+        prods.append((rng.integers(-span + 1, span, size=(n, dims)).tolist(), rng.uniform(-1, 1, n).tolist(), srcs))
     n = int(rng.integers(1, 8))
     offs = rng.integers(-2, 3, size=(n, dims)).tolist()
     coefs = rng.uniform(-1, 1, n).tolist()
     srcs = rng.integers(-1, nprod, size=n).tolist() if nprod else [-1] * n
-    shape = [int(rng.integers(12, 20)) for _ in range(dims)] if dims == 3 else \
-        [int(rng.integers(20, 40)), int(rng.integers(20, 40))]
-    # Reach of every term, for a domain whose reads stay inside the grid.
+    shape = [int(rng.integers(14, 22)) for _ in range(dims)] if dims == 3 else \
+        [int(rng.integers(24, 44)), int(rng.integers(24, 44))]
+    # Reach of every term (through producer chains), for a domain whose
+    # reads stay inside the grid.
+    preach = []
+    for po, pc, ps in prods:
+        lo_ = [min(w[a] + (preach[s][0][a] if s >= 0 else 0) for w, s in zip(po, ps)) for a in range(dims)]
+        hi_ = [max(w[a] + (preach[s][1][a] if s >= 0 else 0) for w, s in zip(po, ps)) for a in range(dims)]
+        preach.append((lo_, hi_))
     rlo, rhi = [0] * dims, [0] * dims
     for j in range(n):
-        pts = [offs[j]] if srcs[j] < 0 else [np.add(offs[j], w).tolist() for w in prods[srcs[j]][0]]
-        for p in pts:
-            for a in range(dims):
-                rlo[a], rhi[a] = min(rlo[a], p[a]), max(rhi[a], p[a])
+        for a in range(dims):
+            s = srcs[j]
+            rlo[a] = min(rlo[a], offs[j][a] + (preach[s][0][a] if s >= 0 else 0))
+            rhi[a] = max(rhi[a], offs[j][a] + (preach[s][1][a] if s >= 0 else 0))
     lo = [-rlo[a] + int(rng.integers(0, 3)) for a in range(dims)]
     hi = [shape[a] - rhi[a] - int(rng.integers(0, 3)) for a in range(dims)]
+    if any(h <= l for l, h in zip(lo, hi)):
+        return random_program(rng, dims, chains)
     return shape, prods, offs, coefs, srcs, lo, hi
 
 
@@ -86,12 +134,27 @@ def test_oracle_plain_is_two_pass(orc, seed):
 def max_program(rng, dims):
     """The largest program the interface takes: 4 producers of 32 terms and
     an output kernel of 32 terms, 8 from each producer (offsets in [-1, 1])."""
-    prods = [(rng.integers(-1, 2, size=(32, dims)).tolist(), rng.uniform(-1, 1, 32).tolist()) for _ in range(4)]
+    prods = [(rng.integers(-1, 2, size=(32, dims)).tolist(), rng.uniform(-1, 1, 32).tolist(), [-1] * 32)
+             for _ in range(4)]
     offs = rng.integers(-1, 2, size=(32, dims)).tolist()
     coefs = rng.uniform(-1, 1, 32).tolist()
     srcs = [j % 4 for j in range(32)]
     shape = [14, 15, 70] if dims == 3 else [40, 150]
     return shape, prods, offs, coefs, srcs, [2] * dims, [n - 2 for n in shape]
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_oracle_chain_program(orc, dims):
+    from tests.test_gpu_program import chain_program
+    shape, prods, offs, coefs, srcs, lo, hi = chain_program(dims)
+    rng = np.random.default_rng(5 + dims)
+    A0, B0 = rng.random(shape), rng.random(shape)
+    A, B = A0.copy(), B0.copy()
+    assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, 1, PLAIN) == 0
+    assert np.array_equal(B[slab(lo, hi, [0] * dims)], two_pass(A0, prods, offs, coefs, srcs, lo, hi))
+    # A producer reading a later one is refused.
+    bad = [prods[0], (prods[1][0], prods[1][1], [2, -1, 0])] + prods[2:]
+    assert orc.program(A, B, shape, bad, offs, coefs, srcs, lo, hi, 1, PLAIN) == -1
 
 
 @pytest.mark.parametrize("dims", [2, 3])
@@ -189,7 +252,7 @@ def in_place_check_loop(inp, T, h2):
 
 
 def run_parsed(orc, prog, A, B, T, mode):
-    prods = [(p.offsets.tolist(), p.coeffs.tolist()) for p in prog.producers]
+    prods = prog.producer_tuples()
     return orc.program(A, B, [int(v) for v in A.shape], prods, prog.output.offsets.tolist(),
                        prog.output.coeffs.tolist(), prog.sources.tolist(), prog.lo, prog.hi, T, mode)
 
@@ -256,19 +319,44 @@ def test_single_kernel_program_matches_parse_dsl(uhs):
     assert prog.lo == [1, 1, 0] and prog.hi == [9, 12, 13]
 
 
+def test_program_reader_chains(uhs, orc):
+    """Chains fuse completely: k3 reads k2, which reads k1 and the array."""
+    chain = """stencil c(double a[20][24]) { operation(k3)
+      kernel k1 { expr { 0.5*a[y][x+1] + 0.25*a[y-1][x] } }
+      kernel k2 { expr { 0.5*k1[y][x] - 0.75*a[y][x-1] + 0.125*k1[y+1][x-1] } }
+      kernel unused { expr { a[y][x] } }
+      kernel k3 { domain([3,17][3,21]) expr { 0.5*k2[y][x-1] + 2.0*a[y][x] + k2[y-1][x+1] } } }"""
+    prog, info = uhs.parse_dsl_program(chain)
+    assert info.nproducers == 2
+    assert [info.producer_names[i].value.decode() for i in range(2)] == ["k1", "k2"]
+    assert prog.producer_sources[0].tolist() == [-1, -1]
+    assert prog.producer_sources[1].tolist() == [0, -1, 0]
+    assert prog.sources.tolist() == [1, -1, 1]
+    rng = np.random.default_rng(2)
+    A0, B0 = rng.random((20, 24)), rng.random((20, 24))
+    A, B = A0.copy(), B0.copy()
+    assert run_parsed(orc, prog, A, B, 1, PLAIN) == 0
+    assert np.array_equal(B[3:17, 3:21], two_pass(A0, prog.producer_tuples(), prog.output.offsets.tolist(),
+                                                   prog.output.coeffs.tolist(), prog.sources.tolist(), [3, 3], [17, 21]))
+
+
 def test_program_reader_errors(uhs):
-    chain = """stencil c(double a[20][20]) { operation(k3)
-      kernel k1 { expr { a[y][x+1] } }
-      kernel k2 { expr { 0.5*k1[y][x] } }
-      kernel k3 { expr { 0.5*k2[y][x-1] } } }"""
-    with pytest.raises(uhs.UhsError, match="producer"):
-        uhs.parse_dsl_program(chain)
+    forward = """stencil c(double a[20][20]) { operation(k2)
+      kernel k1 { expr { 0.5*k2[y][x] } }
+      kernel k2 { expr { 0.5*k1[y][x-1] } } }"""
+    with pytest.raises(uhs.UhsError, match="earlier kernel"):
+        uhs.parse_dsl_program(forward)
     unknown = """stencil u(double a[20][20]) { kernel k { expr { 0.5*b[y][x] } } }"""
     with pytest.raises(uhs.UhsError):
         uhs.parse_dsl_program(unknown)
     missing_op = """stencil m(double a[20][20]) { operation(nope) kernel k { expr { a[y][x] } } }"""
     with pytest.raises(uhs.UhsError):
         uhs.parse_dsl_program(missing_op)
+    five = "stencil f(double a[20][20]) { " + " ".join(
+        "kernel p%d { expr { 0.5*a[y][x] } }" % i for i in range(5)) + \
+        " kernel o { expr { " + " + ".join("p%d[y][x]" % i for i in range(5)) + " } } }"
+    with pytest.raises(uhs.UhsError, match="more than 4"):
+        uhs.parse_dsl_program(five)
 
 
 class HostBuf:
@@ -302,6 +390,11 @@ def test_program_sweep_rejects_bad_programs(uhs):
         sweep([], [[0, 0]], [1.0], [-1], [2, 2], [14, 14], T=-1)
     with pytest.raises(uhs.UhsError, match="1 to 32"):
         sweep([], [[0, 0]] * 33, [1.0] * 33, [-1] * 33, [2, 2], [14, 14])
+    with pytest.raises(uhs.UhsError, match="later producer"):
+        sweep([([[0, 1]], [1.0], [0])], [[0, 0]], [1.0], [0], [2, 2], [14, 14])
+    with pytest.raises(uhs.UhsError, match="outside the grid"):
+        # the chain's reach (1 + 1 + 1 columns) exceeds the 2 left of the domain
+        sweep([([[0, -1]], [1.0], [-1]), ([[0, -1]], [1.0], [0])], [[0, -1]], [1.0], [1], [2, 2], [14, 14])
 
 
 def test_program_kernel_choice_errors(uhs):

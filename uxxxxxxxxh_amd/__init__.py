@@ -39,7 +39,8 @@ MAX_PRODUCERS, MAX_PROGRAM_TERMS = 4, 32
 class Program(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int), ("nproducers", ctypes.c_int), ("producers", ctypes.POINTER(Stencil)),
                 ("output", Stencil), ("sources", ctypes.POINTER(ctypes.c_int)),
-                ("lo", ctypes.c_int64 * 3), ("hi", ctypes.c_int64 * 3)]
+                ("lo", ctypes.c_int64 * 3), ("hi", ctypes.c_int64 * 3),
+                ("producer_sources", ctypes.POINTER(ctypes.c_int))]
 
 
 class DslProgram(ctypes.Structure):
@@ -50,6 +51,7 @@ class DslProgram(ctypes.Structure):
                 ("producer_npoints", ctypes.c_int * MAX_PRODUCERS),
                 ("producer_offsets", (ctypes.c_int * (MAX_PROGRAM_TERMS * 3)) * MAX_PRODUCERS),
                 ("producer_coeffs", (ctypes.c_double * MAX_PROGRAM_TERMS) * MAX_PRODUCERS),
+                ("producer_sources", (ctypes.c_int * MAX_PROGRAM_TERMS) * MAX_PRODUCERS),
                 ("npoints", ctypes.c_int), ("offsets", ctypes.c_int * (MAX_PROGRAM_TERMS * 3)),
                 ("coeffs", ctypes.c_double * MAX_PROGRAM_TERMS), ("sources", ctypes.c_int * MAX_PROGRAM_TERMS)]
 
@@ -254,22 +256,37 @@ def builtin(case, coeffs=None):
 
 
 class ProgramSpec:
-    """A fused two-level program (uhs_program) kept alive for ctypes: producers
-    are (offsets, coeffs) over the input grid; the output kernel's term j reads
-    the input (sources[j] == -1) or producer sources[j]; cells [lo, hi) are
-    written (halo-inclusive coordinates, axis 0 first)."""
+    """A fused program (uhs_program) kept alive for ctypes.  producers are
+    (offsets, coeffs) over the input grid, or (offsets, coeffs, sources)
+    whose term i reads the input (-1) or an earlier producer; the output
+    kernel's term j reads the input (sources[j] == -1) or producer
+    sources[j]; cells [lo, hi) are written (halo-inclusive coordinates, axis
+    0 first)."""
 
     def __init__(self, dims, producers, offsets, coeffs, sources, lo, hi):
         self.dims = int(dims)
-        self.producers = [StencilSpec(dims, o, c) for o, c in producers]
+        self.producers = [StencilSpec(dims, pr[0], pr[1]) for pr in producers]
+        self.producer_sources = [np.asarray(pr[2] if len(pr) > 2 else [-1] * len(pr[1]), np.int32)
+                                 for pr in producers]
         self.output = StencilSpec(dims, offsets, coeffs)
         self.sources = np.ascontiguousarray(np.asarray(sources, np.int32))
         self.lo, self.hi = [int(v) for v in lo], [int(v) for v in hi]
         self._parr = (Stencil * max(1, len(self.producers)))(*[p.c for p in self.producers])
+        psrc = np.full((max(1, len(producers)), MAX_PROGRAM_TERMS), -1, np.int32)
+        for k, ps in enumerate(self.producer_sources):
+            psrc[k, :len(ps)] = ps
+        self._psrc = psrc
+        chained = any((ps >= 0).any() for ps in self.producer_sources)
         self.c = Program(self.dims, len(self.producers), self._parr, self.output.c,
                          self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                          (ctypes.c_int64 * 3)(*(self.lo + [0] * (3 - dims))),
-                         (ctypes.c_int64 * 3)(*(self.hi + [0] * (3 - dims))))
+                         (ctypes.c_int64 * 3)(*(self.hi + [0] * (3 - dims))),
+                         psrc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if chained else None)
+
+    def producer_tuples(self):
+        """[(offsets, coeffs, sources)] for the oracle."""
+        return [(p.offsets.tolist(), p.coeffs.tolist(), s.tolist())
+                for p, s in zip(self.producers, self.producer_sources)]
 
 
 PROGRAM_AUTO, PROGRAM_TILED, PROGRAM_POINTWISE = 0, 1, 2
@@ -295,7 +312,8 @@ def parse_dsl_program(text=None, path=None):
     prods = []
     for k in range(info.nproducers):
         n = info.producer_npoints[k]
-        prods.append((np.array(info.producer_offsets[k][:n * d], np.int32), np.array(info.producer_coeffs[k][:n])))
+        prods.append((np.array(info.producer_offsets[k][:n * d], np.int32), np.array(info.producer_coeffs[k][:n]),
+                      np.array(info.producer_sources[k][:n], np.int32)))
     n = info.npoints
     prog = ProgramSpec(d, prods, np.array(info.offsets[:n * d], np.int32), np.array(info.coeffs[:n]),
                        np.array(info.sources[:n], np.int32), list(info.domain_lo[:d]), list(info.domain_hi[:d]))

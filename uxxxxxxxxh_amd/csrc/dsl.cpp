@@ -21,7 +21,9 @@
 #include "uhs_host.h"
 #include "uhs_dsl.h"
 
+#include <algorithm>
 #include <cctype>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -536,18 +538,42 @@ int uhs_parse_dsl_program(const char *text, uhs_dsl_program *out)
     const KDef &ok = kernels[op];
     if ((int)ok.terms.size() > UHS_MAX_PROGRAM_TERMS)
         return uhs_set_error(UHS_E_UNSUPPORTED, "more than 32 terms in the operation kernel");
-    // Producers: the earlier kernels the operation reads, in definition order.
-    std::vector<int> prod;
-    for (int k = 0; k < op; k++)
-        for (const KTerm &t : ok.terms)
-            if (t.src == kernels[k].name) {
-                prod.push_back(k);
-                break;
+    // Producers: every kernel the operation reads, directly or through other
+    // kernels, in definition order (a kernel reads only the array and
+    // kernels defined before it, so that order is a dependency order).
+    auto index_of = [&](const std::string &name) {
+        for (int k = 0; k < (int)kernels.size(); k++)
+            if (kernels[k].name == name) return k;
+        return -1;
+    };
+    std::vector<char> used(kernels.size(), 0);
+    std::vector<int> stack{op};
+    while (!stack.empty()) {
+        const int k = stack.back();
+        stack.pop_back();
+        for (const KTerm &t : kernels[k].terms) {
+            if (t.src == arr) continue;
+            const int j = index_of(t.src);
+            if (j < 0 || j >= k)
+                return uhs_set_error(UHS_E_PARSE, ("kernel '" + kernels[k].name + "' reads '" + t.src +
+                                                   "', which is neither the array nor an earlier kernel").c_str());
+            if (!used[j]) {
+                used[j] = 1;
+                stack.push_back(j);
             }
+        }
+    }
+    std::vector<int> prod, slot(kernels.size(), -1);
+    for (int k = 0; k < op; k++)
+        if (used[k]) {
+            slot[k] = (int)prod.size();
+            prod.push_back(k);
+        }
     if ((int)prod.size() > UHS_MAX_PRODUCERS)
         return uhs_set_error(UHS_E_UNSUPPORTED, "the operation reads more than 4 kernels");
-    // Reach of the program per axis, below and above the written cell (one-
-    // sided programs read only one side).
+    // Reach per producer and of the program, per axis, below and above the
+    // written cell (one-sided programs read only one side).
+    int prlo[UHS_MAX_PRODUCERS][3], prhi[UHS_MAX_PRODUCERS][3];
     int rlo[3] = {0, 0, 0}, rhi[3] = {0, 0, 0};
     for (int i = 0; i < (int)prod.size(); i++) {
         const KDef &pk = kernels[prod[i]];
@@ -555,35 +581,30 @@ int uhs_parse_dsl_program(const char *text, uhs_dsl_program *out)
             return uhs_set_error(UHS_E_UNSUPPORTED, ("more than 32 terms in kernel '" + pk.name + "'").c_str());
         snprintf(out->producer_names[i], sizeof out->producer_names[i], "%s", pk.name.c_str());
         out->producer_npoints[i] = (int)pk.terms.size();
+        for (int a = 0; a < dims; a++) { prlo[i][a] = INT32_MAX; prhi[i][a] = INT32_MIN; }
         for (int t = 0; t < (int)pk.terms.size(); t++) {
-            if (pk.terms[t].src != arr)
-                return uhs_set_error(UHS_E_UNSUPPORTED,
-                                     ("kernel '" + pk.name + "' reads another kernel: only one level of producers is fused").c_str());
-            for (int a = 0; a < dims; a++) out->producer_offsets[i][t * dims + a] = pk.terms[t].off[a];
+            const int s = pk.terms[t].src == arr ? -1 : slot[index_of(pk.terms[t].src)];
+            out->producer_sources[i][t] = s;
             out->producer_coeffs[i][t] = pk.terms[t].c;
+            for (int a = 0; a < dims; a++) {
+                const int w = pk.terms[t].off[a];
+                out->producer_offsets[i][t * dims + a] = w;
+                prlo[i][a] = std::min(prlo[i][a], s < 0 ? w : w + prlo[s][a]);
+                prhi[i][a] = std::max(prhi[i][a], s < 0 ? w : w + prhi[s][a]);
+            }
         }
     }
     out->nproducers = (int)prod.size();
     out->npoints = (int)ok.terms.size();
     for (int t = 0; t < out->npoints; t++) {
         const KTerm &tm = ok.terms[t];
-        int src = -2;
-        if (tm.src == arr) src = -1;
-        for (int i = 0; i < (int)prod.size(); i++)
-            if (tm.src == kernels[prod[i]].name) src = i;
-        if (src == -2)
-            return uhs_set_error(UHS_E_PARSE, ("'" + tm.src + "' is neither the array nor an earlier kernel").c_str());
+        const int src = tm.src == arr ? -1 : slot[index_of(tm.src)];
         out->sources[t] = src;
         out->coeffs[t] = tm.c;
         for (int a = 0; a < dims; a++) {
             out->offsets[t * dims + a] = tm.off[a];
-            int lo = tm.off[a], hi = tm.off[a];
-            if (src >= 0)
-                for (int u = 0; u < out->producer_npoints[src]; u++) {
-                    const int w = tm.off[a] + out->producer_offsets[src][u * dims + a];
-                    lo = std::min(lo, w);
-                    hi = std::max(hi, w);
-                }
+            const int lo = src < 0 ? tm.off[a] : tm.off[a] + prlo[src][a];
+            const int hi = src < 0 ? tm.off[a] : tm.off[a] + prhi[src][a];
             rlo[a] = std::max(rlo[a], -lo);
             rhi[a] = std::max(rhi[a], hi);
         }
@@ -624,6 +645,10 @@ int uhs_dsl_program_view(const uhs_dsl_program *dp, uhs_stencil *producers, uhs_
     pg->producers = producers;
     pg->output = uhs_stencil{dp->dims, dp->npoints, dp->offsets, dp->coeffs};
     pg->sources = dp->sources;
+    bool chained = false;
+    for (int i = 0; i < dp->nproducers; i++)
+        for (int t = 0; t < dp->producer_npoints[i]; t++) chained |= dp->producer_sources[i][t] >= 0;
+    pg->producer_sources = chained ? &dp->producer_sources[0][0] : nullptr;
     for (int a = 0; a < dp->dims; a++) {
         pg->lo[a] = dp->domain_lo[a];
         pg->hi[a] = dp->domain_hi[a];

@@ -46,8 +46,35 @@ struct ProgParams {
     double ocoef[kMaxTerms];
     int poff[kMaxTerms * kMaxProd];
     double pcoef[kMaxTerms * kMaxProd];
+    signed char psrc[kMaxTerms * kMaxProd]; // producer terms: -1 input, else an earlier producer
 };
 
+// Producer k at linear cell q, its sources evaluated recursively.  D bounds
+// the depth at compile time: producer k reads only producers below k, so a
+// chain from producer kMaxProd - 1 ends at producer 0, which reads only the
+// input.
+template <int D>
+__device__ double eval_producer(const ProgParams &p, int k, long long q)
+{
+    const int b = p.pstart[k], e = p.pstart[k + 1];
+    double s = 0.0;
+    for (int t = b; t < e; t++) {
+        const long long r = q + p.poff[t];
+        double v;
+        if constexpr (D > 0) {
+            const int src = p.psrc[t];
+            v = src < 0 ? p.in[r] : eval_producer<D - 1>(p, src, r);
+        } else {
+            v = p.in[r];
+        }
+        s = t == b ? p.pcoef[t] * v : __builtin_fma(p.pcoef[t], v, s);
+    }
+    return s;
+}
+
+// DEPTH: producer nesting the launch evaluates (0: producers read only the
+// input grid).
+template <int DEPTH>
 __global__ void __launch_bounds__(256) k_program(ProgParams p)
 {
     const long long total = p.n0 * p.n1 * p.n2;
@@ -62,14 +89,7 @@ __global__ void __launch_bounds__(256) k_program(ProgParams p)
             const long long q = c + p.ooff[j];
             double v;
             const int k = p.osrc[j];
-            if (k < 0) {
-                v = p.in[q];
-            } else {
-                const int b = p.pstart[k], e = p.pstart[k + 1];
-                double s = p.pcoef[b] * p.in[q + p.poff[b]];
-                for (int t = b + 1; t < e; t++) s = __builtin_fma(p.pcoef[t], p.in[q + p.poff[t]], s);
-                v = s;
-            }
+            v = k < 0 ? p.in[q] : eval_producer<DEPTH>(p, k, q);
             acc = j == 0 ? p.ocoef[0] * v : __builtin_fma(p.ocoef[j], v, acc);
         }
         p.out[c] = acc;
@@ -115,8 +135,11 @@ struct StreamParams {
     int clo_b[kMaxProd], clo_c[kMaxProd];   // producer k is read at [clo, clo + cr]
     int cr_a[kMaxProd], cr_b[kMaxProd], cr_c[kMaxProd];
     int pstart[kMaxProd + 1];
-    int pd[kMaxTerms * kMaxProd];  // producer term: input ring distance, in-plane offset
-    int poff[kMaxTerms * kMaxProd];
+    int pbar;                      // bit k: a barrier before producer k (it reads a producer
+                                   // computed since the last barrier)
+    int pd[kMaxTerms * kMaxProd];  // producer term: distance in its source's ring,
+    int psN[kMaxTerms * kMaxProd]; // that ring's length, its first cell plus the
+    int poff[kMaxTerms * kMaxProd]; // in-plane offset, and the coefficient
     double pcoef[kMaxTerms * kMaxProd];
     int oring[kMaxTerms];          // output term: its ring's first cell, length, distance, offset
     int oN[kMaxTerms];
@@ -213,6 +236,7 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
     const int t1 = p.pstart[min(2, p.nprod)];
     const int pl0 = min(lane, max(t1 - 1, 0)), pl1 = min(t1 + lane, max(nptot - 1, 0));
     const int p_off0 = p.poff[pl0], p_off1 = p.poff[pl1];
+    const int p_N0 = p.psN[pl0], p_N1 = p.psN[pl1];
     int p_slot0 = p.pd[pl0], p_slot1 = p.pd[pl1];
     const double p_c0 = p.pcoef[pl0], p_c1 = p.pcoef[pl1];
     // Lane k: producer k's ring slot and length, and its region.
@@ -282,19 +306,21 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
         const int sin = in_slot, so = o_slot, sp0 = p_slot0, sp1 = p_slot1, sk = k_slot;
         in_slot = step(in_slot, p.NI);
         o_slot = step(o_slot, o_N);
-        p_slot0 = step(p_slot0, p.NI);
-        p_slot1 = step(p_slot1, p.NI);
+        p_slot0 = step(p_slot0, p_N0);
+        p_slot1 = step(p_slot1, p_N1);
         k_slot = step(k_slot, k_N);
         if (a < a_end) store_plane(sin, pf[ph]);
         load_plane(a + PF + nhi, pf[ph]);
         if (a >= a_end) return;
         __syncthreads();
-        // Producer k's plane a + chi_a.
+        // Producer k's plane a + chi_a, in dependency order; its terms read the
+        // input ring or an earlier producer's ring.
         const int pb0 = sp0 * p.plane + p_off0;
         const int pb1 = sp1 * p.plane + p_off1;
         for (int k = 0; k < p.nprod; k++) {
+            if ((p.pbar >> k) & 1) __syncthreads();
             const int rows = eb + lane_int(k_rows, k), cols = ec + lane_int(k_cols, k);
-            if (rows <= 0) continue;   // a producer the output kernel never reads
+            if (rows <= 0) continue;   // a producer the output kernel never needs
             double *dst = lds + lane_int(k_ring, k) + lane_int(sk, k) * p.plane;
             const int b = lane_int(k_first, k), e = lane_int(k_end, k), lk = lane_int(k_li, k);
             // This producer's table (a per-lane select once per producer).
@@ -372,7 +398,7 @@ void off3(const int *offs, int d, int t, long long v[3])
 }
 
 struct Reach {
-    long long lo[3], hi[3];
+    long long lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     bool any = false;
     void add(const long long v[3])
     {
@@ -384,11 +410,18 @@ struct Reach {
     }
 };
 
+// Source of producer k's term t: -1 the input grid, else an earlier producer.
+int psrc(const uhs_program *pg, int k, int t)
+{
+    return pg->producer_sources ? pg->producer_sources[k * kMaxTerms + t] : -1;
+}
+
 // Everything both kernels need, validated.
 struct Checked {
     long long shp[3], lo[3], hi[3], sy, sz;
     Reach in;                      // every input read, relative to the written cell
-    Reach cons[kMaxProd];          // offsets at which producer k is read
+    Reach cons[kMaxProd];          // offsets (from the written cell) at which producer k is needed
+    Reach pin[kMaxProd];           // input offsets producer k reads, relative to its own cell
 };
 
 int check_program(const int64_t *shape, const uhs_program *pg, Checked &c)
@@ -413,10 +446,26 @@ int check_program(const int64_t *shape, const uhs_program *pg, Checked &c)
     c.sy = c.shp[2];
     c.sz = c.shp[1] * c.shp[2];
     if (c.sz * c.shp[0] > (1LL << 40)) return perr(UHS_E_INVAL, "grid too large");
+    // Producers in dependency order; each one's input reach.
     for (int k = 0; k < pg->nproducers; k++) {
         const uhs_stencil &s = pg->producers[k];
         if (s.dims != d || s.npoints < 1 || s.npoints > kMaxTerms || !s.offsets || !s.coeffs)
             return perr(UHS_E_INVAL, "each producer needs 1 to 32 terms, offsets and coefficients");
+        for (int t = 0; t < s.npoints; t++) {
+            const int src = psrc(pg, k, t);
+            if (src < -1 || src >= k)
+                return perr(UHS_E_INVAL, "a producer term reads itself or a later producer");
+            long long w[3];
+            off3(s.offsets, d, t, w);
+            if (src < 0) {
+                c.pin[k].add(w);
+            } else {
+                long long lo[3], hi[3];
+                for (int a = 0; a < 3; a++) { lo[a] = w[a] + c.pin[src].lo[a]; hi[a] = w[a] + c.pin[src].hi[a]; }
+                c.pin[k].add(lo);
+                c.pin[k].add(hi);
+            }
+        }
     }
     for (int j = 0; j < o.npoints; j++) {
         const int k = pg->sources[j];
@@ -428,13 +477,10 @@ int check_program(const int64_t *shape, const uhs_program *pg, Checked &c)
             r.add(v);
         } else {
             c.cons[k].add(v);
-            const uhs_stencil &s = pg->producers[k];
-            for (int t = 0; t < s.npoints; t++) {
-                long long w[3];
-                off3(s.offsets, d, t, w);
-                for (int a = 0; a < 3; a++) w[a] += v[a];
-                r.add(w);
-            }
+            long long lo[3], hi[3];
+            for (int a = 0; a < 3; a++) { lo[a] = v[a] + c.pin[k].lo[a]; hi[a] = v[a] + c.pin[k].hi[a]; }
+            r.add(lo);
+            r.add(hi);
         }
         for (int a = 0; a < 3; a++)
             if (c.lo[a] + r.lo[a] < 0 || c.hi[a] - 1 + r.hi[a] >= c.shp[a]) {
@@ -444,6 +490,21 @@ int check_program(const int64_t *shape, const uhs_program *pg, Checked &c)
             }
         c.in.add(r.lo);
         c.in.add(r.hi);
+    }
+    // Where each producer is needed: by the output kernel, and by later
+    // producers where they are needed (consumers first).
+    for (int k = pg->nproducers - 1; k >= 0; k--) {
+        if (!c.cons[k].any) continue;
+        const uhs_stencil &s = pg->producers[k];
+        for (int t = 0; t < s.npoints; t++) {
+            const int src = psrc(pg, k, t);
+            if (src < 0) continue;
+            long long w[3], lo[3], hi[3];
+            off3(s.offsets, d, t, w);
+            for (int a = 0; a < 3; a++) { lo[a] = c.cons[k].lo[a] + w[a]; hi[a] = c.cons[k].hi[a] + w[a]; }
+            c.cons[src].add(lo);
+            c.cons[src].add(hi);
+        }
     }
     return UHS_OK;
 }
@@ -473,6 +534,7 @@ int build_pointwise(const uhs_program *pg, const Checked &c, ProgParams &p)
             off3(s.offsets, d, t, v);
             if (!lin(v, &p.poff[nterms])) return perr(UHS_E_INVAL, "term offset exceeds 2^31 cells");
             p.pcoef[nterms] = s.coeffs[t];
+            p.psrc[nterms] = (signed char)psrc(pg, k, t);
         }
     }
     p.pstart[p.nprod] = nterms;
@@ -564,6 +626,7 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
         if (!c.cons[k].any) continue;
         const uhs_stencil &s = pg->producers[k];
         for (int t = 0; t < s.npoints; t++) {
+            if (psrc(pg, k, t) >= 0) continue;
             long long w[3];
             off3(s.offsets, d, t, w);
             nlo = std::min(nlo, c.cons[k].hi[ax_a] + w[ax_a]);
@@ -609,13 +672,33 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
         for (int t = 0; t < s.npoints; t++, nterms++) {
             long long w[3];
             off3(s.offsets, d, t, w);
-            // Plane a + chi_a + w_a of the input: ring distance from the step.
-            p.pd[nterms] = r.any ? (int)(chi_a + w[ax_a] - nlo) : 0;
-            p.poff[nterms] = (int)(w[ax_b] * p.Hc + w[ax_c]);
+            // Plane a + chi_a + w_a of its source: ring distance from the step
+            // (the input ring starts at plane A + nlo_a, producer j's at
+            // A + clo_j).
+            const int src = psrc(pg, k, t);
+            if (r.any) {
+                // A needed producer's sources are needed too (their reach covers it).
+                const long long base = src < 0 ? nlo : c.cons[src].lo[ax_a];
+                p.pd[nterms] = (int)(chi_a + w[ax_a] - base);
+            } else {
+                p.pd[nterms] = 0;
+            }
+            p.psN[nterms] = src < 0 ? p.NI : p.pN[src];
+            p.poff[nterms] = (src < 0 ? 0 : p.pring[src]) + (int)(w[ax_b] * p.Hc + w[ax_c]);
             p.pcoef[nterms] = s.coeffs[t];
         }
     }
     p.pstart[p.nprod] = nterms;
+    // A barrier before a producer that reads one computed since the last one.
+    p.pbar = 0;
+    for (int k = 0, first = 0; k < p.nprod; k++) {
+        bool reads_recent = false;
+        for (int t = 0; t < pg->producers[k].npoints; t++) reads_recent |= psrc(pg, k, t) >= first;
+        if (reads_recent) {
+            p.pbar |= 1 << k;
+            first = k;
+        }
+    }
     for (int j = 0; j < o.npoints; j++) {
         long long v[3];
         off3(o.offsets, d, j, v);
@@ -748,7 +831,10 @@ extern "C" int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *sha
     for (int st = 1; st <= timesteps; st++) {
         p.in = (st & 1) ? A : B;
         p.out = (st & 1) ? B : A;
-        hipLaunchKernelGGL(k_program, dim3(blocks), dim3(256), 0, s, p);
+        if (pg->producer_sources)
+            hipLaunchKernelGGL(k_program<kMaxProd - 1>, dim3(blocks), dim3(256), 0, s, p);
+        else
+            hipLaunchKernelGGL(k_program<0>, dim3(blocks), dim3(256), 0, s, p);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return perr(UHS_E_HIP, std::string("program launch: ") + hipGetErrorString(e));
     }

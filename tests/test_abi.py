@@ -83,7 +83,8 @@ def test_argument_validation(uhs):
     many = [[a, b, c] for a in range(-3, 4) for b in range(-3, 4) for c in range(-3, 4)][::-1]
     expect_code(uhs, -2, lambda: uhs.plan_kind(ext, 3, uhs.StencilSpec(3, many, np.ones(343))))
     expect_code(uhs, -2, lambda: uhs.plan_kind(ext, 5, uhs.StencilSpec(3, [[5, 0, 0]], [1.0])))
-    assert uhs.plan_kind(ext, 2, uhs.StencilSpec(3, [[0, 0, 0], [1, 0, 0]], [0.5, 0.5])) == "gather"
+    assert uhs.plan_kind(ext, 2, uhs.StencilSpec(3, [[0, 0, 0], [1, 0, 0]], [0.5, 0.5])) == "program"
+    assert uhs.plan_kind(ext, 2, uhs.StencilSpec(3, [[0, 0, 0], [1, 0, 0]], [0.5, 0.5]), uhs.DT_F32) == "gather"
 
 
 def test_no_gpu_needed_for_planning(uhs):

@@ -51,7 +51,7 @@ def test_hand_written_program(uhs):
     assert list(info.extents) == [16, 18, 20]
     assert st.offsets.tolist() == [[0, 0, 0], [-1, 0, 0], [0, -2, 1], [2, 0, -2]]
     assert st.coeffs.tolist() == [-0.25, 0.5, -0.125, 1.0]
-    assert uhs.plan_kind([16, 18, 20], 2, st) == "gather"
+    assert uhs.plan_kind([16, 18, 20], 2, st) == "program"
 
 
 def test_float_program_and_defaults(uhs):

@@ -238,7 +238,10 @@ def test_f32_parity(gpu, orc, case, compensated):
     assert np.array_equal(gA, A) and np.array_equal(gB, B)
 
 
-def test_general_stencil_uses_gather(gpu, orc):
+def test_general_stencil_paths(gpu, orc):
+    """Stencils outside the 12 built-in forms: FP64 with at most 32 terms
+    runs the streaming program kernel, FP32 and larger stencils the general
+    one-thread-per-point kernel; all bitwise equal to the oracle."""
     u = gpu
     rng = np.random.default_rng(3)
     cases = [
@@ -248,17 +251,25 @@ def test_general_stencil_uses_gather(gpu, orc):
         (3, [[0, 0, 0], [-1, 0, 0], [0, -2, 0], [0, 0, 1], [1, 1, 1], [-2, 0, 2]]),
         # 2D upwind-like stencil
         (2, [[0, 0], [0, -1], [-1, 0], [-2, 0], [0, -3]]),
+        # 40 distinct offsets within radius 3: more terms than the program kernel takes
+        (3, [list(o) for o in sorted({tuple(int(v) for v in rng.integers(-3, 4, 3)) for _ in range(200)})[:40]]),
     ]
     for dims, offs in cases:
         c = rng.random(len(offs))
         st = u.StencilSpec(dims, offs, c)
         h = 3
         ext = [23, 31, 45] if dims == 3 else [50, 77]
-        assert u.plan_kind(ext, h, st) == "gather"
+        assert u.plan_kind(ext, h, st) == ("program" if len(offs) <= 32 else "gather")
+        assert u.plan_kind(ext, h, st, u.DT_F32) == "gather"
+        log("general stencil", dims, "D", len(offs), "terms")
         A0, B0 = random_grids(ext, h, 4)
         gA, gB = run_gpu(u, st, A0, B0, ext, h, 3)
         A, B = A0.copy(), B0.copy()
         orc.iterate(A, B, ext, h, offs, c, 3, FUSED)
+        assert np.array_equal(gA, A) and np.array_equal(gB, B)
+        gA, gB = run_gpu(u, st, A0, B0, ext, h, 2, dtype=np.float32)
+        A, B = A0.astype(np.float32), B0.astype(np.float32)
+        orc.iterate(A, B, ext, h, offs, c, 2, FUSED)
         assert np.array_equal(gA, A) and np.array_equal(gB, B)
 
 

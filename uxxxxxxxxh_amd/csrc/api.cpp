@@ -396,6 +396,34 @@ int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream)
     return UHS_OK;
 }
 
+// FP64 stencils outside the built-in forms with at most
+// UHS_MAX_PROGRAM_TERMS terms run as a one-kernel program over the interior:
+// the streaming program kernel computes the same sum in the same order
+// (first term a product, then fused multiply-adds) as the one-thread-per-
+// point general kernel, staged through LDS instead of the caches.
+static bool as_program(const Plan &pl, const uhs_stencil *st)
+{
+    return pl.family == FAM_GATHER && pl.args.dtype == DT_F64 && st->npoints <= UHS_MAX_PROGRAM_TERMS;
+}
+
+static int sweep_as_program(double *A, double *B, int dims, const int64_t *ext, int halo,
+                            const uhs_stencil *st, int timesteps, hipStream_t stream)
+{
+    int src[UHS_MAX_PROGRAM_TERMS];
+    for (int t = 0; t < st->npoints; t++) src[t] = -1;
+    uhs_program pg{};
+    pg.dims = dims;
+    pg.output = *st;
+    pg.sources = src;
+    int64_t shape[3];
+    for (int a = 0; a < dims; a++) {
+        shape[a] = ext[a] + 2 * (int64_t)halo;
+        pg.lo[a] = halo;
+        pg.hi[a] = halo + ext[a];
+    }
+    return uhs_program_sweep_ex_f64(A, B, shape, &pg, timesteps, UHS_PROGRAM_AUTO, nullptr, stream);
+}
+
 extern "C" {
 
 int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
@@ -404,6 +432,12 @@ int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
     Plan pl;
     int rc = uhs_make_plan(dims, ext, halo, st, DT_F64, cfg, &pl);
     if (rc) return rc;
+    if (as_program(pl, st)) {
+        if (!A || !B) return fail(UHS_E_INVAL, "null grid pointer");
+        if (A == B) return fail(UHS_E_INVAL, "A and B must be distinct grids");
+        if (timesteps < 0) return fail(UHS_E_INVAL, "negative timestep count");
+        return sweep_as_program(A, B, dims, ext, halo, st, timesteps, (hipStream_t)stream);
+    }
     return uhs_run_plan(pl, A, B, timesteps, (hipStream_t)stream);
 }
 
@@ -423,7 +457,7 @@ int uhs_plan_kind(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
     Plan pl;
     int rc = uhs_make_plan(dims, ext, halo, st, dtype, cfg, &pl);
     if (rc) return rc;
-    if (buf && buflen) snprintf(buf, buflen, "%s", family_name(pl.family));
+    if (buf && buflen) snprintf(buf, buflen, "%s", as_program(pl, st) ? "program" : family_name(pl.family));
     return UHS_OK;
 }
 

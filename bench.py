@@ -192,8 +192,8 @@ NESTED_PROGRAMS = {
 
 
 def programs(u, reps=3, names=None):
-    """Fused multi-kernel programs: the tiled kernel (the library's choice)
-    and the pointwise kernel, GStencils/s over the written domain, against
+    """Fused multi-kernel programs: the tiled kernel (the library's default
+    tile, and the tile the program tuner picks) and the pointwise kernel, GStencils/s over the written domain, against
     16 algorithmic bytes per written cell and the FP64 rate of the fused
     formula (each producer re-evaluated at every access)."""
     out = {}
@@ -211,23 +211,29 @@ def programs(u, reps=3, names=None):
         roof = min(PEAK_HBM_GBPS / 16.0, PEAK_FP64_TFLOPS * 1e3 / flops)
         r = {"shape": d["shape"], "domain": [d["lo"], d["hi"]], "timesteps": d["T"],
              "flops_per_cell": flops, "roof_GStencils_s": round(roof, 1)}
-        for label, kernel in (("tiled", u.PROGRAM_AUTO), ("pointwise", u.PROGRAM_POINTWISE)):
-            u.program_sweep_f64(A, B, d["shape"], prog, 2, kernel=kernel)
+        t0 = time.time()
+        tuned_tile, _, ntiles = u.tune_program_f64(A, B, d["shape"], prog, reps=3)
+        t_tune = time.time() - t0
+        for label, kernel, tile in (("tiled", u.PROGRAM_AUTO, None), ("tuned", u.PROGRAM_TILED, tuned_tile),
+                                    ("pointwise", u.PROGRAM_POINTWISE, None)):
+            u.program_sweep_f64(A, B, d["shape"], prog, 2, kernel=kernel, tile=tile)
             u.synchronize()
             tm = u.Timer()
             tm.start()
             for _ in range(reps):
-                u.program_sweep_f64(A, B, d["shape"], prog, d["T"], kernel=kernel)
+                u.program_sweep_f64(A, B, d["shape"], prog, d["T"], kernel=kernel, tile=tile)
             tm.stop()
             ms = tm.ms() / (reps * d["T"])
             g = cells / (ms * 1e-3) / 1e9
             r[label] = {"ms_per_sweep": round(ms, 4), "GStencils_s": round(g, 2),
                         "hbm_frac_at_16B": round(16 * g / PEAK_HBM_GBPS, 4), "roof_frac": round(g / roof, 4)}
+        r["tuned"].update({"tile": tuned_tile, "tiles_timed": ntiles, "tuning_s": round(t_tune, 3)})
         A.close()
         B.close()
         out[name] = r
-        log("program %-16s tiled %.1f GSt/s, pointwise %.1f GSt/s (roof %.0f)"
-            % (name, r["tiled"]["GStencils_s"], r["pointwise"]["GStencils_s"], roof))
+        log("program %-16s tiled %.1f GSt/s, tuned %.1f GSt/s %s, pointwise %.1f GSt/s (roof %.0f)"
+            % (name, r["tiled"]["GStencils_s"], r["tuned"]["GStencils_s"], tuned_tile,
+               r["pointwise"]["GStencils_s"], roof))
     return out
 
 

@@ -275,13 +275,12 @@ int uhs_program_sweep_f64(double *A, double *B, const int64_t *shape, const uhs_
 #define UHS_PROGRAM_TILED 1     /* a block of the input staged in LDS once, each producer
                                    evaluated once per cell of the block */
 #define UHS_PROGRAM_POINTWISE 2 /* one thread per cell, producers re-evaluated per access */
-/* tile (tiled kernel): {tile_x, tile_y, tile_z, planes_per_step}.  3D: a
- * workgroup owns tile_x x tile_y cells of axes 2 and 1 and walks a slab of
- * tile_z planes of axis 0; 2D: tile_x columns and a slab of tile_y rows
- * (tile_z 1).  planes_per_step (0 = chosen) planes are staged and written
- * between two barriers.  NULL = chosen from the program's reach.  The LDS
- * rings (staged planes of the tile plus reach, for the input and each
- * producer) must fit in 64 KiB (UHS_E_CONFIG otherwise). */
+/* tile (tiled kernel): {tile_x, tile_y, tile_z}.  3D: a workgroup owns
+ * tile_x x tile_y cells of axes 2 and 1 and walks a slab of tile_z planes of
+ * axis 0; 2D: tile_x columns and a slab of tile_y rows (tile_z 1).  NULL =
+ * chosen from the program's reach (uhs_tune_program_f64 in uhs_tune.h times
+ * the candidates).  The LDS rings (staged planes of the tile plus reach, for
+ * the input and each producer) must fit in 64 KiB (UHS_E_CONFIG otherwise). */
 int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
                              int timesteps, int kernel, const int *tile, uhs_stream_t stream);
 

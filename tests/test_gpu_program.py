@@ -195,3 +195,36 @@ def test_chain_program_bitwise(gpu, orc, dims, kernel):
         A, B = A0.copy(), B0.copy()
         assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, T, FUSED) == 0
         assert np.array_equal(gA, A) and np.array_equal(gB, B), T
+
+
+def test_tuned_tile_bitwise(gpu, orc):
+    """The program tuner picks a streaming tile that fits; that tile's
+    result is the oracle's."""
+    u = gpu
+    prog, info = u.parse_dsl_program(NESTED_2D)
+    rng = np.random.default_rng(13)
+    A0 = rng.random((98, 98))
+    A = u.DeviceArray.from_host(A0)
+    B = u.DeviceArray.from_host(A0)
+    try:
+        tile, ms, n = u.tune_program_f64(A, B, [98, 98], prog, reps=2)
+    finally:
+        A.close()
+        B.close()
+    log("tuned tile", tile, ms, n)
+    assert n > 0 and ms > 0 and tile[2] == 1
+    gA, gB = run_gpu(u, prog, A0, A0.copy(), 4, 1, tile)
+    A, B = A0.copy(), A0.copy()
+    assert run_parsed(orc, prog, A, B, 4, FUSED) == 0
+    assert np.array_equal(gA, A) and np.array_equal(gB, B)
+    # Nothing fits: far-reaching terms leave only the pointwise kernel.
+    far = u.ProgramSpec(3, [], [[-40, 0, 0], [40, 0, 0], [0, -8, 0], [0, 8, 0]], [0.5, 0.5, 0.1, 0.1],
+                        [-1] * 4, [41, 9, 1], [49, 11, 69])
+    A = u.DeviceArray([90, 20, 70], np.float64)
+    B = u.DeviceArray([90, 20, 70], np.float64)
+    try:
+        with pytest.raises(u.UhsError, match="no tile"):
+            u.tune_program_f64(A, B, [90, 20, 70], far)
+    finally:
+        A.close()
+        B.close()

@@ -397,6 +397,18 @@ def test_program_sweep_rejects_bad_programs(uhs):
         sweep([([[0, -1]], [1.0], [-1]), ([[0, -1]], [1.0], [0])], [[0, -1]], [1.0], [1], [2, 2], [14, 14])
 
 
+def test_program_tuner_argument_errors(uhs):
+    import ctypes
+    A, B = HostBuf((20, 20)), HostBuf((20, 20))
+    prog = uhs.ProgramSpec(2, [], [[0, 1]], [1.0], [-1], [1, 1], [19, 19])
+    tile = (ctypes.c_int * 3)()
+    L = uhs.lib()
+    assert L.uhs_tune_program_f64(A.ptr, B.ptr, uhs.ext_array([20, 20]), ctypes.byref(prog.c), 0, tile,
+                                  None, None, None) == -1
+    assert L.uhs_tune_program_f64(A.ptr, B.ptr, uhs.ext_array([20, 20]), ctypes.byref(prog.c), 2, None,
+                                  None, None, None) == -1
+
+
 def test_program_kernel_choice_errors(uhs):
     """Tile and kernel checks happen before any launch."""
     A, B = HostBuf((90, 20, 70)), HostBuf((90, 20, 70))

@@ -119,6 +119,8 @@ _SIGS = {
     "uhs_parse_dsl_program_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslProgram)]),
     "uhs_program_sweep_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _P]),
     "uhs_dsl_program_view": (_I, [ctypes.POINTER(DslProgram), ctypes.POINTER(Stencil), ctypes.POINTER(Program)]),
+    "uhs_tune_program_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), _P]),
     "uhs_program_sweep_ex_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _I, ctypes.POINTER(ctypes.c_int), _P]),
     "uhs_parse_dsl_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
     "uhs_batch_create": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(BatchConfig), ctypes.POINTER(_P)]),
@@ -296,9 +298,20 @@ def program_sweep_f64(A, B, shape, prog, timesteps, stream=None, kernel=PROGRAM_
     if kernel == PROGRAM_AUTO and tile is None:
         check(lib().uhs_program_sweep_f64(A.ptr, B.ptr, ext_array(shape), ctypes.byref(prog.c), timesteps, stream))
         return
-    t = (ctypes.c_int * 4)(*(list(tile) + [0] * (4 - len(tile)))) if tile is not None else None
+    t = (ctypes.c_int * 3)(*tile) if tile is not None else None
     check(lib().uhs_program_sweep_ex_f64(A.ptr, B.ptr, ext_array(shape), ctypes.byref(prog.c), timesteps,
                                          kernel, t, stream))
+
+
+def tune_program_f64(A, B, shape, prog, reps=3, stream=None):
+    """Times the streaming kernel's candidate tiles on A, B (overwritten):
+    (best tile, ms per sweep, tiles timed)."""
+    tile = (ctypes.c_int * 3)()
+    ms = ctypes.c_double()
+    n = ctypes.c_int()
+    check(lib().uhs_tune_program_f64(A.ptr, B.ptr, ext_array(shape), ctypes.byref(prog.c), reps, tile,
+                                     ctypes.byref(ms), ctypes.byref(n), stream))
+    return list(tile), ms.value, n.value
 
 
 def parse_dsl_program(text=None, path=None):

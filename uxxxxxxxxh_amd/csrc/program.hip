@@ -19,9 +19,12 @@
 // The term tables travel in the kernel arguments (uniform: scalar loads).
 #include "uhs.h"
 #include "uhs_host.h"
+#include "uhs_tune.h"
 #include "uhs_internal.h"
 
 #include <algorithm>
+#include <array>
+#include <vector>
 #include <climits>
 #include <cstdint>
 #include <cstdio>
@@ -223,9 +226,10 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
     };
 
     // Term tables in lanes: lane l holds output term l, producer term l of
-    // each of two tables, and producer l's ring slot and region.  Each step advances the slots by
-    // one with one vector operation per table; the term loops read them with
-    // readlane (no scalar arithmetic or scalar loads per term).
+    // each of two tables, and producer l's ring slot and region.  Each step
+    // advances the slots by one with one vector operation per table; the
+    // term loops read them with readlane (no scalar arithmetic or scalar
+    // loads per term).
     const int ol = min(lane, p.nout - 1);
     const int o_N = p.oN[ol], o_fix = p.oring[ol] + p.ooff[ol];
     int o_slot = p.od[ol];
@@ -845,4 +849,63 @@ extern "C" int uhs_program_sweep_f64(double *A, double *B, const int64_t *shape,
                                      int timesteps, uhs_stream_t stream)
 {
     return uhs_program_sweep_ex_f64(A, B, shape, pg, timesteps, UHS_PROGRAM_AUTO, nullptr, stream);
+}
+
+extern "C" int uhs_tune_program_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
+                                    int reps, int *tile_out, double *best_ms, int *evaluated,
+                                    uhs_stream_t stream)
+{
+    if (!A || !B || !shape || !pg || !tile_out) return perr(UHS_E_INVAL, "null argument");
+    if (reps < 1) return perr(UHS_E_INVAL, "reps must be at least 1");
+    Checked c;
+    int rc = check_program(shape, pg, c);
+    if (rc) return rc;
+    // Candidates: rows of 64 << n cells once the reach is added, a range of
+    // tile heights and slab lengths; only those whose rings fit are timed.
+    const long long rc2 = c.in.hi[2] - c.in.lo[2];
+    std::vector<std::array<int, 3>> cand;
+    if (pg->dims == 2) {
+        for (int w : {256, 512, 1024})
+            if (w - rc2 >= 32)
+                for (int ta : {8, 16, 32, 64}) cand.push_back({(int)(w - rc2), ta, 1});
+    } else {
+        for (int w : {64, 128, 256})
+            if (w - rc2 >= 16)
+                for (int tb : {1, 2, 4, 6, 8, 12})
+                    for (int ta : {8, 16, 32}) cand.push_back({(int)(w - rc2), tb, ta});
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return perr(UHS_E_HIP, "event creation failed");
+    double best = 0.0;
+    int n = 0, found = 0;
+    for (const auto &t : cand) {
+        StreamParams sp;
+        int us, uc;
+        if (build_stream(pg, c, t.data(), sp, &us, &uc) > kMaxLdsBytes) continue;
+        rc = uhs_program_sweep_ex_f64(A, B, shape, pg, 1, UHS_PROGRAM_TILED, t.data(), stream);
+        if (rc) break;
+        hipEventRecord(e0, s);
+        rc = uhs_program_sweep_ex_f64(A, B, shape, pg, reps, UHS_PROGRAM_TILED, t.data(), stream);
+        if (rc) break;
+        hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = perr(UHS_E_HIP, "event wait failed"); break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        const double per = ms / reps;
+        n++;
+        if (!found || per < best) {
+            best = per;
+            for (int a = 0; a < 3; a++) tile_out[a] = t[a];
+            found = 1;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    if (evaluated) *evaluated = n;
+    if (!found) return perr(UHS_E_CONFIG, "no tile of the streaming kernel fits this program in the LDS");
+    if (best_ms) *best_ms = best;
+    return UHS_OK;
 }

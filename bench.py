@@ -228,12 +228,30 @@ def programs(u, reps=3, names=None):
             r[label] = {"ms_per_sweep": round(ms, 4), "GStencils_s": round(g, 2),
                         "hbm_frac_at_16B": round(16 * g / PEAK_HBM_GBPS, 4), "roof_frac": round(g / roof, 4)}
         r["tuned"].update({"tile": tuned_tile, "tiles_timed": ntiles, "tuning_s": round(t_tune, 3)})
+        # FP32 storage, Kahan-compensated sums (the DSL's float mode), default tile: 8 B per cell.
+        A32 = u.DeviceArray(d["shape"], np.float32)
+        B32 = u.DeviceArray(d["shape"], np.float32)
+        u.fill_uniform(A32, ext, 1, 5)
+        u.fill_uniform(B32, ext, 1, 6)
+        u.program_sweep_f32(A32, B32, d["shape"], prog, 2, compensated=True)
+        u.synchronize()
+        tm = u.Timer()
+        tm.start()
+        for _ in range(reps):
+            u.program_sweep_f32(A32, B32, d["shape"], prog, d["T"], compensated=True)
+        tm.stop()
+        ms = tm.ms() / (reps * d["T"])
+        g = cells / (ms * 1e-3) / 1e9
+        r["f32_kahan"] = {"ms_per_sweep": round(ms, 4), "GStencils_s": round(g, 2),
+                          "hbm_frac_at_8B": round(8 * g / PEAK_HBM_GBPS, 4)}
+        A32.close()
+        B32.close()
         A.close()
         B.close()
         out[name] = r
-        log("program %-16s tiled %.1f GSt/s, tuned %.1f GSt/s %s, pointwise %.1f GSt/s (roof %.0f)"
-            % (name, r["tiled"]["GStencils_s"], r["tuned"]["GStencils_s"], tuned_tile,
-               r["pointwise"]["GStencils_s"], roof))
+        log("program %-16s tiled %.1f GSt/s, tuned %.1f GSt/s %s, pointwise %.1f GSt/s, f32 kahan %.1f GSt/s "
+            "(roof %.0f)" % (name, r["tiled"]["GStencils_s"], r["tuned"]["GStencils_s"], tuned_tile,
+                              r["pointwise"]["GStencils_s"], r["f32_kahan"]["GStencils_s"], roof))
     return out
 
 

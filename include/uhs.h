@@ -284,6 +284,15 @@ int uhs_program_sweep_f64(double *A, double *B, const int64_t *shape, const uhs_
 int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
                              int timesteps, int kernel, const int *tile, uhs_stream_t stream);
 
+/* FP32 storage and arithmetic (the reference DSL's `float` mode), as
+ * uhs_sweep_f32: coefficients rounded to float; compensated = 1 sums every
+ * kernel's terms, each producer's included, with Kahan compensation. */
+int uhs_program_sweep_f32(float *A, float *B, const int64_t *shape, const uhs_program *pg,
+                          int timesteps, int compensated, uhs_stream_t stream);
+int uhs_program_sweep_ex_f32(float *A, float *B, const int64_t *shape, const uhs_program *pg,
+                             int timesteps, int compensated, int kernel, const int *tile,
+                             uhs_stream_t stream);
+
 /* ---- plumbing: device memory, streams, timing ---------------------------- */
 
 /* Streaming device-to-device copy of n doubles, one element per thread: the

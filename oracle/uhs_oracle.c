@@ -206,112 +206,143 @@ void orc_fill_rand50_f64(double *a, size_t n)
 
 /* ---- fused multi-kernel programs ---------------------------------------- */
 
-struct prog_ctx {
-    const double *in;
-    const int *pstart, *psrc;      /* producer k's terms [pstart[k], pstart[k+1]); psrc NULL: all input */
-    const long long *poff;
-    const double *pcoef;
-    int mode;
-};
+/* Left-to-right sum step in `mode`: s += c*v (PLAIN: separate product and
+ * add; FUSED: one fused multiply-add; KAHAN: the compensated form of the
+ * FP32 sweep, e carrying the compensation). */
+#define PROG_ADD(FMA, mode, s, e, c, v)                                        \
+    do {                                                                       \
+        if ((mode) == ORC_PLAIN) (s) = (s) + (c) * (v);                        \
+        else if ((mode) == ORC_FUSED) (s) = FMA((c), (v), (s));                \
+        else {                                                                 \
+            const __typeof__(s) y_ = FMA((c), (v), -(e));                      \
+            const __typeof__(s) u_ = (s) + y_;                                 \
+            (e) = (u_ - (s)) - y_;                                             \
+            (s) = u_;                                                          \
+        }                                                                      \
+    } while (0)
 
 /* The value of producer k at linear cell q: its terms left to right, each
  * reading the input grid or an earlier producer evaluated at the shifted
  * cell (the reference's inlining at every access offset). */
-static double prog_value(const struct prog_ctx *c, int k, long long q)
-{
-    double s = 0.0;
-    for (int t = c->pstart[k]; t < c->pstart[k + 1]; t++) {
-        const int src = c->psrc ? c->psrc[t] : -1;
-        const double v = src < 0 ? c->in[q + c->poff[t]] : prog_value(c, src, q + c->poff[t]);
-        if (t == c->pstart[k]) s = c->pcoef[t] * v;
-        else s = c->mode == ORC_FUSED ? fma(c->pcoef[t], v, s) : s + c->pcoef[t] * v;
+#define PROG_VALUE_FN(T, FMA, NAME)                                            \
+    struct NAME##_ctx {                                                        \
+        const T *in;                                                           \
+        const int *pstart, *psrc;                                              \
+        const long long *poff;                                                 \
+        const T *pcoef;                                                        \
+        int mode;                                                              \
+    };                                                                         \
+    static T NAME(const struct NAME##_ctx *c, int k, long long q)              \
+    {                                                                          \
+        T s = 0, e = 0;                                                        \
+        for (int t = c->pstart[k]; t < c->pstart[k + 1]; t++) {                \
+            const int src = c->psrc ? c->psrc[t] : -1;                         \
+            const T v = src < 0 ? c->in[q + c->poff[t]] : NAME(c, src, q + c->poff[t]); \
+            if (t == c->pstart[k]) s = c->pcoef[t] * v;                        \
+            else PROG_ADD(FMA, c->mode, s, e, c->pcoef[t], v);                 \
+        }                                                                      \
+        return s;                                                              \
     }
-    return s;
-}
+
+PROG_VALUE_FN(double, fma, prog_value_f64)
+PROG_VALUE_FN(float, fmaf, prog_value_f32)
+
+#define PROG_BODY(T, FMA, NAME)                                                \
+    if (!A || !B || !shape || (dims != 2 && dims != 3) || np < 1 || !off || !coef || !src || \
+        nprod < 0 || (nprod && (!prod_np || !prod_off || !prod_coef)) || !lo || !hi || T_ < 0) \
+        return -1;                                                             \
+    /* 3D view: a 2D grid is one plane. */                                     \
+    long long shp[3] = {1, 1, 1}, l[3] = {0, 0, 0}, h[3] = {1, 1, 1};          \
+    for (int k = 0; k < dims; k++) {                                           \
+        shp[3 - dims + k] = shape[k];                                          \
+        l[3 - dims + k] = lo[k];                                               \
+        h[3 - dims + k] = hi[k];                                               \
+        if (lo[k] < 0 || hi[k] > shape[k] || lo[k] >= hi[k]) return -1;       \
+    }                                                                          \
+    const long long sy = shp[2], sz = shp[1] * shp[2];                         \
+    int ntot = 0;                                                              \
+    for (int k = 0; k < nprod; k++) {                                          \
+        if (prod_np[k] < 1) return -1;                                         \
+        ntot += prod_np[k];                                                    \
+    }                                                                          \
+    long long *poff = (long long *)malloc(sizeof(long long) * (ntot + 1));     \
+    long long *ooff = (long long *)malloc(sizeof(long long) * np);             \
+    int *pstart = (int *)malloc(sizeof(int) * (nprod + 1));                    \
+    long long *reach = (long long *)malloc(sizeof(long long) * 6 * (nprod + 1)); \
+    if (!poff || !ooff || !pstart || !reach) {                                 \
+        free(poff); free(ooff); free(pstart); free(reach);                     \
+        return -1;                                                             \
+    }                                                                          \
+    int rc = 0, t = 0;                                                         \
+    /* reach[k]: input offsets producer k reads, per axis [lo, hi]; a          \
+     * producer may read only earlier producers. */                            \
+    for (int k = 0; k < nprod && !rc; k++) {                                   \
+        pstart[k] = t;                                                         \
+        long long *r = reach + 6 * k;                                          \
+        for (int a = 0; a < 3; a++) { r[a] = LLONG_MAX; r[3 + a] = LLONG_MIN; } \
+        for (int i = 0; i < prod_np[k]; i++, t++) {                            \
+            long long v[3] = {0, 0, 0};                                        \
+            for (int a = 0; a < dims; a++) v[3 - dims + a] = prod_off[t * dims + a]; \
+            poff[t] = v[0] * sz + v[1] * sy + v[2];                            \
+            const int s = prod_src ? prod_src[t] : -1;                         \
+            if (s < -1 || s >= k) { rc = -1; break; }                          \
+            for (int a = 0; a < 3; a++) {                                      \
+                const long long rl = s < 0 ? v[a] : v[a] + reach[6 * s + a];   \
+                const long long rh = s < 0 ? v[a] : v[a] + reach[6 * s + 3 + a]; \
+                if (rl < r[a]) r[a] = rl;                                      \
+                if (rh > r[3 + a]) r[3 + a] = rh;                              \
+            }                                                                  \
+        }                                                                      \
+    }                                                                          \
+    pstart[nprod] = t;                                                         \
+    /* Reads must stay inside the grid for every written cell. */              \
+    for (int j = 0; j < np && !rc; j++) {                                      \
+        if (src[j] < -1 || src[j] >= nprod) { rc = -1; break; }                \
+        long long v[3] = {0, 0, 0};                                            \
+        for (int a = 0; a < dims; a++) v[3 - dims + a] = off[j * dims + a];    \
+        ooff[j] = v[0] * sz + v[1] * sy + v[2];                                \
+        for (int a = 0; a < 3; a++) {                                          \
+            const long long rl = src[j] < 0 ? v[a] : v[a] + reach[6 * src[j] + a]; \
+            const long long rh = src[j] < 0 ? v[a] : v[a] + reach[6 * src[j] + 3 + a]; \
+            if (l[a] + rl < 0 || h[a] - 1 + rh >= shp[a]) rc = -1;             \
+        }                                                                      \
+    }                                                                          \
+    for (int step = 1; step <= T_ && !rc; step++) {                            \
+        const T *in = (step & 1) ? A : B;                                      \
+        T *out = (step & 1) ? B : A;                                           \
+        struct NAME##_ctx c = {in, pstart, prod_src, poff, prod_coef, mode};   \
+        for (long long z = l[0]; z < h[0]; z++)                                \
+            for (long long y = l[1]; y < h[1]; y++)                            \
+                for (long long x = l[2]; x < h[2]; x++) {                      \
+                    const long long q = z * sz + y * sy + x;                   \
+                    T acc = 0, e = 0;                                          \
+                    for (int j = 0; j < np; j++) {                             \
+                        const T xv = src[j] < 0 ? in[q + ooff[j]] : NAME(&c, src[j], q + ooff[j]); \
+                        if (j == 0) acc = coef[0] * xv;                        \
+                        else PROG_ADD(FMA, mode, acc, e, coef[j], xv);         \
+                    }                                                          \
+                    out[q] = acc;                                              \
+                }                                                              \
+    }                                                                          \
+    free(poff); free(ooff); free(pstart); free(reach);                         \
+    return rc;
 
 int orc_program_f64(double *A, double *B, int dims, const int64_t *shape,
                     int nprod, const int *prod_np, const int *prod_off, const double *prod_coef,
                     const int *prod_src,
                     int np, const int *off, const double *coef, const int *src,
-                    const int64_t *lo, const int64_t *hi, int T, int mode)
+                    const int64_t *lo, const int64_t *hi, int T_, int mode)
 {
-    if (!A || !B || !shape || (dims != 2 && dims != 3) || np < 1 || !off || !coef || !src ||
-        nprod < 0 || (nprod && (!prod_np || !prod_off || !prod_coef)) || !lo || !hi || T < 0 ||
-        (mode != ORC_PLAIN && mode != ORC_FUSED))
-        return -1;
-    /* 3D view: a 2D grid is one plane. */
-    long long shp[3] = {1, 1, 1}, l[3] = {0, 0, 0}, h[3] = {1, 1, 1};
-    for (int k = 0; k < dims; k++) {
-        shp[3 - dims + k] = shape[k];
-        l[3 - dims + k] = lo[k];
-        h[3 - dims + k] = hi[k];
-        if (lo[k] < 0 || hi[k] > shape[k] || lo[k] >= hi[k]) return -1;
-    }
-    const long long sy = shp[2], sz = shp[1] * shp[2];
-    int ntot = 0;
-    for (int k = 0; k < nprod; k++) {
-        if (prod_np[k] < 1) return -1;
-        ntot += prod_np[k];
-    }
-    long long *poff = (long long *)malloc(sizeof(long long) * (ntot + 1));
-    long long *ooff = (long long *)malloc(sizeof(long long) * np);
-    int *pstart = (int *)malloc(sizeof(int) * (nprod + 1));
-    long long *reach = (long long *)malloc(sizeof(long long) * 6 * (nprod + 1));
-    if (!poff || !ooff || !pstart || !reach) {
-        free(poff); free(ooff); free(pstart); free(reach);
-        return -1;
-    }
-    int rc = 0, t = 0;
-    /* reach[k]: input offsets producer k reads, per axis [lo, hi]; a
-     * producer may read only earlier producers. */
-    for (int k = 0; k < nprod && !rc; k++) {
-        pstart[k] = t;
-        long long *r = reach + 6 * k;
-        for (int a = 0; a < 3; a++) { r[a] = LLONG_MAX; r[3 + a] = LLONG_MIN; }
-        for (int i = 0; i < prod_np[k]; i++, t++) {
-            long long v[3] = {0, 0, 0};
-            for (int a = 0; a < dims; a++) v[3 - dims + a] = prod_off[t * dims + a];
-            poff[t] = v[0] * sz + v[1] * sy + v[2];
-            const int s = prod_src ? prod_src[t] : -1;
-            if (s < -1 || s >= k) { rc = -1; break; }
-            for (int a = 0; a < 3; a++) {
-                const long long rl = s < 0 ? v[a] : v[a] + reach[6 * s + a];
-                const long long rh = s < 0 ? v[a] : v[a] + reach[6 * s + 3 + a];
-                if (rl < r[a]) r[a] = rl;
-                if (rh > r[3 + a]) r[3 + a] = rh;
-            }
-        }
-    }
-    pstart[nprod] = t;
-    /* Reads must stay inside the grid for every written cell. */
-    for (int j = 0; j < np && !rc; j++) {
-        if (src[j] < -1 || src[j] >= nprod) { rc = -1; break; }
-        long long v[3] = {0, 0, 0};
-        for (int a = 0; a < dims; a++) v[3 - dims + a] = off[j * dims + a];
-        ooff[j] = v[0] * sz + v[1] * sy + v[2];
-        for (int a = 0; a < 3; a++) {
-            const long long rl = src[j] < 0 ? v[a] : v[a] + reach[6 * src[j] + a];
-            const long long rh = src[j] < 0 ? v[a] : v[a] + reach[6 * src[j] + 3 + a];
-            if (l[a] + rl < 0 || h[a] - 1 + rh >= shp[a]) rc = -1;
-        }
-    }
-    for (int step = 1; step <= T && !rc; step++) {
-        const double *in = (step & 1) ? A : B;
-        double *out = (step & 1) ? B : A;
-        struct prog_ctx c = {in, pstart, prod_src, poff, prod_coef, mode};
-        for (long long z = l[0]; z < h[0]; z++)
-            for (long long y = l[1]; y < h[1]; y++)
-                for (long long x = l[2]; x < h[2]; x++) {
-                    const long long q = z * sz + y * sy + x;
-                    double acc = 0.0;
-                    for (int j = 0; j < np; j++) {
-                        const double xv = src[j] < 0 ? in[q + ooff[j]] : prog_value(&c, src[j], q + ooff[j]);
-                        if (j == 0) acc = coef[0] * xv;
-                        else acc = mode == ORC_FUSED ? fma(coef[j], xv, acc) : acc + coef[j] * xv;
-                    }
-                    out[q] = acc;
-                }
-    }
-    free(poff); free(ooff); free(pstart); free(reach);
-    return rc;
+    if (mode != ORC_PLAIN && mode != ORC_FUSED) return -1;
+    PROG_BODY(double, fma, prog_value_f64)
+}
+
+int orc_program_f32(float *A, float *B, int dims, const int64_t *shape,
+                    int nprod, const int *prod_np, const int *prod_off, const float *prod_coef,
+                    const int *prod_src,
+                    int np, const int *off, const float *coef, const int *src,
+                    const int64_t *lo, const int64_t *hi, int T_, int mode)
+{
+    if (mode != ORC_PLAIN && mode != ORC_FUSED && mode != ORC_KAHAN) return -1;
+    PROG_BODY(float, fmaf, prog_value_f32)
 }

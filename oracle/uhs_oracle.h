@@ -82,6 +82,14 @@ int orc_program_f64(double *A, double *B, int dims, const int64_t *shape,
                     const int *prod_src,
                     int np, const int *off, const double *coef, const int *src,
                     const int64_t *lo, const int64_t *hi, int T, int mode);
+/* The same in FP32 storage and arithmetic (coefficients already rounded to
+ * float), mode ORC_PLAIN, ORC_FUSED or ORC_KAHAN: every kernel's sum, each
+ * producer's included, compensated as in orc_sweep_f32. */
+int orc_program_f32(float *A, float *B, int dims, const int64_t *shape,
+                    int nprod, const int *prod_np, const int *prod_off, const float *prod_coef,
+                    const int *prod_src,
+                    int np, const int *off, const float *coef, const int *src,
+                    const int64_t *lo, const int64_t *hi, int T, int mode);
 
 /* Seeded synthetic inputs.  Interior cell with halo-inclusive linear index L
  * gets u01(mix64(seed + (L+1)*0x9E3779B97F4A7C15)), halo cells get 0. */

@@ -32,6 +32,8 @@ class Oracle:
         L.orc_fill_rand50_f64.argtypes = [_dp, ctypes.c_size_t]
         L.orc_program_f64.argtypes = [_dp, _dp, I, _i64, I, _ip, _ip, _dp, _ip, I, _ip, _dp, _ip, _i64, _i64, I, I]
         L.orc_program_f64.restype = I
+        L.orc_program_f32.argtypes = [_fp, _fp, I, _i64, I, _ip, _ip, _fp, _ip, I, _ip, _fp, _ip, _i64, _i64, I, I]
+        L.orc_program_f32.restype = I
         L.orc_mix64.argtypes = [ctypes.c_uint64]
         L.orc_mix64.restype = ctypes.c_uint64
         for f in ("orc_sweep_f64", "orc_sweep_f32", "orc_iterate_f64", "orc_iterate_f32"):
@@ -80,6 +82,12 @@ class Oracle:
             [np.asarray(pr[2] if len(pr) > 2 else [-1] * len(pr[1]), np.int32) for pr in producers])
             if producers else np.full(1, -1, np.int32), dtype=np.int32)
         off = self._st(offsets, dims)
+        if A.dtype == np.float32:   # FP32 storage: coefficients rounded to float
+            return self.L.orc_program_f32(A, B, dims, np.asarray(shape, np.int64), len(producers), pnp, poff,
+                                          pcoef.astype(np.float32), psrc, len(off), off,
+                                          np.asarray(coeffs, np.float64).astype(np.float32),
+                                          np.asarray(sources, np.int32), np.asarray(lo, np.int64),
+                                          np.asarray(hi, np.int64), T, mode)
         rc = self.L.orc_program_f64(A, B, dims, np.asarray(shape, np.int64), len(producers), pnp, poff, pcoef,
                                     psrc, len(off), off, np.asarray(coeffs, np.float64),
                                     np.asarray(sources, np.int32), np.asarray(lo, np.int64),

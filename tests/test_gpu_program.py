@@ -10,7 +10,7 @@ import sys
 import numpy as np
 import pytest
 
-from tests.oracle_py import FUSED, PLAIN
+from tests.oracle_py import FUSED, KAHAN, PLAIN
 from tests.test_program import NESTED_2D, ONE_SIDED_2D, max_program, random_program, run_parsed
 
 pytestmark = pytest.mark.gpu
@@ -228,3 +228,39 @@ def test_tuned_tile_bitwise(gpu, orc):
     finally:
         A.close()
         B.close()
+
+
+def run_gpu_f32(u, prog, A0, B0, T, compensated, kernel=0, tile=None):
+    A = u.DeviceArray.from_host(A0.astype(np.float32))
+    B = u.DeviceArray.from_host(B0.astype(np.float32))
+    try:
+        u.program_sweep_f32(A, B, list(A0.shape), prog, T, compensated, kernel=kernel, tile=tile)
+        u.synchronize()
+        return A.download(), B.download()
+    finally:
+        A.close()
+        B.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("kernel", [0, 2])
+@pytest.mark.parametrize("compensated", [False, True])
+def test_f32_programs_bitwise(gpu, orc, seed, kernel, compensated):
+    """FP32 storage and arithmetic (plain fused sums, or Kahan-compensated),
+    chains included, against the oracle's FP32 modes."""
+    u = gpu
+    rng = np.random.default_rng(300 + seed)
+    dims = 2 + seed % 2
+    shape, prods, offs, coefs, srcs, lo, hi = random_program(rng, dims)
+    prog = u.ProgramSpec(dims, prods, offs, coefs, srcs, lo, hi)
+    A0, B0 = rng.random(shape).astype(np.float32), rng.random(shape).astype(np.float32)
+    log("f32 program seed", seed, "kernel", kernel, "compensated", compensated)
+    for T in (1, 2):
+        try:
+            gA, gB = run_gpu_f32(u, prog, A0, B0, T, compensated, kernel)
+        except u.UhsError as e:
+            assert e.code == -3 and kernel == 1, e
+            return
+        A, B = A0.copy(), B0.copy()
+        assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, T, KAHAN if compensated else FUSED) == 0
+        assert np.array_equal(gA, A) and np.array_equal(gB, B), (seed, T)

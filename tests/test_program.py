@@ -15,7 +15,7 @@ import os
 import numpy as np
 import pytest
 
-from tests.oracle_py import FUSED, PLAIN
+from tests.oracle_py import FUSED, KAHAN, PLAIN
 
 REF = "/root/reference/UHStencil/examples"
 
@@ -34,6 +34,7 @@ def materialise(inp, producers, absolute=False):
     cell outside the grid whose own reads are inside.  Cells whose reads
     leave the grid hold NaN.  absolute: the same sums of |terms|."""
     dims, M = inp.ndim, MARGIN
+    dt = inp.dtype.type                      # the arithmetic type: float64 or float32
     src_in = np.abs(inp) if absolute else inp
     P, reach = [], []
     for pr in producers:
@@ -45,7 +46,7 @@ def materialise(inp, producers, absolute=False):
         # Valid cells in grid coordinates, limited to the margin.
         lo = [max(-M, -r) for r in rlo]
         hi = [min(n + M, n - r) for n, r in zip(inp.shape, rhi)]
-        arr = np.full([n + 2 * M for n in inp.shape], np.nan)
+        arr = np.full([n + 2 * M for n in inp.shape], np.nan, dtype=inp.dtype)
         if all(h > l for l, h in zip(lo, hi)):
             acc = None
             for w, c, s in zip(offs, coefs, srcs):
@@ -53,7 +54,7 @@ def materialise(inp, producers, absolute=False):
                     v = src_in[slab(lo, hi, w)]
                 else:
                     v = P[s][slab([l + M for l in lo], [h + M for h in hi], w)]
-                t = (abs(c) if absolute else c) * v
+                t = dt(abs(c) if absolute else c) * v
                 acc = t if acc is None else acc + t
             arr[slab([l + M for l in lo], [h + M for h in hi], [0] * dims)] = acc
         P.append(arr)
@@ -65,13 +66,14 @@ def two_pass(inp, producers, offsets, coeffs, sources, lo, hi, absolute=False):
     temporary grids, then the output kernel reads them."""
     P = materialise(inp, producers, absolute)
     M = MARGIN
+    dt = inp.dtype.type
     out = None
     for v, c, s in zip(offsets, coeffs, sources):
         if s < 0:
             x = (np.abs(inp) if absolute else inp)[slab(lo, hi, v)]
         else:
             x = P[s][slab([l + M for l in lo], [h + M for h in hi], v)]
-        t = (abs(c) if absolute else c) * x
+        t = dt(abs(c) if absolute else c) * x
         out = t if out is None else out + t
     assert not np.isnan(out).any()
     return out
@@ -166,6 +168,39 @@ def test_oracle_max_program(orc, dims):
     assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, 1, PLAIN) == 0
     want = two_pass(A0, prods, offs, coefs, srcs, lo, hi)
     assert np.array_equal(B[slab(lo, hi, [0] * dims)], want)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_oracle_plain_f32_is_two_pass(orc, seed):
+    """FP32: the oracle's plain mode is the float32 two-pass evaluation
+    (coefficients rounded to float), chains included."""
+    rng = np.random.default_rng(200 + seed)
+    dims = 2 + seed % 2
+    shape, prods, offs, coefs, srcs, lo, hi = random_program(rng, dims)
+    A0 = rng.random(shape).astype(np.float32)
+    B0 = rng.random(shape).astype(np.float32)
+    A, B = A0.copy(), B0.copy()
+    assert orc.program(A, B, shape, prods, offs, coefs, srcs, lo, hi, 1, PLAIN) == 0
+    assert B.dtype == np.float32
+    assert np.array_equal(B[slab(lo, hi, [0] * dims)], two_pass(A0, prods, offs, coefs, srcs, lo, hi))
+
+
+def test_oracle_kahan_f32_closed_form(orc):
+    """Compensated FP32 sums, closed form: on a field of ones, a producer of
+    1 followed by 7 terms of 2^-25 gives exactly 1 in plain FP32 (each
+    2^-25 is below half an ulp of 1) and 1 + 7 * 2^-25 compensated; the
+    output kernel reads it once with coefficient 1."""
+    shape, lo, hi = [12, 20], [2, 2], [10, 18]
+    offs = [[0, 0], [0, 1], [1, 0], [0, -1], [-1, 0], [1, 1], [-1, -1], [1, -1]]
+    prods = [(offs, [1.0] + [2.0 ** -25] * 7, [-1] * 8)]
+    for mode, want in ((PLAIN, np.float32(1.0)), (KAHAN, np.float32(1.0 + 7 * 2.0 ** -25))):
+        A = np.ones(shape, np.float32)
+        B = np.zeros(shape, np.float32)
+        assert orc.program(A, B, shape, prods, [[0, 0]], [1.0], [0], lo, hi, 1, mode) == 0
+        assert np.all(B[2:10, 2:18] == want), mode
+    # FP64 has no compensated mode.
+    A, B = np.ones(shape), np.zeros(shape)
+    assert orc.program(A, B, shape, prods, [[0, 0]], [1.0], [0], lo, hi, 1, KAHAN) == -1
 
 
 @pytest.mark.parametrize("seed", range(6))

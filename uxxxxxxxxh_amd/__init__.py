@@ -122,6 +122,9 @@ _SIGS = {
     "uhs_tune_program_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), _P]),
     "uhs_program_sweep_ex_f64": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _I, ctypes.POINTER(ctypes.c_int), _P]),
+    "uhs_program_sweep_f32": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _I, _P]),
+    "uhs_program_sweep_ex_f32": (_I, [_P, _P, _I64P, ctypes.POINTER(Program), _I, _I, _I,
+                                      ctypes.POINTER(ctypes.c_int), _P]),
     "uhs_parse_dsl_file": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
     "uhs_batch_create": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(BatchConfig), ctypes.POINTER(_P)]),
     "uhs_batch_run": (_I, [_P, _P]),
@@ -301,6 +304,12 @@ def program_sweep_f64(A, B, shape, prog, timesteps, stream=None, kernel=PROGRAM_
     t = (ctypes.c_int * 3)(*tile) if tile is not None else None
     check(lib().uhs_program_sweep_ex_f64(A.ptr, B.ptr, ext_array(shape), ctypes.byref(prog.c), timesteps,
                                          kernel, t, stream))
+
+
+def program_sweep_f32(A, B, shape, prog, timesteps, compensated=False, stream=None, kernel=PROGRAM_AUTO, tile=None):
+    t = (ctypes.c_int * 3)(*tile) if tile is not None else None
+    check(lib().uhs_program_sweep_ex_f32(A.ptr, B.ptr, ext_array(shape), ctypes.byref(prog.c), timesteps,
+                                         1 if compensated else 0, kernel, t, stream))
 
 
 def tune_program_f64(A, B, shape, prog, reps=3, stream=None):

@@ -36,8 +36,8 @@ constexpr int kMaxTerms = UHS_MAX_PROGRAM_TERMS, kMaxProd = UHS_MAX_PRODUCERS;
 constexpr int kTileThreads = 256;
 
 struct ProgParams {
-    const double *in;
-    double *out;
+    const void *in;                // grids of T (double, or float for the FP32 forms)
+    void *out;
     long long lo0, lo1, lo2;       // first written cell (halo-inclusive coordinates)
     long long n0, n1, n2;          // written extents
     long long sy, sz;              // row and plane strides
@@ -56,30 +56,34 @@ struct ProgParams {
 // the depth at compile time: producer k reads only producers below k, so a
 // chain from producer kMaxProd - 1 ends at producer 0, which reads only the
 // input.
-template <int D>
-__device__ double eval_producer(const ProgParams &p, int k, long long q)
+template <typename T, bool KAHAN, int D>
+__device__ T eval_producer(const ProgParams &p, int k, long long q)
 {
+    const T *in = static_cast<const T *>(p.in);
     const int b = p.pstart[k], e = p.pstart[k + 1];
-    double s = 0.0;
+    Acc<T, KAHAN> s;
     for (int t = b; t < e; t++) {
         const long long r = q + p.poff[t];
-        double v;
+        T v;
         if constexpr (D > 0) {
             const int src = p.psrc[t];
-            v = src < 0 ? p.in[r] : eval_producer<D - 1>(p, src, r);
+            v = src < 0 ? in[r] : eval_producer<T, KAHAN, D - 1>(p, src, r);
         } else {
-            v = p.in[r];
+            v = in[r];
         }
-        s = t == b ? p.pcoef[t] * v : __builtin_fma(p.pcoef[t], v, s);
+        if (t == b) s.first((T)p.pcoef[t], v);
+        else s.add((T)p.pcoef[t], v);
     }
-    return s;
+    return s.value();
 }
 
 // DEPTH: producer nesting the launch evaluates (0: producers read only the
 // input grid).
-template <int DEPTH>
+template <typename T, bool KAHAN, int DEPTH>
 __global__ void __launch_bounds__(256) k_program(ProgParams p)
 {
+    const T *in = static_cast<const T *>(p.in);
+    T *out = static_cast<T *>(p.out);
     const long long total = p.n0 * p.n1 * p.n2;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
@@ -87,15 +91,15 @@ __global__ void __launch_bounds__(256) k_program(ProgParams p)
         const long long z = i / plane, r = i - z * plane;
         const long long y = r / p.n2, x = r - y * p.n2;
         const long long c = (z + p.lo0) * p.sz + (y + p.lo1) * p.sy + (x + p.lo2);
-        double acc = 0.0;
+        Acc<T, KAHAN> acc;
         for (int j = 0; j < p.nout; j++) {
             const long long q = c + p.ooff[j];
-            double v;
             const int k = p.osrc[j];
-            v = k < 0 ? p.in[q] : eval_producer<DEPTH>(p, k, q);
-            acc = j == 0 ? p.ocoef[0] * v : __builtin_fma(p.ocoef[j], v, acc);
+            const T v = k < 0 ? in[q] : eval_producer<T, KAHAN, DEPTH>(p, k, q);
+            if (j == 0) acc.first((T)p.ocoef[0], v);
+            else acc.add((T)p.ocoef[j], v);
         }
-        p.out[c] = acc;
+        out[c] = acc.value();
     }
 }
 
@@ -117,8 +121,8 @@ __global__ void __launch_bounds__(256) k_program(ProgParams p)
 // ring; a term at distance d reads ring index s + d at step s.  Each lane
 // keeps its term's slot and advances it by one per step.
 struct StreamParams {
-    const double *in;
-    double *out;
+    const void *in;                // grids of T (double, or float for the FP32 forms)
+    void *out;
     long long base;                // grid index of the domain's first cell
     long long sa, sb;              // strides of axes a and b
     int na, nb, nc;                // domain extents
@@ -169,14 +173,14 @@ __device__ __forceinline__ double lane_dbl(double v, int l)
 // first fused multiply-add; the order of the additions is unchanged.  (The
 // term loop reads its tables with readlane, which the compiler does not
 // unroll.)
-template <int U, class Fetch>
-__device__ __forceinline__ void accumulate(const double *lds, const int (&li)[U], int b, int e, double (&s)[U],
-                                           Fetch &&fetch)
+template <typename T, bool KAHAN, int U, class Fetch>
+__device__ __forceinline__ void accumulate(const T *lds, const int (&li)[U], int b, int e,
+                                           Acc<T, KAHAN> (&s)[U], Fetch &&fetch)
 {
     int q = b;
     for (; q + 4 <= e; q += 4) {
         int off[4];
-        double cf[4], v[4][U];
+        T cf[4], v[4][U];
 #pragma unroll
         for (int g = 0; g < 4; g++) fetch(q + g, off[g], cf[g]);
 #pragma unroll
@@ -186,14 +190,14 @@ __device__ __forceinline__ void accumulate(const double *lds, const int (&li)[U]
 #pragma unroll
         for (int g = 0; g < 4; g++)
 #pragma unroll
-            for (int u = 0; u < U; u++) s[u] = __builtin_fma(cf[g], v[g][u], s[u]);
+            for (int u = 0; u < U; u++) s[u].add(cf[g], v[g][u]);
     }
     for (; q < e; q++) {
         int off;
-        double cf;
+        T cf;
         fetch(q, off, cf);
 #pragma unroll
-        for (int u = 0; u < U; u++) s[u] = __builtin_fma(cf, lds[li[u] + off], s[u]);
+        for (int u = 0; u < U; u++) s[u].add(cf, lds[li[u] + off]);
     }
 }
 
@@ -204,10 +208,11 @@ __device__ __forceinline__ void accumulate(const double *lds, const int (&li)[U]
 // (fewer than the staged rows).  PF: planes of loads in flight.  Items
 // outside a phase's region are clamped onto its last cell (loads stay in
 // bounds) and not written.
-template <int US, int UC, int PF>
+template <typename T, bool KAHAN, int US, int UC, int PF>
 __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
 {
-    extern __shared__ double lds[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
     constexpr int waves = kTileThreads / 64;
     const int lane = threadIdx.x & 63, wave = uniform(threadIdx.x >> 6);
     const int t = xcd_remap(blockIdx.x, p.ntiles);
@@ -267,17 +272,17 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
         gin[u] = (long long)(b0 + p.ilo_b + r) * p.sb + (c0 + p.ilo_c + c);
         lin[u] = (r + p.ilo_b - p.flo_b) * p.Hc + (c + p.ilo_c - p.flo_c);
     }
-    const double *gbase = p.in + p.base;
+    const T *gbase = static_cast<const T *>(p.in) + p.base;
     // Nominal plane q (domain-relative) is read from plane clamp(q): planes
     // outside [qmin, qmax] are staged but never read.
     const int qmin = a0 + p.ilo_a, qmax = a_end - 1 + p.ilo_a + p.ir_a;
-    auto load_plane = [&](int q, double (&v)[US]) {
-        const double *g = gbase + (long long)min(max(q, qmin), qmax) * p.sa;
+    auto load_plane = [&](int q, T (&v)[US]) {
+        const T *g = gbase + (long long)min(max(q, qmin), qmax) * p.sa;
 #pragma unroll
         for (int u = 0; u < US; u++) v[u] = g[gin[u]];
     };
-    auto store_plane = [&](int slot, const double (&v)[US]) {
-        double *s = lds + slot * p.plane;
+    auto store_plane = [&](int slot, const T (&v)[US]) {
+        T *s = lds + slot * p.plane;
 #pragma unroll
         for (int u = 0; u < US; u++)
             if (okin[u]) s[lin[u]] = v[u];
@@ -286,7 +291,7 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
     const int A = a0 - p.lead;     // first step: planes [A + nlo_a, ...] staged
     const int nhi = p.nlo_a + p.span_a;
     for (int i = 0; i < p.span_a; i++) {
-        double v[US];
+        T v[US];
         load_plane(A + p.nlo_a + i, v);
         store_plane(i, v);
     }
@@ -294,7 +299,7 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
     // (the plane of ring index s + span_a) and refills it with the plane PF
     // steps ahead.  The loop is unrolled by PF so slots are registers; the
     // refill loads are unconditional (clamped planes) and never copied.
-    double pf[PF][US];
+    T pf[PF][US];
     // Issued in slot order (no reordering across slots), so the loop entry
     // holds the same loads in flight as the loop's back edge.
     static_for<PF>([&](auto f) {
@@ -325,35 +330,35 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
             if ((p.pbar >> k) & 1) __syncthreads();
             const int rows = eb + lane_int(k_rows, k), cols = ec + lane_int(k_cols, k);
             if (rows <= 0) continue;   // a producer the output kernel never needs
-            double *dst = lds + lane_int(k_ring, k) + lane_int(sk, k) * p.plane;
+            T *dst = lds + lane_int(k_ring, k) + lane_int(sk, k) * p.plane;
             const int b = lane_int(k_first, k), e = lane_int(k_end, k), lk = lane_int(k_li, k);
             // This producer's table (a per-lane select once per producer).
             const int pb = k < 2 ? pb0 : pb1;
             const double pc = k < 2 ? p_c0 : p_c1;
             int li[UC];
             bool ok[UC];
-            double s[UC];
+            Acc<T, KAHAN> s[UC];
 #pragma unroll
             for (int u = 0; u < UC; u++) {
                 int r, c;
                 ok[u] = cell(u, rows, cols, r, c);
                 li[u] = r * p.Hc + c + lk;
             }
-            auto term = [&](int q, int &off, double &cf) {
+            auto term = [&](int q, int &off, T &cf) {
                 off = lane_int(pb, q);
-                cf = lane_dbl(pc, q);
+                cf = (T)lane_dbl(pc, q);
             };
             {
                 int off;
-                double cf;
+                T cf;
                 term(b, off, cf);
 #pragma unroll
-                for (int u = 0; u < UC; u++) s[u] = cf * lds[li[u] + off];
+                for (int u = 0; u < UC; u++) s[u].first(cf, lds[li[u] + off]);
             }
-            accumulate<UC>(lds, li, b + 1, e, s, term);
+            accumulate<T, KAHAN, UC>(lds, li, b + 1, e, s, term);
 #pragma unroll
             for (int u = 0; u < UC; u++)
-                if (ok[u]) dst[li[u]] = s[u];
+                if (ok[u]) dst[li[u]] = s[u].value();
         }
         if (p.nprod) __syncthreads();
         if (a < a0) return;
@@ -362,7 +367,7 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
             int li[UC];
             long long go[UC];
             bool ok[UC];
-            double acc[UC];
+            Acc<T, KAHAN> acc[UC];
 #pragma unroll
             for (int u = 0; u < UC; u++) {
                 int r, c;
@@ -371,22 +376,22 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
                 go[u] = (long long)a * p.sa + (long long)(b0 + r) * p.sb + (c0 + c);
             }
             const int ob = o_fix + so * p.plane;
-            auto term = [&](int j, int &off, double &cf) {
+            auto term = [&](int j, int &off, T &cf) {
                 off = lane_int(ob, j);
-                cf = lane_dbl(o_c, j);
+                cf = (T)lane_dbl(o_c, j);
             };
             {
                 int off;
-                double cf;
+                T cf;
                 term(0, off, cf);
 #pragma unroll
-                for (int u = 0; u < UC; u++) acc[u] = cf * lds[li[u] + off];
+                for (int u = 0; u < UC; u++) acc[u].first(cf, lds[li[u] + off]);
             }
-            accumulate<UC>(lds, li, 1, p.nout, acc, term);
-            double *out = p.out + p.base;
+            accumulate<T, KAHAN, UC>(lds, li, 1, p.nout, acc, term);
+            T *out = static_cast<T *>(p.out) + p.base;
 #pragma unroll
             for (int u = 0; u < UC; u++)
-                if (ok[u]) out[go[u]] = acc[u];
+                if (ok[u]) out[go[u]] = acc[u].value();
         }
         });
 }
@@ -558,7 +563,8 @@ constexpr int kMaxItemsPerWave = 8;
 // Builds the streaming kernel's parameters for tile {TC, TB, TA} (3D) or
 // {TC, TA, 1} (2D).  Returns the LDS bytes of one workgroup, SIZE_MAX when
 // the frame cannot fit; *us, *uc = staged and compute items per wave.
-size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], StreamParams &p, int *us, int *uc)
+size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], size_t esz, StreamParams &p, int *us,
+                    int *uc)
 {
     const int d = pg->dims;
     const uhs_stencil &o = pg->output;
@@ -596,7 +602,7 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
             widen(c.cons[k].lo, c.cons[k].hi);
             maxcr = (int)std::max<long long>(maxcr, c.cons[k].hi[ax_a] - c.cons[k].lo[ax_a]);
         }
-    constexpr long long kCells = kMaxLdsBytes / sizeof(double);
+    const long long kCells = (long long)(kMaxLdsBytes / esz);
     const long long hb = p.TB + fhi[ax_b] - flo[ax_b], wc = p.TC + fhi[ax_c] - flo[ax_c];
     const long long ira = c.in.hi[ax_a] - c.in.lo[ax_a];
     if (hb > kCells || wc > kCells || ira > kCells) return SIZE_MAX;
@@ -719,7 +725,7 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
         p.ooff[j] = (int)(v[ax_b] * p.Hc + v[ax_c]);
         p.ocoef[j] = o.coeffs[j];
     }
-    return (size_t)cells * sizeof(double);
+    return (size_t)cells * esz;
 }
 
 // Default tiles, first fit: rows of 64 << n cells once the reach is added;
@@ -747,31 +753,32 @@ typedef void (*LaunchProgram)(StreamParams);
 // Instantiations: staged items per wave 1, 2, 4 or 8, compute items 1 to 4
 // or 8 (never more than staged); planes in flight more where a plane is few
 // registers.
+template <typename T, bool KAHAN>
 LaunchProgram pick_stream(int us, int uc)
 {
-    if (us <= 1) return k_program_stream<1, 1, 4>;
-    if (us <= 2) return uc <= 1 ? k_program_stream<2, 1, 4> : k_program_stream<2, 2, 4>;
+    if (us <= 1) return k_program_stream<T, KAHAN, 1, 1, 4>;
+    if (us <= 2) return uc <= 1 ? k_program_stream<T, KAHAN, 2, 1, 4> : k_program_stream<T, KAHAN, 2, 2, 4>;
     if (us <= 4) {
         switch (uc) {
-        case 1: return k_program_stream<4, 1, 2>;
-        case 2: return k_program_stream<4, 2, 2>;
-        case 3: return k_program_stream<4, 3, 2>;
-        default: return k_program_stream<4, 4, 2>;
+        case 1: return k_program_stream<T, KAHAN, 4, 1, 2>;
+        case 2: return k_program_stream<T, KAHAN, 4, 2, 2>;
+        case 3: return k_program_stream<T, KAHAN, 4, 3, 2>;
+        default: return k_program_stream<T, KAHAN, 4, 4, 2>;
         }
     }
     switch (uc) {
-    case 1: return k_program_stream<8, 1, 1>;
-    case 2: return k_program_stream<8, 2, 1>;
-    case 3: return k_program_stream<8, 3, 1>;
-    case 4: return k_program_stream<8, 4, 1>;
-    default: return k_program_stream<8, 8, 1>;
+    case 1: return k_program_stream<T, KAHAN, 8, 1, 1>;
+    case 2: return k_program_stream<T, KAHAN, 8, 2, 1>;
+    case 3: return k_program_stream<T, KAHAN, 8, 3, 1>;
+    case 4: return k_program_stream<T, KAHAN, 8, 4, 1>;
+    default: return k_program_stream<T, KAHAN, 8, 8, 1>;
     }
 }
 
-} // namespace
-
-extern "C" int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
-                                        int timesteps, int kernel, const int *tile, uhs_stream_t stream)
+// T sweeps of a program on grids of T (KAHAN: compensated sums).
+template <typename T, bool KAHAN>
+int program_sweep(void *A, void *B, const int64_t *shape, const uhs_program *pg, int timesteps, int kernel,
+                  const int *tile, uhs_stream_t stream)
 {
     if (!A || !B || !shape || !pg) return perr(UHS_E_INVAL, "null argument");
     if (A == B) return perr(UHS_E_INVAL, "A and B must be distinct grids");
@@ -798,7 +805,7 @@ extern "C" int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *sha
             n = default_tiles(c, pg->dims, cand);
         }
         for (int i = 0; i < n; i++) {
-            lds = build_stream(pg, c, cand[i], sp, &us, &uc);
+            lds = build_stream(pg, c, cand[i], sizeof(T), sp, &us, &uc);
             if (lds <= kMaxLdsBytes) break;
         }
         if (lds > kMaxLdsBytes) {
@@ -817,7 +824,7 @@ extern "C" int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *sha
         }
     }
     if (tiled) {
-        const LaunchProgram kern = pick_stream(us, uc);
+        const LaunchProgram kern = pick_stream<T, KAHAN>(us, uc);
         for (int st = 1; st <= timesteps; st++) {
             sp.in = (st & 1) ? A : B;
             sp.out = (st & 1) ? B : A;
@@ -836,13 +843,35 @@ extern "C" int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *sha
         p.in = (st & 1) ? A : B;
         p.out = (st & 1) ? B : A;
         if (pg->producer_sources)
-            hipLaunchKernelGGL(k_program<kMaxProd - 1>, dim3(blocks), dim3(256), 0, s, p);
+            hipLaunchKernelGGL((k_program<T, KAHAN, kMaxProd - 1>), dim3(blocks), dim3(256), 0, s, p);
         else
-            hipLaunchKernelGGL(k_program<0>, dim3(blocks), dim3(256), 0, s, p);
+            hipLaunchKernelGGL((k_program<T, KAHAN, 0>), dim3(blocks), dim3(256), 0, s, p);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return perr(UHS_E_HIP, std::string("program launch: ") + hipGetErrorString(e));
     }
     return UHS_OK;
+}
+
+} // namespace
+
+extern "C" int uhs_program_sweep_ex_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
+                                        int timesteps, int kernel, const int *tile, uhs_stream_t stream)
+{
+    return program_sweep<double, false>(A, B, shape, pg, timesteps, kernel, tile, stream);
+}
+
+extern "C" int uhs_program_sweep_ex_f32(float *A, float *B, const int64_t *shape, const uhs_program *pg,
+                                        int timesteps, int compensated, int kernel, const int *tile,
+                                        uhs_stream_t stream)
+{
+    return compensated ? program_sweep<float, true>(A, B, shape, pg, timesteps, kernel, tile, stream)
+                       : program_sweep<float, false>(A, B, shape, pg, timesteps, kernel, tile, stream);
+}
+
+extern "C" int uhs_program_sweep_f32(float *A, float *B, const int64_t *shape, const uhs_program *pg,
+                                     int timesteps, int compensated, uhs_stream_t stream)
+{
+    return uhs_program_sweep_ex_f32(A, B, shape, pg, timesteps, compensated, UHS_PROGRAM_AUTO, nullptr, stream);
 }
 
 extern "C" int uhs_program_sweep_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
@@ -883,16 +912,16 @@ extern "C" int uhs_tune_program_f64(double *A, double *B, const int64_t *shape, 
     for (const auto &t : cand) {
         StreamParams sp;
         int us, uc;
-        if (build_stream(pg, c, t.data(), sp, &us, &uc) > kMaxLdsBytes) continue;
+        if (build_stream(pg, c, t.data(), sizeof(double), sp, &us, &uc) > kMaxLdsBytes) continue;
         rc = uhs_program_sweep_ex_f64(A, B, shape, pg, 1, UHS_PROGRAM_TILED, t.data(), stream);
         if (rc) break;
-        hipEventRecord(e0, s);
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = perr(UHS_E_HIP, "event record failed"); break; }
         rc = uhs_program_sweep_ex_f64(A, B, shape, pg, reps, UHS_PROGRAM_TILED, t.data(), stream);
         if (rc) break;
-        hipEventRecord(e1, s);
+        if (hipEventRecord(e1, s) != hipSuccess) { rc = perr(UHS_E_HIP, "event record failed"); break; }
         if (hipEventSynchronize(e1) != hipSuccess) { rc = perr(UHS_E_HIP, "event wait failed"); break; }
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = perr(UHS_E_HIP, "event timing failed"); break; }
         const double per = ms / reps;
         n++;
         if (!found || per < best) {
@@ -901,8 +930,8 @@ extern "C" int uhs_tune_program_f64(double *A, double *B, const int64_t *shape, 
             found = 1;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (evaluated) *evaluated = n;
     if (!found) return perr(UHS_E_CONFIG, "no tile of the streaming kernel fits this program in the LDS");

@@ -130,9 +130,9 @@ int uhs_sweep_f32(float *A, float *B, int dims, const int64_t *ext, int halo,
 
 /* Which kernel a stencil/precision/config maps to, without launching:
  * writes a short name into buf ("star3d", "box3d", "win2d", "program" --
- * an FP64 stencil of other forms with at most UHS_MAX_PROGRAM_TERMS terms,
- * run by the streaming program kernel --, "gather" -- the general
- * one-thread-per-point kernel otherwise).
+ * a stencil of other forms with at most UHS_MAX_PROGRAM_TERMS terms, run by
+ * the streaming program kernel --, "gather" -- the general one-thread-per-
+ * point kernel for larger ones).
  * Returns UHS_OK, or the error the sweep would return. */
 int uhs_plan_kind(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
                   int dtype /* 0 f64, 1 f32, 2 f32 compensated */,

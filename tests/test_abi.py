@@ -84,7 +84,7 @@ def test_argument_validation(uhs):
     expect_code(uhs, -2, lambda: uhs.plan_kind(ext, 3, uhs.StencilSpec(3, many, np.ones(343))))
     expect_code(uhs, -2, lambda: uhs.plan_kind(ext, 5, uhs.StencilSpec(3, [[5, 0, 0]], [1.0])))
     assert uhs.plan_kind(ext, 2, uhs.StencilSpec(3, [[0, 0, 0], [1, 0, 0]], [0.5, 0.5])) == "program"
-    assert uhs.plan_kind(ext, 2, uhs.StencilSpec(3, [[0, 0, 0], [1, 0, 0]], [0.5, 0.5]), uhs.DT_F32) == "gather"
+    assert uhs.plan_kind(ext, 2, uhs.StencilSpec(3, [[0, 0, 0], [1, 0, 0]], [0.5, 0.5]), uhs.DT_F32) == "program"
 
 
 def test_no_gpu_needed_for_planning(uhs):

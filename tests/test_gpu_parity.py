@@ -239,9 +239,10 @@ def test_f32_parity(gpu, orc, case, compensated):
 
 
 def test_general_stencil_paths(gpu, orc):
-    """Stencils outside the 12 built-in forms: FP64 with at most 32 terms
-    runs the streaming program kernel, FP32 and larger stencils the general
-    one-thread-per-point kernel; all bitwise equal to the oracle."""
+    """Stencils outside the 12 built-in forms: up to 32 terms run the
+    streaming program kernel, larger ones the general one-thread-per-point
+    kernel; FP64, FP32 and FP32 compensated all bitwise equal to the
+    oracle."""
     u = gpu
     rng = np.random.default_rng(3)
     cases = [
@@ -259,18 +260,19 @@ def test_general_stencil_paths(gpu, orc):
         st = u.StencilSpec(dims, offs, c)
         h = 3
         ext = [23, 31, 45] if dims == 3 else [50, 77]
-        assert u.plan_kind(ext, h, st) == ("program" if len(offs) <= 32 else "gather")
-        assert u.plan_kind(ext, h, st, u.DT_F32) == "gather"
+        kind = "program" if len(offs) <= 32 else "gather"
+        assert u.plan_kind(ext, h, st) == kind and u.plan_kind(ext, h, st, u.DT_F32) == kind
         log("general stencil", dims, "D", len(offs), "terms")
         A0, B0 = random_grids(ext, h, 4)
         gA, gB = run_gpu(u, st, A0, B0, ext, h, 3)
         A, B = A0.copy(), B0.copy()
         orc.iterate(A, B, ext, h, offs, c, 3, FUSED)
         assert np.array_equal(gA, A) and np.array_equal(gB, B)
-        gA, gB = run_gpu(u, st, A0, B0, ext, h, 2, dtype=np.float32)
-        A, B = A0.astype(np.float32), B0.astype(np.float32)
-        orc.iterate(A, B, ext, h, offs, c, 2, FUSED)
-        assert np.array_equal(gA, A) and np.array_equal(gB, B)
+        for compensated, mode in ((False, FUSED), (True, KAHAN)):
+            gA, gB = run_gpu(u, st, A0, B0, ext, h, 2, dtype=np.float32, compensated=compensated)
+            A, B = A0.astype(np.float32), B0.astype(np.float32)
+            orc.iterate(A, B, ext, h, offs, c, 2, mode)
+            assert np.array_equal(gA, A) and np.array_equal(gB, B)
 
 
 def test_config1_3d7pt_256_T10(gpu, orc):

@@ -396,18 +396,18 @@ int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream)
     return UHS_OK;
 }
 
-// FP64 stencils outside the built-in forms with at most
-// UHS_MAX_PROGRAM_TERMS terms run as a one-kernel program over the interior:
-// the streaming program kernel computes the same sum in the same order
-// (first term a product, then fused multiply-adds) as the one-thread-per-
-// point general kernel, staged through LDS instead of the caches.
+// Stencils outside the built-in forms with at most UHS_MAX_PROGRAM_TERMS
+// terms run as a one-kernel program over the interior: the streaming program
+// kernel computes the same sum in the same order (first term a product, then
+// fused multiply-adds, or their compensated form) as the one-thread-per-point
+// general kernel, staged through LDS instead of the caches.
 static bool as_program(const Plan &pl, const uhs_stencil *st)
 {
-    return pl.family == FAM_GATHER && pl.args.dtype == DT_F64 && st->npoints <= UHS_MAX_PROGRAM_TERMS;
+    return pl.family == FAM_GATHER && st->npoints <= UHS_MAX_PROGRAM_TERMS;
 }
 
-static int sweep_as_program(double *A, double *B, int dims, const int64_t *ext, int halo,
-                            const uhs_stencil *st, int timesteps, hipStream_t stream)
+static int sweep_as_program(void *A, void *B, int dims, const int64_t *ext, int halo, const uhs_stencil *st,
+                            int timesteps, int dtype, hipStream_t stream)
 {
     int src[UHS_MAX_PROGRAM_TERMS];
     for (int t = 0; t < st->npoints; t++) src[t] = -1;
@@ -421,7 +421,23 @@ static int sweep_as_program(double *A, double *B, int dims, const int64_t *ext, 
         pg.lo[a] = halo;
         pg.hi[a] = halo + ext[a];
     }
-    return uhs_program_sweep_ex_f64(A, B, shape, &pg, timesteps, UHS_PROGRAM_AUTO, nullptr, stream);
+    if (dtype == DT_F64)
+        return uhs_program_sweep_ex_f64((double *)A, (double *)B, shape, &pg, timesteps, UHS_PROGRAM_AUTO, nullptr,
+                                        stream);
+    return uhs_program_sweep_ex_f32((float *)A, (float *)B, shape, &pg, timesteps, dtype == DT_F32K ? 1 : 0,
+                                    UHS_PROGRAM_AUTO, nullptr, stream);
+}
+
+static int sweep_or_program(Plan &pl, void *A, void *B, int dims, const int64_t *ext, int halo,
+                            const uhs_stencil *st, int timesteps, hipStream_t stream)
+{
+    if (as_program(pl, st)) {
+        if (!A || !B) return fail(UHS_E_INVAL, "null grid pointer");
+        if (A == B) return fail(UHS_E_INVAL, "A and B must be distinct grids");
+        if (timesteps < 0) return fail(UHS_E_INVAL, "negative timestep count");
+        return sweep_as_program(A, B, dims, ext, halo, st, timesteps, pl.args.dtype, stream);
+    }
+    return uhs_run_plan(pl, A, B, timesteps, stream);
 }
 
 extern "C" {
@@ -432,13 +448,7 @@ int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
     Plan pl;
     int rc = uhs_make_plan(dims, ext, halo, st, DT_F64, cfg, &pl);
     if (rc) return rc;
-    if (as_program(pl, st)) {
-        if (!A || !B) return fail(UHS_E_INVAL, "null grid pointer");
-        if (A == B) return fail(UHS_E_INVAL, "A and B must be distinct grids");
-        if (timesteps < 0) return fail(UHS_E_INVAL, "negative timestep count");
-        return sweep_as_program(A, B, dims, ext, halo, st, timesteps, (hipStream_t)stream);
-    }
-    return uhs_run_plan(pl, A, B, timesteps, (hipStream_t)stream);
+    return sweep_or_program(pl, A, B, dims, ext, halo, st, timesteps, (hipStream_t)stream);
 }
 
 int uhs_sweep_f32(float *A, float *B, int dims, const int64_t *ext, int halo,
@@ -448,7 +458,7 @@ int uhs_sweep_f32(float *A, float *B, int dims, const int64_t *ext, int halo,
     Plan pl;
     int rc = uhs_make_plan(dims, ext, halo, st, compensated ? DT_F32K : DT_F32, cfg, &pl);
     if (rc) return rc;
-    return uhs_run_plan(pl, A, B, timesteps, (hipStream_t)stream);
+    return sweep_or_program(pl, A, B, dims, ext, halo, st, timesteps, (hipStream_t)stream);
 }
 
 int uhs_plan_kind(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,

@@ -86,6 +86,27 @@ def temporal2(u, n, T):
     return out
 
 
+XGMI_LINK_GBPS = 153.0       # MI355X, one xGMI link, one direction, published
+
+
+def shard_share(u, T=100, gpus=8):
+    """One GPU's share of the 512^3 3d1r_star grid slab-decomposed along
+    axis 0 over `gpus` GPUs: the sweep of a 512x512x(512/gpus) slab (its
+    ghost plane is the halo), measured, beside the time to exchange one
+    2 MiB face per neighbour per step over one xGMI link each (a model: the
+    pool has one GPU)."""
+    n0 = 512 // gpus
+    ms_step, ms_sweep, cfg = time_sweeps(u, "3d1r_star", [n0, 512, 512], T, 2, 1)
+    face_bytes = 512 * 512 * 8
+    exch_us = face_bytes / (XGMI_LINK_GBPS * 1e9) * 1e6
+    return {"workload": "3d1r_star FP64 %dx512x512 slab (one of %d), T=%d" % (n0, gpus, T),
+            "config": [cfg.tile_x, cfg.tile_y, cfg.tile_z, cfg.vec, cfg.flags],
+            "GStencils_s": round(n0 * 512 * 512 / (ms_sweep * 1e-3) / 1e9, 2),
+            "sweep_us": round(ms_sweep * 1e3, 2),
+            "face_exchange_us_model": round(exch_us, 2),
+            "exchange_over_sweep": round(exch_us / (ms_sweep * 1e3), 3)}
+
+
 def copy_bandwidth(u, nbytes=2 << 30, reps=10):
     """Rate of the library's streaming copy kernel (read + write bytes / s):
     the HBM bandwidth this box reaches, for context next to the roofline."""
@@ -411,6 +432,8 @@ def main():
         log("temporal2", out["temporal2"])
     if not args.no_programs:
         out["programs"] = programs(u)
+    out["shard_share"] = shard_share(u, T)
+    log("shard share", out["shard_share"])
     if not args.no_batch:
         out["batched"] = batched(u)
         log("batched", out["batched"])

@@ -129,9 +129,10 @@ int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *ext,
  * {tile_x, tile_y, tile_z} whose LDS rings fit runs one warm-up sweep and
  * then `reps` timed sweeps on the caller's grids (overwritten).  tile_out[3]
  * receives the fastest tile, *best_ms its time per sweep, *evaluated the
- * number of tiles timed (any output pointer but tile_out may be NULL).
- * UHS_E_CONFIG when no candidate fits: the pointwise kernel is then the only
- * one. */
+ * number of tiles timed (any output pointer but tile_out may be NULL).  The
+ * winner becomes the tile uhs_program_sweep_f64 uses for this program and
+ * grid shape when no tile is given (this process).  UHS_E_CONFIG when no
+ * candidate fits: the pointwise kernel is then the only one. */
 int uhs_tune_program_f64(double *A, double *B, const int64_t *shape, const uhs_program *pg,
                          int reps, int *tile_out, double *best_ms, int *evaluated,
                          uhs_stream_t stream);

@@ -214,6 +214,9 @@ def test_tuned_tile_bitwise(gpu, orc):
     log("tuned tile", tile, ms, n)
     assert n > 0 and ms > 0 and tile[2] == 1
     gA, gB = run_gpu(u, prog, A0, A0.copy(), 4, 1, tile)
+    # The tuned tile is now the untiled call's choice; same result.
+    gA2, gB2 = run_gpu(u, prog, A0, A0.copy(), 4)
+    assert np.array_equal(gA, gA2) and np.array_equal(gB, gB2)
     A, B = A0.copy(), A0.copy()
     assert run_parsed(orc, prog, A, B, 4, FUSED) == 0
     assert np.array_equal(gA, A) and np.array_equal(gB, B)

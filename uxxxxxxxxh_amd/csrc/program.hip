@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <array>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 #include <climits>
 #include <cstdint>
@@ -775,6 +777,39 @@ LaunchProgram pick_stream(int us, int uc)
     }
 }
 
+// Tiles uhs_tune_program_f64 picked in this process, by program, grid shape
+// and element size: the tile an untiled call uses.
+std::mutex g_tuned_mu;
+std::unordered_map<std::string, std::array<int, 3>> g_tuned;
+
+std::string program_key(const int64_t *shape, const uhs_program *pg, size_t esz)
+{
+    std::string k;
+    auto put = [&](const void *p, size_t n) { k.append(static_cast<const char *>(p), n); };
+    const int d = pg->dims;
+    put(&esz, sizeof esz);
+    put(&d, sizeof d);
+    put(shape, sizeof(int64_t) * d);
+    put(pg->lo, sizeof(int64_t) * d);
+    put(pg->hi, sizeof(int64_t) * d);
+    put(&pg->nproducers, sizeof(int));
+    auto stencil = [&](const uhs_stencil &s) {
+        put(&s.npoints, sizeof(int));
+        put(s.offsets, sizeof(int) * s.npoints * d);
+        put(s.coeffs, sizeof(double) * s.npoints);
+    };
+    for (int k2 = 0; k2 < pg->nproducers; k2++) {
+        stencil(pg->producers[k2]);
+        for (int t = 0; t < pg->producers[k2].npoints; t++) {
+            const int s = psrc(pg, k2, t);
+            put(&s, sizeof s);
+        }
+    }
+    stencil(pg->output);
+    put(pg->sources, sizeof(int) * pg->output.npoints);
+    return k;
+}
+
 // T sweeps of a program on grids of T (KAHAN: compensated sums).
 template <typename T, bool KAHAN>
 int program_sweep(void *A, void *B, const int64_t *shape, const uhs_program *pg, int timesteps, int kernel,
@@ -794,7 +829,7 @@ int program_sweep(void *A, void *B, const int64_t *shape, const uhs_program *pg,
     size_t lds = SIZE_MAX;
     int us = 0, uc = 0;
     if (tiled) {
-        int cand[8][3], n = 0;
+        int cand[12][3], n = 0;   // a tuned tile, then at most 8 defaults
         if (tile) {
             for (int a = 0; a < 3; a++) cand[0][a] = tile[a];
             if (tile[0] < 1 || tile[1] < 1 || tile[2] < 1 || tile[0] > 4096 || tile[1] > 4096 || tile[2] > 4096 ||
@@ -802,7 +837,16 @@ int program_sweep(void *A, void *B, const int64_t *shape, const uhs_program *pg,
                 return perr(UHS_E_CONFIG, "tile extents must be 1..4096 (tile_z 1 for 2D)");
             n = 1;
         } else {
-            n = default_tiles(c, pg->dims, cand);
+            n = 0;
+            {
+                std::lock_guard<std::mutex> lk(g_tuned_mu);
+                auto it = g_tuned.find(program_key(shape, pg, sizeof(T)));
+                if (it != g_tuned.end()) {
+                    for (int a = 0; a < 3; a++) cand[0][a] = it->second[a];
+                    n = 1;
+                }
+            }
+            n += default_tiles(c, pg->dims, cand + n);
         }
         for (int i = 0; i < n; i++) {
             lds = build_stream(pg, c, cand[i], sizeof(T), sp, &us, &uc);
@@ -936,5 +980,9 @@ extern "C" int uhs_tune_program_f64(double *A, double *B, const int64_t *shape, 
     if (evaluated) *evaluated = n;
     if (!found) return perr(UHS_E_CONFIG, "no tile of the streaming kernel fits this program in the LDS");
     if (best_ms) *best_ms = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        g_tuned[program_key(shape, pg, sizeof(double))] = {tile_out[0], tile_out[1], tile_out[2]};
+    }
     return UHS_OK;
 }

@@ -107,6 +107,26 @@ def shard_share(u, T=100, gpus=8):
             "exchange_over_sweep": round(exch_us / (ms_sweep * 1e3), 3)}
 
 
+def fma_rate(u, n=256 * 32 * 256, iters=2048, reps=5):
+    """Rate of the library's FP64 FMA probe (32 independent chains per
+    thread, no memory traffic), in TFLOP/s at 2 flops per FMA: the FP64
+    vector rate this box reaches, for the FP64-ALU-bound stencils."""
+    out = u.DeviceArray([n], np.float64)
+    L = u.lib()
+    x, y = 1.0 - 2.0 ** -20, 2.0 ** -21
+    for _ in range(2):
+        u.check(L.uhs_fma_f64(out.ptr, n, x, y, iters, None))
+    u.synchronize()
+    tm = u.Timer()
+    tm.start()
+    for _ in range(reps):
+        u.check(L.uhs_fma_f64(out.ptr, n, x, y, iters, None))
+    tm.stop()
+    ms = tm.ms()
+    out.close()
+    return 2.0 * 32 * iters * n * reps / (ms * 1e-3) / 1e12
+
+
 def copy_bandwidth(u, nbytes=2 << 30, reps=10):
     """Rate of the library's streaming copy kernel (read + write bytes / s):
     the HBM bandwidth this box reaches, for context next to the roofline."""
@@ -417,6 +437,11 @@ def main():
         log("device copy bandwidth %.0f GB/s" % out["measured_copy_GBps"])
     except Exception as e:  # context only
         log("copy bandwidth probe failed:", e)
+    try:
+        out["measured_fp64_TFLOPs"] = round(fma_rate(u), 2)
+        log("device FP64 FMA rate %.1f TFLOP/s" % out["measured_fp64_TFLOPs"])
+    except Exception as e:  # context only
+        log("FMA rate probe failed:", e)
     if not args.no_shapes:
         per = {}
         for case in u.builtin_names():
@@ -432,6 +457,12 @@ def main():
                          "hbm_frac": round(g * 16 / PEAK_HBM_GBPS, 4),
                          "roof_GStencils_s": round(roof, 1), "roof_frac": round(g / roof, 4),
                          "tile": [c.tile_x, c.tile_y, c.tile_z, c.vec, c.flags]}
+            if "measured_copy_GBps" in out and "measured_fp64_TFLOPs" in out:
+                # The same roof from the rates this box reaches (copy kernel,
+                # FMA probe) instead of the published peaks.
+                mroof = min(out["measured_copy_GBps"] / 16.0, out["measured_fp64_TFLOPs"] * 1e3 / flops)
+                per[case]["measured_roof_GStencils_s"] = round(mroof, 1)
+                per[case]["measured_roof_frac"] = round(g / mroof, 4)
             log("%-10s %8.2f GSt/s  %.3f ms/sweep  roof %.0f  frac %.3f" % (case, g, ms_w, roof, g / roof))
         out["per_shape"] = per
     if not args.no_temporal:

@@ -299,6 +299,12 @@ int uhs_program_sweep_ex_f32(float *A, float *B, const int64_t *shape, const uhs
  * attainable-HBM-bandwidth reference the benchmark reports, and the
  * calibration kernel for the traffic counters. */
 int uhs_copy_f64(double *dst, const double *src, size_t n, uhs_stream_t stream);
+/* FP64 FMA throughput probe: n threads each run `iters` rounds of 32
+ * independent chains a_i <- fma(a_i, x, y), a_i starting at i, and write
+ * sum_i a_i to out[thread] (out holds n doubles).  32 * iters * n FMAs per
+ * call, no memory traffic inside the loop: the attainable FP64 vector rate,
+ * which the benchmark reports next to the FP64-ALU-bound stencils' roof. */
+int uhs_fma_f64(double *out, size_t n, double x, double y, int iters, uhs_stream_t stream);
 const char *uhs_last_error(void);
 const char *uhs_version(void);
 int uhs_device_count(void);

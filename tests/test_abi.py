@@ -87,6 +87,13 @@ def test_argument_validation(uhs):
     assert uhs.plan_kind(ext, 2, uhs.StencilSpec(3, [[0, 0, 0], [1, 0, 0]], [0.5, 0.5]), uhs.DT_F32) == "program"
 
 
+def test_probe_argument_checks(uhs):
+    # Rejected before any device call.
+    L = uhs.lib()
+    assert L.uhs_fma_f64(None, 10, 1.0, 0.0, 1, None) == -1
+    assert L.uhs_copy_f64(None, None, 10, None) == -1
+
+
 def test_no_gpu_needed_for_planning(uhs):
     # Planning, defaults and coefficient generation are host-only.
     st = uhs.builtin("2d3r_box")

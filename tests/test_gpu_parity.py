@@ -315,6 +315,20 @@ def test_error_norms(gpu, orc):
         d.close()
 
 
+def test_fma_probe_exact(gpu):
+    # x = 0.5, y = 0.25, a_i starting at i: three rounds stay exact in
+    # binary64, so every thread's sum is known in closed form:
+    # a_i -> i/8 + 0.25/4 + 0.25/2 + 0.25.
+    u = gpu
+    n = 1000
+    out = u.DeviceArray([n], np.float64)
+    u.check(u.lib().uhs_fma_f64(out.ptr, n, 0.5, 0.25, 3, None))
+    got = out.download()
+    out.close()
+    want = sum(i / 8 + 0.0625 + 0.125 + 0.25 for i in range(32))
+    assert np.all(got == want)
+
+
 def test_argument_errors(gpu):
     u = gpu
     st = u.builtin("3d1r_star")

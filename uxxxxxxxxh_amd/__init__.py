@@ -164,6 +164,7 @@ _SIGS = {
     "uhs_mlp_save": (_I, [_P, ctypes.c_char_p]),
     "uhs_mlp_load": (_I, [_P, ctypes.c_char_p]),
     "uhs_copy_f64": (_I, [_P, _P, ctypes.c_size_t, _P]),
+    "uhs_fma_f64": (_I, [_P, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, _I, _P]),
     "uhs_last_error": (ctypes.c_char_p, []),
     "uhs_version": (ctypes.c_char_p, []),
     "uhs_device_count": (_I, []),

@@ -610,6 +610,16 @@ int uhs_copy_f64(double *dst, const double *src, size_t n, uhs_stream_t stream)
     return e == hipSuccess ? UHS_OK : hip_fail(e, "copy");
 }
 
+int uhs_fma_f64(double *out, size_t n, double x, double y, int iters, uhs_stream_t stream)
+{
+    if (!out) return fail(UHS_E_INVAL, "null pointer");
+    if (iters < 0) return fail(UHS_E_INVAL, "negative iteration count");
+    if (n == 0) return UHS_OK;
+    if (n > (size_t)(1LL << 32)) return fail(UHS_E_INVAL, "too many threads");
+    hipError_t e = uhs_launch_fma(out, (long long)n, x, y, iters, (hipStream_t)stream);
+    return e == hipSuccess ? UHS_OK : hip_fail(e, "fma");
+}
+
 const char *uhs_last_error(void) { return g_err.c_str(); }
 const char *uhs_version(void) { return "uhs-mi355x 0.1 (gfx950)"; }
 

@@ -113,7 +113,32 @@ __global__ void __launch_bounds__(256) k_copy(const double *__restrict__ a, doub
     if (i < n) b[i] = a[i];
 }
 
+// FP64 FMA throughput: 32 independent chains per thread, no memory traffic
+// inside the loop (x, y and iters are runtime values, so nothing folds).
+__global__ void __launch_bounds__(256) k_fma(double *__restrict__ out, long long n, double x, double y, int iters)
+{
+    constexpr int NA = 32;
+    double a[NA];
+#pragma unroll
+    for (int i = 0; i < NA; i++) a[i] = (double)i;
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int i = 0; i < NA; i++) a[i] = __builtin_fma(a[i], x, y);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NA; i++) s += a[i];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = s;
+}
+
 } // namespace
+
+hipError_t uhs_launch_fma(double *out, long long n, double x, double y, int iters, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fma, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, x, y, iters);
+    return hipGetLastError();
+}
 
 hipError_t uhs_launch_copy(const double *src, double *dst, long long n, hipStream_t st)
 {

@@ -16,3 +16,4 @@ hipError_t uhs_launch_fill(void *grid, int is_f32, const GridShape &s, uint64_t 
 hipError_t uhs_launch_norms(const void *a, int a_is_f32, const double *b, const GridShape &s,
                             double *partial, int nparts, double *out3, hipStream_t st);
 hipError_t uhs_launch_copy(const double *src, double *dst, long long n, hipStream_t st);
+hipError_t uhs_launch_fma(double *out, long long n, double x, double y, int iters, hipStream_t st);

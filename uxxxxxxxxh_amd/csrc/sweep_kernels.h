@@ -748,6 +748,14 @@ template <class S> constexpr bool row_grouped()
 // box3d with two output rows per thread (vec 1): every staged value is read
 // from LDS once and applied to the pending sums of both rows, which halves
 // the LDS reads per output -- the boxes' other bottleneck next to FP64 FMA.
+// Whether box3d_tile_rows2 keeps its plane loads in a register ring (see
+// there).  Measured on MI355X at 512^3: the radius-2 box gains 10%; the
+// radius-1 box loses 20% (its 8 cells per thread become 8 unconditional
+// loads where a typical tile needs 3); for the radius-3 box, whose 343
+// coefficients already fill the scalar registers, the unrolled loop does not
+// compile (the register allocator fails).
+template <class S> constexpr bool box_plane_ring() { return S::R == 2; }
+
 template <typename T, class S, bool KAHAN, int GPF, int NL = box_cells_per_thread(S::R), class CF>
 __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__restrict__ out,
                                                  const TileGeom &p, int x0, int y0, int zs, int ze,
@@ -782,13 +790,14 @@ __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__
         const int ly = gv[k] ? c / LW : 0, lx = gv[k] ? c % LW : 0;
         goff[k] = clampi(y0 - R + ly, -R, p.n1 + R - 1) * sy + clampi(x0 - R + lx, -R, p.n2 + R - 1);
     }
+    constexpr bool RING = GPF > 1 && box_plane_ring<S>();
     T pl[GPF][NL];
 #pragma unroll
     for (int f = 0; f < GPF; f++) {
         const T *pp = in + (long long)min(zs + DZMIN + f, p.n0 + R - 1) * sz;
 #pragma unroll
         for (int k = 0; k < NL; k++)
-            if (gv[k]) pl[f][k] = pp[goff[k]];
+            if (RING || gv[k]) pl[f][k] = pp[goff[k]];
     }
     Acc<T, KAHAN> acc[2][NJ];
 #pragma unroll
@@ -796,21 +805,35 @@ __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__
 #pragma unroll
         for (int j = 0; j < NJ; j++) acc[r][j].zero();
     const int steps = (ze - zs) + NJ - 1;
-    for (int s = 0; s < steps; ++s) {
+    // With RING the plane loads in flight form a register ring: step s
+    // stages slot s mod GPF and refills that same slot with the plane GPF
+    // steps ahead.  The step loop is then unrolled by hand by GPF so slots
+    // are constants and no load in flight is ever copied (a copy waits for
+    // it at once), and the loads are unconditional (goff of an unused cell
+    // is a valid cell; a load under a branch makes the compiler drain every
+    // load before the next wait).  Without RING slot 0 is staged and the
+    // slots shift down by one per step.
+    auto step = [&](auto phc, int s) {
+        constexpr int PH = decltype(phc)::value;
         const int zp = zs + DZMIN + s;
         T *L = lds + (s & 1) * LSZ;
 #pragma unroll
         for (int k = 0; k < NL; k++)
-            if (gv[k]) L[tid + k * NT] = pl[0][k];
+            if (gv[k]) L[tid + k * NT] = pl[PH][k];
         {
             const T *pn = in + (long long)min(zp + GPF, p.n0 + R - 1) * sz;
+            if constexpr (RING) {
 #pragma unroll
-            for (int f = 0; f + 1 < GPF; f++)
+                for (int k = 0; k < NL; k++) pl[PH][k] = pn[goff[k]];
+            } else {
 #pragma unroll
-                for (int k = 0; k < NL; k++) pl[f][k] = pl[f + 1][k];
+                for (int f = 0; f + 1 < GPF; f++)
 #pragma unroll
-            for (int k = 0; k < NL; k++)
-                if (gv[k]) pl[GPF - 1][k] = pn[goff[k]];
+                    for (int k = 0; k < NL; k++) pl[f][k] = pl[f + 1][k];
+#pragma unroll
+                for (int k = 0; k < NL; k++)
+                    if (gv[k]) pl[GPF - 1][k] = pn[goff[k]];
+            }
         }
         __syncthreads();
         cf.refresh();
@@ -873,6 +896,16 @@ __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__
 #pragma unroll
             for (int j = 0; j + 1 < NJ; j++) acc[r][j] = acc[r][j + 1];
         }
+    };
+    if constexpr (RING) {
+        int s0 = 0;
+        for (; s0 + GPF <= steps; s0 += GPF)
+            static_for<GPF>([&](auto phc) { step(phc, s0 + decltype(phc)::value); });
+        static_for<GPF - 1>([&](auto phc) {
+            if (s0 + decltype(phc)::value < steps) step(phc, s0 + decltype(phc)::value);
+        });
+    } else {
+        for (int s = 0; s < steps; ++s) step(std::integral_constant<int, 0>{}, s);
     }
 }
 

@@ -81,6 +81,8 @@ typedef struct uhs_stencil {
  *           (2D stencils two rows), 3D boxes fetch staged cells three ahead
  *           instead of one;
  *           UHS_CFG_PREFETCH4: 2D stencils keep four rows in flight;
+ *           UHS_CFG_ROWRING: 2D stencils keep 2r+1 rows in flight in a
+ *           register ring (not with UHS_CFG_PREFETCH2/4);
  *           UHS_CFG_ROWS2: 3D boxes compute two rows per thread (vec must
  *           be 1, tile_y even; tile_x * tile_y / 2 threads), halving the
  *           LDS reads per output;
@@ -109,6 +111,7 @@ typedef struct uhs_stencil {
 #define UHS_CFG_ROWS2 16
 #define UHS_CFG_TEMPORAL2 32
 #define UHS_CFG_PLANES2 64
+#define UHS_CFG_ROWRING 128
 typedef struct uhs_config {
     int tile_x, tile_y, tile_z, vec, flags;
 } uhs_config;

@@ -77,6 +77,11 @@ def test_argument_validation(uhs):
     expect_code(uhs, -3, lambda: uhs.plan_kind(ext, 1, st, cfg=(64, 8, 64, 3)))
     expect_code(uhs, -3, lambda: uhs.plan_kind(ext, 1, st, cfg=(1024, 8, 64, 1)))
     expect_code(uhs, -3, lambda: uhs.plan_kind([64, 64], 1, uhs.builtin("2d1r_star"), cfg=(64, 2, 8, 1)))
+    # the row ring: 2D only, not with the shifted prefetch
+    s2 = uhs.builtin("2d1r_star")
+    assert uhs.plan_kind([64, 64], 1, s2, cfg=(64, 1, 8, 1, uhs.CFG_ROWRING)) == "win2d"
+    expect_code(uhs, -3, lambda: uhs.plan_kind([64, 64], 1, s2, cfg=(64, 1, 8, 1, uhs.CFG_ROWRING | uhs.CFG_PREFETCH2)))
+    expect_code(uhs, -3, lambda: uhs.plan_kind(ext, 1, st, cfg=(64, 8, 64, 1, uhs.CFG_ROWRING)))
     big = uhs.StencilSpec(3, [[i % 5 - 2, (i // 5) % 5 - 2, i // 25 - 2] for i in range(125)][::-1],
                           np.ones(125))
     assert uhs.plan_kind(ext, 2, big) == "gather" or True

@@ -113,7 +113,9 @@ CONFIGS = {
               (32, 16, 9, 1, 64), (32, 8, 7, 2, 65), (32, 16, 9, 1, 80), (16, 4, 3, 1, 80)],
     "win2d": [(256, 1, 64, 1), (128, 1, 5, 2), (64, 1, 17, 4), (512, 1, 9, 1), (1024, 1, 3, 2, 1),
               # two / four rows of loads in flight
-              (256, 1, 5, 1, 2), (128, 1, 3, 2, 3), (64, 1, 7, 4, 8), (512, 1, 64, 1, 9)],
+              (256, 1, 5, 1, 2), (128, 1, 3, 2, 3), (64, 1, 7, 4, 8), (512, 1, 64, 1, 9),
+              # ring of 2r+1 rows (whole ring turns plus a remainder)
+              (256, 1, 5, 1, 128), (128, 1, 3, 2, 129), (64, 1, 17, 4, 128), (512, 1, 64, 1, 129)],
 }
 
 

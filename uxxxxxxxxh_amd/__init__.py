@@ -19,6 +19,7 @@ ERRORS = {-1: "UHS_E_INVAL", -2: "UHS_E_UNSUPPORTED", -3: "UHS_E_CONFIG", -4: "U
           -5: "UHS_E_NOMEM", -6: "UHS_E_IO", -7: "UHS_E_PARSE"}
 DT_F64, DT_F32, DT_F32K = 0, 1, 2
 CFG_NT_STORE, CFG_PREFETCH2, CFG_REGTILE, CFG_PREFETCH4, CFG_ROWS2, CFG_TEMPORAL2, CFG_PLANES2 = 1, 2, 4, 8, 16, 32, 64
+CFG_ROWRING = 128
 
 
 class UhsError(RuntimeError):

@@ -123,7 +123,10 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
     }
     case FAM_WIN2D:
         return uhs_find_win2d(pl.prog, dt, c.vec,
-                              (c.flags & UHS_CFG_PREFETCH4) ? 4 : (c.flags & UHS_CFG_PREFETCH2) ? 2 : 1);
+                              (c.flags & UHS_CFG_ROWRING)     ? 0
+                              : (c.flags & UHS_CFG_PREFETCH4) ? 4
+                              : (c.flags & UHS_CFG_PREFETCH2) ? 2
+                                                              : 1);
     default: return uhs_find_gather(dt);
     }
 }
@@ -143,8 +146,11 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     };
     if (pl.family == FAM_GATHER) return UHS_OK;
     if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4 | UHS_CFG_ROWS2 |
-                    UHS_CFG_TEMPORAL2 | UHS_CFG_PLANES2))
+                    UHS_CFG_TEMPORAL2 | UHS_CFG_PLANES2 | UHS_CFG_ROWRING))
         return bad("unknown config flags");
+    if ((c.flags & UHS_CFG_ROWRING) &&
+        (pl.family != FAM_WIN2D || (c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_PREFETCH4))))
+        return bad("the row ring exists for 2D stencils, without UHS_CFG_PREFETCH2/4");
     if ((c.flags & UHS_CFG_PLANES2) &&
         (pl.family != FAM_BOX3D || (c.flags & UHS_CFG_PREFETCH2)))
         return bad("two planes of tile loads in flight exist for 3D boxes without deeper cell prefetch");
@@ -211,7 +217,7 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
     if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
     if ((c.flags & (UHS_CFG_ROWS2 | UHS_CFG_PLANES2 | UHS_CFG_REGTILE | UHS_CFG_TEMPORAL2)) && pl.family == FAM_WIN2D)
-        return bad("2D stencils take UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2 and UHS_CFG_PREFETCH4 only");
+        return bad("2D stencils take UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2/4 and UHS_CFG_ROWRING only");
     if (!uhs_pick_fn(pl, c)) return bad(std::string("no ") + family_name(pl.family) + str(" kernel compiled for vec %lld", c.vec));
     return UHS_OK;
 }
@@ -219,12 +225,12 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
 // Defaults per built-in stencil: the fastest configuration of a timed sweep
 // over the configuration space on MI355X at 512^3 / 8192^2 (FP64).
 static const uhs_config kBuiltinDefaults[12] = {
-    {1024, 1, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4}, // 2d1r_star
-    {1024, 1, 128, 2, 0},                                     // 2d2r_star
-    {512, 1, 128, 1, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4},  // 2d3r_star
-    {512, 1, 32, 1, UHS_CFG_NT_STORE},                       // 2d1r_box
-    {1024, 1, 128, 4, UHS_CFG_NT_STORE},                     // 2d2r_box
-    {512, 1, 256, 1, UHS_CFG_NT_STORE},                      // 2d3r_box
+    {256, 1, 256, 4, UHS_CFG_ROWRING},                        // 2d1r_star
+    {256, 1, 256, 4, UHS_CFG_ROWRING | UHS_CFG_NT_STORE},     // 2d2r_star
+    {512, 1, 128, 1, UHS_CFG_ROWRING | UHS_CFG_NT_STORE},     // 2d3r_star
+    {1024, 1, 256, 4, UHS_CFG_ROWRING | UHS_CFG_NT_STORE},    // 2d1r_box
+    {1024, 1, 128, 4, UHS_CFG_NT_STORE},                      // 2d2r_box
+    {256, 1, 64, 1, UHS_CFG_ROWRING},                         // 2d3r_box
     {128, 16, 256, 8, UHS_CFG_NT_STORE},                     // 3d1r_star
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d2r_star
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d3r_star

@@ -1,6 +1,6 @@
 // win2d kernel instantiations: the six 2D benchmark stencils, FP64, FP32 and
 // FP32 compensated, 1/2/4 points per thread along the unit-stride axis, one,
-// two or four rows of loads in flight.
+// two or four rows of loads in flight or a ring of 2r+1 rows (prefetch 0).
 #include "sweep_kernels.h"
 
 LaunchFn uhs_find_box3d_r1(int dtype, int vec, int deep, int planes);
@@ -22,6 +22,7 @@ template <typename T, bool K, class S, int PF> LaunchFn pick_vec(int vec)
 template <typename T, bool K, class S> LaunchFn pick_pf(int vec, int prefetch)
 {
     switch (prefetch) {
+    case 0: return pick_vec<T, K, S, 0>(vec);
     case 1: return pick_vec<T, K, S, 1>(vec);
     case 2: return pick_vec<T, K, S, 2>(vec);
     case 4: return pick_vec<T, K, S, 4>(vec);

@@ -279,7 +279,8 @@ int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext, const u
     f[15 + (depth >= 4 ? 2 : depth >= 2 ? 1 : 0)] = 1.f;
     f[18] = (c->flags & UHS_CFG_NT_STORE) ? 1.f : 0.f;
     const bool rows2 = c->flags & UHS_CFG_ROWS2, planes2 = c->flags & UHS_CFG_PLANES2;
-    f[19] = rows2 && planes2 ? 6.f : planes2 ? 5.f : rows2 ? 4.f : (c->flags & UHS_CFG_REGTILE) ? 3.f
+    f[19] = (c->flags & UHS_CFG_ROWRING) ? 7.f : rows2 && planes2 ? 6.f : planes2 ? 5.f : rows2 ? 4.f
+          : (c->flags & UHS_CFG_REGTILE) ? 3.f
           : (c->flags & UHS_CFG_PREFETCH4) ? 2.f : (c->flags & UHS_CFG_PREFETCH2) ? 1.f : 0.f;
     return UHS_OK;
 }

@@ -754,7 +754,7 @@ template <class S> constexpr bool row_grouped()
 // loads where a typical tile needs 3); for the radius-3 box, whose 343
 // coefficients already fill the scalar registers, the unrolled loop does not
 // compile (the register allocator fails).
-template <class S> constexpr bool box_plane_ring() { return S::R == 2; }
+template <class S> constexpr bool box_plane_ring() { return S::R >= 2; }
 
 template <typename T, class S, bool KAHAN, int GPF, int NL = box_cells_per_thread(S::R), class CF>
 __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__restrict__ out,
@@ -984,14 +984,24 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
     const int hlx = tid < R ? tid : TX + tid;
     const int hoff = clampi(x0 - R + hlx, -R, p.n2 + R - 1);
 
-    // nrow[f] / nh[f]: row ys - R + s + f, PF rows in flight.
-    T nrow[PF][VX], nh[PF];
+    // PF rows of loads in flight, shifted down one slot per row; or, with
+    // PF == 0, a ring of W = 2R+1 rows: step s stages slot s mod W (the
+    // step loop is already unrolled by W) and refills it with the row W
+    // ahead, so no load in flight is ever copied (a copy waits for it at
+    // once) and W rows stay in flight.  The ring's loads are unconditional
+    // (hoff is a valid cell for every thread): a load under a branch makes
+    // the compiler drain every load before the next wait.
+    constexpr bool RING = PF == 0;
+    constexpr int NS = RING ? W : PF;
+    // nrow[f] / nh[f]: row ys - R + s + f.
+    T nrow[NS][VX], nh[NS];
 #pragma unroll
-    for (int f = 0; f < PF; f++) {
+    for (int f = 0; f < NS; f++) {
         const T *pr = in + (long long)min(ys - R + f, p.n0 + R - 1) * sy;
 #pragma unroll
         for (int v = 0; v < VX; v++) nrow[f][v] = pr[xcl[v]];
-        nh[f] = hthr ? pr[hoff] : T(0);
+        if constexpr (RING) nh[f] = pr[hoff];
+        else nh[f] = hthr ? pr[hoff] : T(0);
     }
     Coefs<T, P> cf(p);
     // Window ring: w[v][r][b] holds row y of the window around point v in
@@ -1012,10 +1022,16 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
         constexpr int PH = decltype(phc)::value; // s mod W
         const int yr = ys - R + s;               // row staged at this step (slot PH)
         T *L = lds + (s & 1) * LW;
+        constexpr int SL = RING ? PH : 0;
 #pragma unroll
-        for (int v = 0; v < VX; v++) L[R + tid + v * BXT] = nrow[0][v];
-        if (hthr) L[hlx] = nh[0];
-        {
+        for (int v = 0; v < VX; v++) L[R + tid + v * BXT] = nrow[SL][v];
+        if (hthr) L[hlx] = nh[SL];
+        if constexpr (RING) {
+            const T *pr = in + (long long)min(yr + W, p.n0 + R - 1) * sy;
+#pragma unroll
+            for (int v = 0; v < VX; v++) nrow[PH][v] = pr[xcl[v]];
+            nh[PH] = pr[hoff];
+        } else {
 #pragma unroll
             for (int f = 0; f + 1 < PF; f++) {
 #pragma unroll
@@ -1049,11 +1065,22 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
             }
         }
     };
-    for (int s0 = 0; s0 < steps; s0 += W)
-        static_for<W>([&](auto phc) {
+    if constexpr (RING) {
+        // Whole ring turns in the loop (no branch inside it), the rest after.
+        int s0 = 0;
+        for (; s0 + W <= steps; s0 += W)
+            static_for<W>([&](auto phc) { step(phc, s0 + decltype(phc)::value); });
+        static_for<W - 1>([&](auto phc) {
             const int s = s0 + decltype(phc)::value;
             if (s < steps) step(phc, s);
         });
+    } else {
+        for (int s0 = 0; s0 < steps; s0 += W)
+            static_for<W>([&](auto phc) {
+                const int s = s0 + decltype(phc)::value;
+                if (s < steps) step(phc, s);
+            });
+    }
 }
 
 // --------------------------------------------------------------------------

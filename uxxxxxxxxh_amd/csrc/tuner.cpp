@@ -128,7 +128,8 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
     const std::vector<int> vecs = pl.family == FAM_STAR3D ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1, 2, 4};
     const bool d3 = pl.args.dims == 3;
     std::vector<int> flagset{0, UHS_CFG_NT_STORE, UHS_CFG_PREFETCH2, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2,
-                             UHS_CFG_PREFETCH4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4};
+                             UHS_CFG_PREFETCH4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH4,
+                             UHS_CFG_ROWRING, UHS_CFG_NT_STORE | UHS_CFG_ROWRING};
     if (pl.family == FAM_BOX3D) {
         flagset.clear();
         for (int base : {0, UHS_CFG_PREFETCH2, UHS_CFG_ROWS2, UHS_CFG_PLANES2, UHS_CFG_ROWS2 | UHS_CFG_PLANES2})

@@ -185,7 +185,16 @@ template <typename T, int P> struct Coefs {
     // motion so the scalar loads are issued inside the step.
     __device__ __forceinline__ void refresh()
     {
-        if constexpr (P > kHoistPoints) asm volatile("" : "+s"(ptr));
+        if constexpr (P > kHoistPoints) {
+            // Through readfirstlane first: the value is then uniform by
+            // construction, so the scalar-register operand never needs a
+            // vector-to-scalar copy (which the backend cannot emit).
+            const unsigned long long v = (unsigned long long)ptr;
+            unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+            unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+            asm volatile("" : "+s"(lo), "+s"(hi));
+            ptr = (kptr)(((unsigned long long)hi << 32) | lo);
+        }
     }
     template <int t> __device__ __forceinline__ T get() const
     {

@@ -119,14 +119,17 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
     constexpr int M = 2 * R + 1 + PF, U = (M % 2 == 0) ? M : 2 * M;
     T rq[VX][M], rh[2][NH];
     const bool nts = p.nts;
-#pragma unroll
-    for (int v = 0; v < VX; v++)
-#pragma unroll
-        for (int j = 0; j < 2 * R + PF; j++)
-            rq[v][j] = in[(long long)min(zs - R + j, p.n0 + R - 1) * sz + coff[v]];
+    // Halo first, then the own columns plane by plane: the order the loop
+    // consumes them in, so the waits on entering the loop count the loads
+    // still in flight instead of draining them.
 #pragma unroll
     for (int k = 0; k < NH; k++)
         if (FULL || hl[k] >= 0) rh[0][k] = in[(long long)zs * sz + hoff[k]];
+#pragma unroll
+    for (int j = 0; j < 2 * R + PF; j++)
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+            rq[v][j] = in[(long long)min(zs - R + j, p.n0 + R - 1) * sz + coff[v]];
 
     auto step = [&](auto phc, int z) {
         constexpr int PH = decltype(phc)::value; // (z - zs) mod U
@@ -171,11 +174,15 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
             if (FULL || act[v]) store_out(oz + coff[v], a.value(), nts);
         }
     };
-    for (int z0 = zs; z0 < ze; z0 += U)
-        static_for<U>([&](auto phc) {
-            const int z = z0 + decltype(phc)::value;
-            if (z < ze) step(phc, z);
-        });
+    // Whole turns of the ring in the loop, with no branch inside it (a
+    // branch makes the compiler's wait counts conservative), the rest after.
+    int z0 = zs;
+    for (; z0 + U <= ze; z0 += U)
+        static_for<U>([&](auto phc) { step(phc, z0 + decltype(phc)::value); });
+    static_for<U - 1>([&](auto phc) {
+        const int z = z0 + decltype(phc)::value;
+        if (z < ze) step(phc, z);
+    });
 }
 
 // The same tile with shifting register queues (radius 2 and 3, where the

@@ -232,7 +232,7 @@ static const uhs_config kBuiltinDefaults[12] = {
     {1024, 1, 128, 4, UHS_CFG_NT_STORE},                      // 2d2r_box
     {256, 1, 64, 1, UHS_CFG_ROWRING},                         // 2d3r_box
     {128, 16, 256, 8, UHS_CFG_NT_STORE},                     // 3d1r_star
-    {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d2r_star
+    {128, 16, 256, 8, UHS_CFG_NT_STORE},                     // 3d2r_star
     {128, 16, 256, 4, UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2}, // 3d3r_star
     {64, 16, 128, 1, UHS_CFG_ROWS2 | UHS_CFG_PLANES2},       // 3d1r_box
     {32, 16, 64, 1, UHS_CFG_ROWS2 | UHS_CFG_PLANES2 | UHS_CFG_NT_STORE}, // 3d2r_box

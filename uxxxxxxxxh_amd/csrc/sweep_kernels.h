@@ -185,8 +185,9 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
     });
 }
 
-// The same tile with shifting register queues (radius 2 and 3, where the
-// ring form's larger unrolled body measured slower).
+// The same tile with shifting register queues (radius 3, where the ring
+// form's larger unrolled body measured slower: 287 against 295 GSt/s at
+// 512^3; at radius 2 the ring form is the faster, 322 against 310).
 template <typename T, class S, int VX, bool KAHAN, int PF, class CF>
 __device__ __forceinline__ void star3d_tile_shift(const T *__restrict__ in, T *__restrict__ out,
                                             const TileGeom &p, int x0, int y0, int zs, int ze,
@@ -316,7 +317,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
     const int zs = chunk * p.ZC;
     const TileGeom g{p.n0, p.n1, p.n2, p.sy, p.sz, p.TX, p.TY, p.BXT, (p.flags & UHS_CFG_NT_STORE) != 0};
     Coefs<T, S::P> cf(p);
-    if constexpr (S::R == 1) {
+    if constexpr (S::R <= 2) {
         const int x0 = tile_x * p.TX, y0 = tile_y * p.TY, ze = min(zs + p.ZC, p.n0);
         T *lds = reinterpret_cast<T *>(smem_raw);
         if (x0 + p.TX <= p.n2 && y0 + p.TY <= p.n1)

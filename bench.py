@@ -376,6 +376,8 @@ def main():
     ap.add_argument("--temporal-only", action="store_true",
                     help="only the two-steps-per-launch sweep (its traffic-counter run)")
     ap.add_argument("--no-programs", action="store_true")
+    ap.add_argument("--no-shard", action="store_true",
+                    help="skip one GPU's share of the slab decomposition")
     ap.add_argument("--programs-only", nargs="*", default=None, metavar="NAME",
                     help="only the fused multi-kernel programs (all, or those named)")
     ap.add_argument("--copy-only", action="store_true",
@@ -470,8 +472,9 @@ def main():
         log("temporal2", out["temporal2"])
     if not args.no_programs:
         out["programs"] = programs(u)
-    out["shard_share"] = shard_share(u, T)
-    log("shard share", out["shard_share"])
+    if not args.no_shard:
+        out["shard_share"] = shard_share(u, T)
+        log("shard share", out["shard_share"])
     if not args.no_batch:
         out["batched"] = batched(u)
         log("batched", out["batched"])

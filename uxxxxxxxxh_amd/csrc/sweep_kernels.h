@@ -757,12 +757,12 @@ template <class S> constexpr bool row_grouped()
 // from LDS once and applied to the pending sums of both rows, which halves
 // the LDS reads per output -- the boxes' other bottleneck next to FP64 FMA.
 // Whether box3d_tile_rows2 keeps its plane loads in a register ring (see
-// there).  Measured on MI355X at 512^3: the radius-2 box gains 10%; the
-// radius-1 box loses 20% (its 8 cells per thread become 8 unconditional
-// loads where a typical tile needs 3); for the radius-3 box, whose 343
-// coefficients already fill the scalar registers, the unrolled loop does not
-// compile (the register allocator fails).
-template <class S> constexpr bool box_plane_ring() { return S::R >= 2; }
+// there).  Measured on MI355X at 512^3: the radius-2 box gains 10% and the
+// radius-3 box 4%; the radius-1 box loses 20% (its 8 cells per thread become
+// 8 unconditional loads where a typical tile needs 3).  The compensated
+// radius-3 form is left out: its unrolled loop moves the running sums to
+// scratch memory and runs 60x slower.
+template <class S, bool KAHAN> constexpr bool box_plane_ring() { return S::R == 2 || (S::R == 3 && !KAHAN); }
 
 template <typename T, class S, bool KAHAN, int GPF, int NL = box_cells_per_thread(S::R), class CF>
 __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__restrict__ out,
@@ -798,7 +798,7 @@ __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__
         const int ly = gv[k] ? c / LW : 0, lx = gv[k] ? c % LW : 0;
         goff[k] = clampi(y0 - R + ly, -R, p.n1 + R - 1) * sy + clampi(x0 - R + lx, -R, p.n2 + R - 1);
     }
-    constexpr bool RING = GPF > 1 && box_plane_ring<S>();
+    constexpr bool RING = GPF > 1 && box_plane_ring<S, KAHAN>();
     T pl[GPF][NL];
 #pragma unroll
     for (int f = 0; f < GPF; f++) {

@@ -191,6 +191,21 @@ def batched(u, T=50, probe_T=8):
     ms_batch = timed(b.run)
     info = b.info()
     b.close()
+    # Splitting the batch over 8 GPUs (design only on this one-GPU box):
+    # longest-processing-time-first packing on each grid's measured plain
+    # sweep time, and on the library's estimate (points x work per point)
+    # judged by the measured times.
+    cost = []
+    for A, B, ext, h, st in grids:
+        cost.append(timed(lambda: u.sweep_f64(A, B, ext, h, st, 4), reps=1))
+    gpu_m, load_m = u.batch_partition(grids, 8, cost)
+    gpu_e, _ = u.batch_partition(grids, 8)
+    load_e = [sum(c for c, g in zip(cost, gpu_e) if g == k) for k in range(8)]
+    mean = sum(cost) / 8
+    partition = {"gpus": 8, "cost": "each grid's plain sweep, 4 steps (ms)",
+                 "max_over_mean_measured_costs": round(max(load_m) / mean, 4),
+                 "max_over_mean_estimated_costs": round(max(load_e) / mean, 4),
+                 "grids_per_gpu": [gpu_m.count(k) for k in range(8)]}
     for A, B, _, _, _ in grids:
         A.close()
         B.close()
@@ -202,7 +217,7 @@ def batched(u, T=50, probe_T=8):
             "config": {"star_depth": best.star_depth, "star_tile": [best.star_tile_x, best.star_tile_y, best.star_tile_z],
                        "box_tile": [best.box_tile_x, best.box_tile_y, best.box_tile_z], "flags": best.flags},
             "tuning_candidates": len(cands), "tuning_s": round(t_tune, 2), "launches": info["launches"],
-            "tasks": info["tasks"]}
+            "tasks": info["tasks"], "partition": partition}
 
 
 # The reference's multi-kernel and one-sided example programs

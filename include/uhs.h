@@ -235,6 +235,19 @@ int uhs_batch_default_config(uhs_batch_config *out);
 /* The unbatched baseline: uhs_sweep_f64 on each grid in turn, same stream. */
 int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps,
                        uhs_stream_t stream);
+/* Splits a batch over `ngpus` GPUs, one batch per GPU and no exchange between
+ * grids: longest-processing-time-first bin packing (each grid, in order of
+ * decreasing cost, goes to the GPU with the least load so far; ties go to
+ * the lower GPU index and equal costs keep grid order), within 4/3 of the
+ * best split.  cost[i] is grid i's cost in any unit (measured or predicted
+ * time); cost NULL estimates it, for the built-in 3D stencils, as interior
+ * points x max(1, npoints / 64) (memory-bound up to a few tens of terms,
+ * then proportional to the terms; fitted to the measured FP64 rates).
+ * Writes the GPU of every grid to gpu_of[n] and every GPU's
+ * summed cost to load[ngpus] (NULL: not written).  Host only; no device
+ * call. */
+int uhs_batch_partition(const uhs_grid_desc *grids, int n, int ngpus, const double *cost,
+                        int *gpu_of, double *load);
 
 /* ---- fused multi-kernel programs ---------------------------------------- */
 

@@ -134,6 +134,8 @@ _SIGS = {
                             ctypes.POINTER(ctypes.c_size_t)]),
     "uhs_batch_default_config": (_I, [ctypes.POINTER(BatchConfig)]),
     "uhs_sweep_each_f64": (_I, [ctypes.POINTER(GridDesc), _I, _I, _P]),
+    "uhs_batch_partition": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
     "uhs_tune_sweep": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, _I, ctypes.c_char_p,
                             ctypes.POINTER(TuneResult), _P]),
     "uhs_tune_sweep_model": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, _P, _I, _I, ctypes.c_char_p,
@@ -529,6 +531,24 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_partition(grids, ngpus, cost=None):
+    """Longest-processing-time-first split of a batch over ngpus GPUs
+    (uhs_batch_partition).  grids: (ext, halo, StencilSpec) or (A, B, ext,
+    halo, StencilSpec) tuples (no device memory is touched); cost: per-grid
+    costs, or None to estimate them.  Returns (gpu of each grid, load of each
+    GPU)."""
+    n = len(grids)
+    arr = (GridDesc * n)()
+    for i, g in enumerate(grids):
+        ext, halo, st = g[-3], g[-2], g[-1]
+        arr[i] = GridDesc(None, None, 3, (ctypes.c_int64 * 3)(*ext), halo, st.c)
+    gpu = (ctypes.c_int * n)()
+    load = (ctypes.c_double * max(1, ngpus))()
+    c = (ctypes.c_double * n)(*cost) if cost is not None else None
+    check(lib().uhs_batch_partition(arr, n, ngpus, c, gpu, load))
+    return list(gpu), list(load)[:ngpus]
 
 
 def batch_default_config():

@@ -8,7 +8,7 @@
 //   axis 1: the lane's own rows, plus R rows above and below the band that
 //           every lane loads for the current plane (L2-resident: they are
 //           the neighbouring bands' rows),
-//   axis 2: the neighbouring lanes, by cross-lane shuffles.
+//   axis 2: the neighbouring lanes, by DPP wavefront shifts.
 // The R lanes at each end of the wave only load (they are the x halo of the
 // 64 - 2R columns the wave computes), so bands overlap by 2R columns.
 // Terms are evaluated in DSL order with the same product-then-fma chain as
@@ -17,8 +17,26 @@
 
 namespace {
 
-template <typename T> __device__ __forceinline__ T lane_down(T v, int d) { return __shfl_down(v, d, 64); }
-template <typename T> __device__ __forceinline__ T lane_up(T v, int d) { return __shfl_up(v, d, 64); }
+// Whole-wavefront shifts by one lane with DPP (a VALU move, no LDS trip):
+// wave_shl:1 gives lane l the value of lane l+1, wave_shr:1 that of lane
+// l-1.  The lane shifted in at the wave's end reads 0; those R lanes at each
+// end only load and never store.
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+template <int CTRL> __device__ __forceinline__ int dpp32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_shift(float v)
+{
+    return __builtin_bit_cast(float, dpp32<CTRL>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_shift(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp32<CTRL>((int)(unsigned)u);
+    const unsigned hi = (unsigned)dpp32<CTRL>((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
 
 template <typename T, class S, int TYR, bool KAHAN>
 __global__ void __launch_bounds__(kMaxThreads) k_star3d_rt(KParams<T, S::P> p)
@@ -83,14 +101,22 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d_rt(KParams<T, S::P> p)
         T *oz = out + (long long)z * sz;
         static_for<TYR>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
+            // The row's x neighbours at distance 1..R, one DPP shift each.
+            T dn[R + 1], up[R + 1];
+            dn[0] = up[0] = q[R][i];
+            static_for<R>([&](auto dc) {
+                constexpr int d = decltype(dc)::value + 1;
+                dn[d] = dpp_shift<kDppWaveShl1>(dn[d - 1]);
+                up[d] = dpp_shift<kDppWaveShr1>(up[d - 1]);
+            });
             Acc<T, KAHAN> a;
             static_for<P>([&](auto tc) {
                 constexpr int t = decltype(tc)::value;
                 constexpr int dz = S::OFF[t][0], dy = S::OFF[t][1], dx = S::OFF[t][2];
                 static_assert((dx != 0) + (dy != 0) + (dz != 0) <= 1, "star3d_rt needs axis-aligned terms");
                 T val;
-                if constexpr (dx > 0) val = lane_down(q[R][i], dx);
-                else if constexpr (dx < 0) val = lane_up(q[R][i], -dx);
+                if constexpr (dx > 0) val = dn[dx];
+                else if constexpr (dx < 0) val = up[-dx];
                 else if constexpr (dy != 0 && i + dy < 0) val = hy[R + i + dy];        // row above the band
                 else if constexpr (dy != 0 && i + dy >= TYR) val = hy[R + i + dy - TYR]; // row below the band
                 else if constexpr (dy != 0) val = q[R][i + dy];

@@ -391,6 +391,8 @@ def main():
     ap.add_argument("--temporal-only", action="store_true",
                     help="only the two-steps-per-launch sweep (its traffic-counter run)")
     ap.add_argument("--no-programs", action="store_true")
+    ap.add_argument("--shape-only", default=None, metavar="CASE",
+                    help="only one benchmark stencil at its per-shape size (copy probe included)")
     ap.add_argument("--no-shard", action="store_true",
                     help="skip one GPU's share of the slab decomposition")
     ap.add_argument("--programs-only", nargs="*", default=None, metavar="NAME",
@@ -407,6 +409,15 @@ def main():
     u.lib()
     if args.copy_only:
         print(json.dumps({"copy_GBps": round(copy_bandwidth(u), 1)}))
+        return
+    if args.shape_only:
+        # One benchmark stencil at its per-shape size (profiling runs).
+        case = args.shape_only
+        e = [8192, 8192] if case.startswith("2d") else [512, 512, 512]
+        ms_s, ms_w, c = time_sweeps(u, case, e, args.timesteps, args.steps, args.warmup)
+        print(json.dumps({"case": case, "GStencils_s": round(float(np.prod(e)) / (ms_w * 1e-3) / 1e9, 2),
+                          "ms_per_sweep": round(ms_w, 5), "tile": list(c.tuple()),
+                          "copy_GBps": round(copy_bandwidth(u, reps=2), 1)}))
         return
     if args.programs_only is not None:
         print(json.dumps(programs(u, names=args.programs_only)))

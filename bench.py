@@ -449,6 +449,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        # Context, not a baseline on this hardware: the reference's own
+        # published per-sweep time for this stencil and size on NVIDIA A100
+        # (UHStencil's figure data, 1.676 ms per 512^3 sweep).
+        "reference_other_hw": {"hardware": "NVIDIA A100", "GStencils_s": 80.1},
         "dtype": "f64",
         "data": "synthetic (seeded U[0,1) interior, zero halo, symmetric normalised coefficients)",
         "config": {"workload": "3d1r_star FP64 %dx%dx%d interior, halo 1, T=%d per step" % (n, n, n, T),

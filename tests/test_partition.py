@@ -82,3 +82,8 @@ def test_argument_errors(uhs):
         uhs.batch_partition([([8, 8, 8], 1, odd)], 2)
     assert e.value.code == -2
     assert uhs.batch_partition([([8, 8, 8], 1, odd)], 2, [1.0]) == ([0], [1.0, 0.0])
+    # a grid without offsets cannot be estimated (rejected, not dereferenced)
+    arr = (uhs.GridDesc * 1)()
+    arr[0] = uhs.GridDesc(None, None, 3, (uhs.ctypes.c_int64 * 3)(8, 8, 8), 1, uhs.Stencil(3, 7, None, None))
+    gpu = (uhs.ctypes.c_int * 1)()
+    assert uhs.lib().uhs_batch_partition(arr, 1, 2, None, gpu, None) == -2

@@ -129,8 +129,9 @@ int uhs_batch_partition(const uhs_grid_desc *grids, int n, int ngpus, const doub
             continue;
         }
         const uhs_grid_desc &d = grids[i];
-        const int prog = d.dims == 3 && d.stencil.dims == 3 ? uhs_match_builtin(3, d.stencil.npoints, d.stencil.offsets)
-                                                            : -1;
+        const int prog = d.dims == 3 && d.stencil.dims == 3 && d.stencil.offsets
+                             ? uhs_match_builtin(3, d.stencil.npoints, d.stencil.offsets)
+                             : -1;
         if (prog < 0) return berr(UHS_E_UNSUPPORTED, "cost estimates exist for the built-in 3D stencils only");
         double pts = 1.0;
         for (int k = 0; k < 3; k++) {
@@ -197,7 +198,7 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
     for (int i = 0; i < n; i++) {
         const uhs_grid_desc &d = grids[i];
         char msg[160];
-        if (d.dims != 3 || d.stencil.dims != 3 || !d.A || !d.B || d.A == d.B) {
+        if (d.dims != 3 || d.stencil.dims != 3 || !d.A || !d.B || d.A == d.B || !d.stencil.offsets) {
             snprintf(msg, sizeof msg, "grid %d: the batch takes 3D FP64 grids with distinct A and B", i);
             return berr(UHS_E_INVAL, msg);
         }

@@ -57,7 +57,13 @@ def test_plan_kinds_and_default_configs(uhs, case):
     kind = uhs.plan_kind(ext, s["radius"], st)
     assert kind == ("win2d" if s["dims"] == 2 else ("star3d" if s["kind"] == "star" else "box3d"))
     for dt in (uhs.DT_F64, uhs.DT_F32, uhs.DT_F32K):
-        uhs.default_config(ext, s["radius"], st, dt)
+        c = uhs.default_config(ext, s["radius"], st, dt)
+        # The measured default is compiled for every precision: the
+        # fallback for a configuration without a kernel (vec 1, no flags)
+        # is never taken at the benchmark sizes (no table entry is vec 1
+        # without flags).
+        assert uhs.plan_kind(ext, s["radius"], st, dt, cfg=c.tuple()) == kind
+        assert c.vec > 1 or c.flags != 0, (case, dt, c.tuple())
     small = [5, 7] if s["dims"] == 2 else [3, 5, 7]
     uhs.default_config(small, s["radius"], st)
 

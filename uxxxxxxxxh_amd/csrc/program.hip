@@ -739,7 +739,10 @@ int default_tiles(const Checked &c, int d, int tiles[][3])
     const long long rc = c.in.hi[2] - c.in.lo[2];
     int n = 0;
     if (d == 2) {
-        for (int w : {1024, 512, 256})
+        // 512-cell rows first: the reference's nested example times the same
+        // with 512 x 16 as with 1024 x 32 tiles, its one-sided example 11%
+        // faster.
+        for (int w : {512, 1024, 256})
             if (w - rc >= 64) { tiles[n][0] = (int)(w - rc); tiles[n][1] = w == 1024 ? 32 : 16; tiles[n][2] = 1; n++; }
     } else {
         // Narrower rows only when the wider ones leave no room in the LDS.

@@ -110,7 +110,7 @@ def shard_share(u, T=100, gpus=8):
 def fma_rate(u, n=256 * 32 * 256, iters=2048, reps=5):
     """Rate of the library's FP64 FMA probe (32 independent chains per
     thread, no memory traffic), in TFLOP/s at 2 flops per FMA: the FP64
-    vector rate this box reaches, for the FP64-ALU-bound stencils."""
+    vector rate this device reaches, for the FP64-ALU-bound stencils."""
     out = u.DeviceArray([n], np.float64)
     L = u.lib()
     x, y = 1.0 - 2.0 ** -20, 2.0 ** -21
@@ -129,7 +129,7 @@ def fma_rate(u, n=256 * 32 * 256, iters=2048, reps=5):
 
 def copy_bandwidth(u, nbytes=2 << 30, reps=10):
     """Rate of the library's streaming copy kernel (read + write bytes / s):
-    the HBM bandwidth this box reaches, for context next to the roofline."""
+    the HBM bandwidth this device reaches, for context next to the roofline."""
     n = nbytes // 8
     a = u.DeviceArray([n], np.float64)
     b = u.DeviceArray([n], np.float64)
@@ -191,7 +191,7 @@ def batched(u, T=50, probe_T=8):
     ms_batch = timed(b.run)
     info = b.info()
     b.close()
-    # Splitting the batch over 8 GPUs (design only on this one-GPU box):
+    # Splitting the batch over 8 GPUs (a model; this run uses one GPU):
     # longest-processing-time-first packing on each grid's measured plain
     # sweep time, and on the library's estimate (points x work per point)
     # judged by the measured times.
@@ -490,7 +490,7 @@ def main():
                          "roof_GStencils_s": round(roof, 1), "roof_frac": round(g / roof, 4),
                          "tile": [c.tile_x, c.tile_y, c.tile_z, c.vec, c.flags]}
             if "measured_copy_GBps" in out and "measured_fp64_TFLOPs" in out:
-                # The same roof from the rates this box reaches (copy kernel,
+                # The same roof from the rates this device reaches (copy kernel,
                 # FMA probe) instead of the published peaks.
                 mroof = min(out["measured_copy_GBps"] / 16.0, out["measured_fp64_TFLOPs"] * 1e3 / flops)
                 per[case]["measured_roof_GStencils_s"] = round(mroof, 1)

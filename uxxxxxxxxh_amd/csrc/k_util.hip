@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *partial, int 
 }
 
 // Streaming copy, one element per thread: the reference point for the HBM
-// bandwidth this box reaches (and the calibration target of the traffic
+// bandwidth this device reaches (and the calibration target of the traffic
 // counters: it reads and writes exactly n * 8 bytes).
 __global__ void __launch_bounds__(256) k_copy(const double *__restrict__ a, double *__restrict__ b, long long n)
 {

@@ -62,27 +62,24 @@ def time_sweeps(u, case, ext, T, steps, warmup, seed=1, cfg=None):
     return ms / steps, ms / (steps * T), cfg
 
 
-# Two steps per launch (temporal blocking), the fastest configuration of a
-# timed sweep at 512^3: 62x6 output tile (64x8 with the ghost zone), 2 columns
-# per thread, 128-plane slabs.
-TEMPORAL2_CFG = (62, 6, 128, 2, 32)
-
-
-def temporal2(u, n, T):
-    """The same workload with UHS_CFG_TEMPORAL2: GStencils/s, and the DRAM
-    bytes per two-step launch measured by the traffic counters (committed
-    summary), beside the algorithmic 16 B per update."""
+def one_step(u, n, T):
+    """The same workload with one launch per timestep (the one-step kernel
+    and its default tile, as the reference runs it): GStencils/s, its
+    fraction of 8 TB/s at 16 B per update, and the DRAM bytes per launch
+    measured by the traffic counters (committed summary)."""
     ext = [n, n, n]
-    ms_step, ms_sweep, cfg = time_sweeps(u, "3d1r_star", ext, T, 2, 1, cfg=TEMPORAL2_CFG)
-    upd = float(n) ** 3
-    g = upd / (ms_sweep * 1e-3) / 1e9
-    out = {"workload": "3d1r_star FP64 %d^3, T=%d, two steps per launch" % (n, T),
-           "config": list(TEMPORAL2_CFG), "GStencils_s": round(g, 2),
+    st = u.builtin("3d1r_star")
+    cfg = u.one_step_config(ext, st.radius, st)
+    ms_step, ms_sweep, _ = time_sweeps(u, "3d1r_star", ext, T, 2, 1, cfg=cfg)
+    g = float(n) ** 3 / (ms_sweep * 1e-3) / 1e9
+    out = {"workload": "3d1r_star FP64 %d^3, T=%d, one step per launch" % (n, T),
+           "config": list(cfg.tuple()), "kernel": kernel_name(cfg), "GStencils_s": round(g, 2),
+           "avg_launch_ms": round(ms_sweep, 5),
            "algorithmic_GBps_at_16B": round(16.0 * g, 1), "frac_of_8TBps_at_16B": round(16.0 * g / PEAK_HBM_GBPS, 4)}
-    tr = load_traffic("3d1r_star_tb2", ext)
+    tr = load_traffic("3d1r_star", ext)
     if tr:
         out["dram_bytes_per_launch"] = tr["bytes_per_launch"]
-        out["dram_GBps"] = round(tr["bytes_per_launch"] / (2 * ms_sweep * 1e-3) / 1e9, 1)
+        out["dram_GBps"] = round(tr["bytes_per_launch"] / (ms_sweep * 1e-3) / 1e9, 1)
     return out
 
 
@@ -359,6 +356,8 @@ def cpu_baseline(ext, case="3d1r_star", budget_s=10.0):
 
 
 def kernel_name(cfg):
+    if cfg.flags & 32:
+        return "k_star3d_tb2<double,Prog_3d1r_star,%d,false,%d>" % (cfg.vec, 2 if cfg.flags & 2 else 1)
     if cfg.flags & 4:
         return "k_star3d_rt<double,Prog_3d1r_star,%d,false>" % cfg.vec
     nt = cfg.tile_x // cfg.vec * cfg.tile_y
@@ -376,6 +375,20 @@ def load_traffic(case, ext):
     return None
 
 
+def probes(u, out):
+    """The copy-bandwidth and FP64 FMA probes: this device's reachable rates."""
+    try:
+        out["measured_copy_GBps"] = round(copy_bandwidth(u), 1)
+        log("device copy bandwidth %.0f GB/s" % out["measured_copy_GBps"])
+    except Exception as e:  # context only
+        log("copy bandwidth probe failed:", e)
+    try:
+        out["measured_fp64_TFLOPs"] = round(fma_rate(u), 2)
+        log("device FP64 FMA rate %.1f TFLOP/s" % out["measured_fp64_TFLOPs"])
+    except Exception as e:  # context only
+        log("FMA rate probe failed:", e)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -387,9 +400,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--no-temporal", action="store_true")
-    ap.add_argument("--temporal-only", action="store_true",
-                    help="only the two-steps-per-launch sweep (its traffic-counter run)")
+    ap.add_argument("--no-one-step", action="store_true")
+    ap.add_argument("--one-step-only", action="store_true",
+                    help="only the one-step-per-launch sweep (its traffic-counter run)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline sweep (profiling runs: no probes, legs or CPU baseline)")
     ap.add_argument("--no-programs", action="store_true")
     ap.add_argument("--shape-only", default=None, metavar="CASE",
                     help="only one benchmark stencil at its per-shape size (copy probe included)")
@@ -422,9 +437,13 @@ def main():
     if args.programs_only is not None:
         print(json.dumps(programs(u, names=args.programs_only)))
         return
-    if args.temporal_only:
-        print(json.dumps(temporal2(u, args.size, args.timesteps)))
+    if args.one_step_only:
+        print(json.dumps(one_step(u, args.size, args.timesteps)))
         return
+    if args.headline_only:
+        for k in ("no_shapes", "no_one_step", "no_programs", "no_shard", "no_batch", "no_fp32", "no_cpu",
+                  "no_probes"):
+            setattr(args, k, True)
     name = __import__("ctypes").create_string_buffer(256)
     u.check(u.lib().uhs_device_name(name, 256))
     log("device:", name.value.decode())
@@ -435,9 +454,14 @@ def main():
     ms_step, ms_sweep, cfg = time_sweeps(u, "3d1r_star", ext, T, args.steps, args.warmup)
     updates = float(n) ** 3
     gst = updates * T / (ms_step * 1e-3) / 1e9
-    achieved = 16.0 * updates / (ms_sweep * 1e-3) / 1e9
-    log("3d1r_star %d^3: %.3f ms/sweep, %.1f GSt/s, %.0f GB/s, cfg %s" % (n, ms_sweep, gst, achieved, cfg))
-    traffic = load_traffic("3d1r_star", ext)
+    # The default advances two steps per launch (UHS_CFG_TEMPORAL2) at this
+    # size; T = 100 is even, so every launch is a two-step one.
+    per_launch = 2 if cfg.flags & 32 else 1
+    ms_launch = ms_sweep * per_launch
+    alg_bytes = 16.0 * updates * per_launch
+    achieved = alg_bytes / (ms_launch * 1e-3) / 1e9
+    log("3d1r_star %d^3: %.3f ms/launch, %.1f GSt/s, %.0f GB/s, cfg %s" % (n, ms_launch, gst, achieved, cfg))
+    traffic = load_traffic("3d1r_star_tb2" if per_launch == 2 else "3d1r_star", ext)
     out = {
         "metric": "GStencils/s (FP64 3d1r_star 512^3 sweep, T=100)",
         "value": round(gst, 3),
@@ -461,19 +485,12 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "kernel": kernel_name(cfg),
-                     "algorithmic_bytes_per_launch": int(16 * updates),
-                     "avg_launch_ms": round(ms_sweep, 5)},
+                     "steps_per_launch": per_launch,
+                     "algorithmic_bytes_per_launch": int(alg_bytes),
+                     "avg_launch_ms": round(ms_launch, 5)},
     }
-    try:
-        out["measured_copy_GBps"] = round(copy_bandwidth(u), 1)
-        log("device copy bandwidth %.0f GB/s" % out["measured_copy_GBps"])
-    except Exception as e:  # context only
-        log("copy bandwidth probe failed:", e)
-    try:
-        out["measured_fp64_TFLOPs"] = round(fma_rate(u), 2)
-        log("device FP64 FMA rate %.1f TFLOP/s" % out["measured_fp64_TFLOPs"])
-    except Exception as e:  # context only
-        log("FMA rate probe failed:", e)
+    if not getattr(args, "no_probes", False):
+        probes(u, out)
     if not args.no_shapes:
         per = {}
         for case in u.builtin_names():
@@ -497,9 +514,9 @@ def main():
                 per[case]["measured_roof_frac"] = round(g / mroof, 4)
             log("%-10s %8.2f GSt/s  %.3f ms/sweep  roof %.0f  frac %.3f" % (case, g, ms_w, roof, g / roof))
         out["per_shape"] = per
-    if not args.no_temporal:
-        out["temporal2"] = temporal2(u, n, T)
-        log("temporal2", out["temporal2"])
+    if not args.no_one_step:
+        out["one_step"] = one_step(u, n, T)
+        log("one step per launch", out["one_step"])
     if not args.no_programs:
         out["programs"] = programs(u)
     if not args.no_shard:

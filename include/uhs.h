@@ -119,7 +119,9 @@ typedef struct uhs_config {
 /* ---- the sweep ---------------------------------------------------------- */
 
 /* FP64 sweep, T timesteps.  ext[dims] = interior extents, halo >= the
- * stencil radius.  cfg NULL = the tuned (or default) configuration. */
+ * stencil radius.  cfg NULL = the tuned (or default) configuration; the
+ * FP64 default of the 7-point star (3d1r_star) advances two steps per
+ * launch (UHS_CFG_TEMPORAL2), with results bitwise those of single sweeps. */
 int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
                   const uhs_stencil *st, int timesteps, const uhs_config *cfg,
                   uhs_stream_t stream);
@@ -144,6 +146,13 @@ int uhs_plan_kind(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
 /* The configuration used when cfg is NULL. */
 int uhs_default_config(int dims, const int64_t *ext, int halo,
                        const uhs_stencil *st, int dtype, uhs_config *out);
+
+/* The default configuration with one launch per timestep: the one-step
+ * kernel and tile that the leading single steps of a two-step default use,
+ * and that uhs_sweep_each_f64 runs.  Equals uhs_default_config's result for
+ * every stencil whose default is one step per launch. */
+int uhs_one_step_config(int dims, const int64_t *ext, int halo,
+                        const uhs_stencil *st, int dtype, uhs_config *out);
 
 /* ---- the 12 benchmark stencils ------------------------------------------ */
 
@@ -232,7 +241,8 @@ int uhs_batch_destroy(uhs_batch_t b);
 int uhs_batch_info(uhs_batch_t b, int *launches, long long *tasks,
                    int *max_workgroups, size_t *max_lds_bytes);
 int uhs_batch_default_config(uhs_batch_config *out);
-/* The unbatched baseline: uhs_sweep_f64 on each grid in turn, same stream. */
+/* The unbatched baseline: uhs_sweep_f64 on each grid in turn, same stream,
+ * one launch per timestep (the one-step kernel and its default tile). */
 int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps,
                        uhs_stream_t stream);
 /* Splits a batch over `ngpus` GPUs, one batch per GPU and no exchange between

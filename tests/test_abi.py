@@ -68,6 +68,30 @@ def test_plan_kinds_and_default_configs(uhs, case):
     uhs.default_config(small, s["radius"], st)
 
 
+@pytest.mark.parametrize("case", list(SHAPES))
+def test_one_step_configs(uhs, case):
+    """Only the FP64 7-point star defaults to two steps per launch; its
+    one-step configuration (leading steps, unbatched baseline) is the
+    one-step default tile, and every other default is its own one-step
+    configuration."""
+    s = SHAPES[case]
+    st = uhs.builtin(case)
+    for ext in ([8192, 8192] if s["dims"] == 2 else [512, 512, 512],
+                [64, 96] if s["dims"] == 2 else [64, 96, 128]):
+        for dt in (uhs.DT_F64, uhs.DT_F32, uhs.DT_F32K):
+            d = uhs.default_config(ext, s["radius"], st, dt)
+            o = uhs.one_step_config(ext, s["radius"], st, dt)
+            assert not o.flags & uhs.CFG_TEMPORAL2
+            assert uhs.plan_kind(ext, s["radius"], st, dt, cfg=o.tuple()) == uhs.plan_kind(ext, s["radius"], st, dt)
+            if case == "3d1r_star" and dt == uhs.DT_F64:
+                assert d.flags == uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE
+                assert (d.tile_x, d.tile_y, d.vec) == (62, 6, 2)
+                assert d.tile_z == (64 if ext[0] == 512 else 8)
+                assert o.tuple() == ((128, 16, 256 if ext[0] == 512 else 8, 8, uhs.CFG_NT_STORE))
+            else:
+                assert d.tuple() == o.tuple()
+
+
 def expect_code(uhs, code, fn):
     with pytest.raises(uhs.UhsError) as e:
         fn()

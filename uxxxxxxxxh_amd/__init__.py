@@ -105,6 +105,7 @@ _SIGS = {
     "uhs_sweep_f32": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, _I, ctypes.POINTER(Config), _P]),
     "uhs_plan_kind": (_I, [_I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), ctypes.c_char_p, ctypes.c_size_t]),
     "uhs_default_config": (_I, [_I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config)]),
+    "uhs_one_step_config": (_I, [_I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config)]),
     "uhs_builtin_count": (_I, []),
     "uhs_builtin_info": (_I, [_I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_I),
                               ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.POINTER(_I)),
@@ -440,6 +441,14 @@ def default_config(ext, halo, stencil, dtype=DT_F64):
     c = Config()
     check(lib().uhs_default_config(len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,
                                    ctypes.byref(c)))
+    return c
+
+
+def one_step_config(ext, halo, stencil, dtype=DT_F64):
+    """The default configuration with one launch per timestep."""
+    c = Config()
+    check(lib().uhs_one_step_config(len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,
+                                    ctypes.byref(c)))
     return c
 
 

@@ -257,13 +257,30 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
     {32, 32, 128, 1, UHS_CFG_ROWS2 | UHS_CFG_NT_STORE},      // 3d3r_box
 };
 
-uhs_config uhs_heuristic_config(const Plan &pl)
+// FP64 defaults that advance two steps per launch (UHS_CFG_TEMPORAL2),
+// where a timed sweep at 512^3 (T = 20) found them faster than the one-step
+// default: 3d1r_star 422 against 337 GSt/s with a 62x6 output tile (64x8
+// with the ghost zone), 2 columns per thread and 64-plane slabs.  The
+// radius-2 and radius-3 stars stay one step per launch (316 against 324 and
+// 209 against 293 GSt/s for their best two-step tiles).  The results are
+// bitwise those of the one-step kernels.
+static const uhs_config kTemporal2None = {0, 0, 0, 0, 0};
+static const uhs_config kBuiltinTemporal2F64[12] = {
+    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+    {62, 6, 64, 2, UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE}, // 3d1r_star
+    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+};
+
+uhs_config uhs_heuristic_config(const Plan &pl, bool one_step)
 {
     uhs_config c{};
     const SweepArgs &a = pl.args;
     if (pl.family == FAM_GATHER) return c;
     c = a.dtype == DT_F64 ? kBuiltinDefaults[pl.prog] : kBuiltinDefaultsF32[pl.prog];
     char why[128];
+    if (!one_step && a.dtype == DT_F64 && kBuiltinTemporal2F64[pl.prog].flags &&
+        uhs_check_config(pl, kBuiltinTemporal2F64[pl.prog], why, sizeof why) == UHS_OK)
+        c = kBuiltinTemporal2F64[pl.prog];
     if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) {
         // e.g. an FP32 form whose vec variant is not compiled
         c.vec = 1;
@@ -358,7 +375,7 @@ int uhs_plan_apply(Plan &pl, const uhs_config &c)
     if (!pl.fn) return fail(UHS_E_UNSUPPORTED, "no kernel for this stencil and precision");
     if (c.flags & UHS_CFG_TEMPORAL2) {
         Plan one = pl;
-        const uhs_config c1 = uhs_heuristic_config(one);
+        const uhs_config c1 = uhs_heuristic_config(one, true);
         one.args.tx = c1.tile_x; one.args.ty = c1.tile_y; one.args.tz = c1.tile_z;
         one.args.vec = c1.vec; one.args.flags = c1.flags;
         pl.fn1 = uhs_pick_fn(one, c1);
@@ -503,6 +520,17 @@ int uhs_default_config(int dims, const int64_t *ext, int halo, const uhs_stencil
     int rc = uhs_make_plan(dims, ext, halo, st, dtype, nullptr, &pl);
     if (rc) return rc;
     *out = pl.cfg;
+    return UHS_OK;
+}
+
+int uhs_one_step_config(int dims, const int64_t *ext, int halo, const uhs_stencil *st, int dtype,
+                        uhs_config *out)
+{
+    if (!out) return fail(UHS_E_INVAL, "null output");
+    Plan pl;
+    int rc = uhs_make_plan(dims, ext, halo, st, dtype, nullptr, &pl);
+    if (rc) return rc;
+    *out = uhs_heuristic_config(pl, true);
     return UHS_OK;
 }
 

@@ -532,9 +532,16 @@ int uhs_batch_tune(const uhs_grid_desc *grids, int n, int timesteps, uhs_batch_c
 int uhs_sweep_each_f64(const uhs_grid_desc *grids, int n, int timesteps, uhs_stream_t stream)
 {
     if (!grids || n < 1) return berr(UHS_E_INVAL, "no grids");
+    // One launch per timestep per grid: the one-step kernel with its
+    // default tile, whatever the per-grid default (which may advance two
+    // steps per launch) or a tuned entry would choose.
     for (int i = 0; i < n; i++) {
         const uhs_grid_desc &d = grids[i];
-        int rc = uhs_sweep_f64(d.A, d.B, d.dims, d.ext, d.halo, &d.stencil, timesteps, nullptr, stream);
+        Plan pl;
+        int rc = uhs_make_plan(d.dims, d.ext, d.halo, &d.stencil, DT_F64, nullptr, &pl);
+        if (rc) return rc;
+        const uhs_config c = uhs_heuristic_config(pl, true);
+        rc = uhs_sweep_f64(d.A, d.B, d.dims, d.ext, d.halo, &d.stencil, timesteps, &c, stream);
         if (rc) return rc;
     }
     return UHS_OK;

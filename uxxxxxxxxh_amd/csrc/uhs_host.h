@@ -28,7 +28,10 @@ int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream);
 // UHS_CFG_TEMPORAL2 the single-step kernel of the leading steps.
 int uhs_plan_apply(Plan &pl, const uhs_config &c);
 int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whylen);
-uhs_config uhs_heuristic_config(const Plan &pl);
+// The built-in default configuration for a plan; one_step excludes the
+// two-steps-per-launch defaults (for the leading single steps of such a
+// plan, and for one-launch-per-step baselines).
+uhs_config uhs_heuristic_config(const Plan &pl, bool one_step = false);
 // The launcher for a plan under configuration c (nullptr: not compiled).
 LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c);
 long long uhs_config_blocks(const SweepArgs &a, const uhs_config &c, int radius);

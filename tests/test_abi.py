@@ -70,8 +70,8 @@ def test_plan_kinds_and_default_configs(uhs, case):
 
 @pytest.mark.parametrize("case", list(SHAPES))
 def test_one_step_configs(uhs, case):
-    """Only the FP64 7-point star defaults to two steps per launch; its
-    one-step configuration (leading steps, unbatched baseline) is the
+    """Only the FP64 radius-1 and radius-2 stars default to two steps per
+    launch; their one-step configuration (leading steps, unbatched baseline) is the
     one-step default tile, and every other default is its own one-step
     configuration."""
     s = SHAPES[case]
@@ -83,11 +83,13 @@ def test_one_step_configs(uhs, case):
             o = uhs.one_step_config(ext, s["radius"], st, dt)
             assert not o.flags & uhs.CFG_TEMPORAL2
             assert uhs.plan_kind(ext, s["radius"], st, dt, cfg=o.tuple()) == uhs.plan_kind(ext, s["radius"], st, dt)
+            big = ext[0] == 512
             if case == "3d1r_star" and dt == uhs.DT_F64:
-                assert d.flags == uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE
-                assert (d.tile_x, d.tile_y, d.vec) == (62, 6, 2)
-                assert d.tile_z == (64 if ext[0] == 512 else 8)
-                assert o.tuple() == ((128, 16, 256 if ext[0] == 512 else 8, 8, uhs.CFG_NT_STORE))
+                assert d.tuple() == (62, 14, 32 if big else 8, 2, uhs.CFG_TEMPORAL2)
+                assert o.tuple() == (128, 16, 256 if big else 8, 8, uhs.CFG_NT_STORE)
+            elif case == "3d2r_star" and dt == uhs.DT_F64:
+                assert d.tuple() == (28, 12, 64 if big else 16, 2, uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE)
+                assert o.tuple() == (128, 16, 256 if big else 16, 8, uhs.CFG_NT_STORE)
             else:
                 assert d.tuple() == o.tuple()
 

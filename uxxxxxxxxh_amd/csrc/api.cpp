@@ -258,17 +258,19 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 };
 
 // FP64 defaults that advance two steps per launch (UHS_CFG_TEMPORAL2),
-// where a timed sweep at 512^3 (T = 20) found them faster than the one-step
-// default: 3d1r_star 422 against 337 GSt/s with a 62x6 output tile (64x8
-// with the ghost zone), 2 columns per thread and 64-plane slabs.  The
-// radius-2 and radius-3 stars stay one step per launch (316 against 324 and
-// 209 against 293 GSt/s for their best two-step tiles).  The results are
-// bitwise those of the one-step kernels.
+// where a timed sweep at 512^3 (T = 40) found them faster than the one-step
+// default: 3d1r_star 446 against 428 GSt/s for the earlier two-step default
+// and 337 for one step per launch (62x14 output tile, 64x16 with the ghost
+// zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327 against 320
+// (28x12 output tile, 64-plane slabs).  The radius-3 star stays one step per
+// launch (209 against 293 GSt/s for its best two-step tile).  The results
+// are bitwise those of the one-step kernels.
 static const uhs_config kTemporal2None = {0, 0, 0, 0, 0};
 static const uhs_config kBuiltinTemporal2F64[12] = {
     kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
-    {62, 6, 64, 2, UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE}, // 3d1r_star
-    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+    {62, 14, 32, 2, UHS_CFG_TEMPORAL2},                   // 3d1r_star
+    {28, 12, 64, 2, UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE}, // 3d2r_star
+    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
 };
 
 uhs_config uhs_heuristic_config(const Plan &pl, bool one_step)

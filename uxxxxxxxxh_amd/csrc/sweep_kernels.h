@@ -541,11 +541,16 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
             }
         }
     };
-    for (int i0 = 0; i0 <= N; i0 += M0)
-        static_for<M0>([&](auto phc) {
-            const int i = i0 + decltype(phc)::value;
-            if (i <= N) step(phc, i);
-        });
+    // Whole ring turns without a branch around any step, so the waits at the
+    // loop's entry count the loads still in flight instead of draining them;
+    // then the remaining iterations i0 .. N (fewer than M0).
+    int i0 = 0;
+    for (; i0 + M0 - 1 <= N; i0 += M0)
+        static_for<M0>([&](auto phc) { step(phc, i0 + decltype(phc)::value); });
+    static_for<M0>([&](auto phc) {
+        const int i = i0 + decltype(phc)::value;
+        if (i <= N) step(phc, i);
+    });
 }
 
 

@@ -174,6 +174,13 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
                         if (tz > 2 * pl.args.n0 && tz > 16) continue;
                         out.push_back(c);
                     }
+    // The untuned default is always a candidate (its slab depth is shortened
+    // for small grids, outside the grid of tile sizes above), so a search
+    // never picks something slower than not searching, timing noise aside.
+    const uhs_config d = uhs_heuristic_config(pl);
+    if (pl.family != FAM_GATHER &&
+        std::none_of(out.begin(), out.end(), [&](const uhs_config &c) { return ct(c) == ct(d); }))
+        out.push_back(d);
     return out;
 }
 

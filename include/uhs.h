@@ -120,8 +120,9 @@ typedef struct uhs_config {
 
 /* FP64 sweep, T timesteps.  ext[dims] = interior extents, halo >= the
  * stencil radius.  cfg NULL = the tuned (or default) configuration; the
- * FP64 default of the 7-point star (3d1r_star) advances two steps per
- * launch (UHS_CFG_TEMPORAL2), with results bitwise those of single sweeps. */
+ * FP64 defaults of the radius-1 and radius-2 stars (3d1r_star, 3d2r_star)
+ * advance two steps per launch (UHS_CFG_TEMPORAL2), with results bitwise
+ * those of single sweeps. */
 int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
                   const uhs_stencil *st, int timesteps, const uhs_config *cfg,
                   uhs_stream_t stream);

@@ -129,7 +129,9 @@ int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
 
 /* FP32 storage and arithmetic.  compensated = 0: plain fused sum (the
  * reference's `float` DSL mode, Parser.h:86-91); compensated = 1:
- * Kahan-compensated sum of the terms. */
+ * Kahan-compensated sum of the terms.  cfg NULL = the tuned (or default)
+ * configuration; the plain FP32 default of 3d1r_star advances two steps per
+ * launch (UHS_CFG_TEMPORAL2), bitwise the same as single sweeps. */
 int uhs_sweep_f32(float *A, float *B, int dims, const int64_t *ext, int halo,
                   const uhs_stencil *st, int timesteps, int compensated,
                   const uhs_config *cfg, uhs_stream_t stream);

@@ -70,8 +70,8 @@ def test_plan_kinds_and_default_configs(uhs, case):
 
 @pytest.mark.parametrize("case", list(SHAPES))
 def test_one_step_configs(uhs, case):
-    """Only the FP64 radius-1 and radius-2 stars default to two steps per
-    launch; their one-step configuration (leading steps, unbatched baseline) is the
+    """Only the FP64 radius-1 and radius-2 stars and the plain FP32 radius-1
+    star default to two steps per launch; their one-step configuration (leading steps, unbatched baseline) is the
     one-step default tile, and every other default is its own one-step
     configuration."""
     s = SHAPES[case]
@@ -84,9 +84,12 @@ def test_one_step_configs(uhs, case):
             assert not o.flags & uhs.CFG_TEMPORAL2
             assert uhs.plan_kind(ext, s["radius"], st, dt, cfg=o.tuple()) == uhs.plan_kind(ext, s["radius"], st, dt)
             big = ext[0] == 512
-            if case == "3d1r_star" and dt == uhs.DT_F64:
+            if case == "3d1r_star" and dt in (uhs.DT_F64, uhs.DT_F32):
                 assert d.tuple() == (62, 14, 32 if big else 8, 2, uhs.CFG_TEMPORAL2)
-                assert o.tuple() == (128, 16, 256 if big else 8, 8, uhs.CFG_NT_STORE)
+                if dt == uhs.DT_F64:
+                    assert o.tuple() == (128, 16, 256 if big else 8, 8, uhs.CFG_NT_STORE)
+                else:
+                    assert o.tuple()[:2] == (256, 16) and o.flags == 0
             elif case == "3d2r_star" and dt == uhs.DT_F64:
                 assert d.tuple() == (28, 12, 64 if big else 16, 2, uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE)
                 assert o.tuple() == (128, 16, 256 if big else 16, 8, uhs.CFG_NT_STORE)

@@ -273,6 +273,17 @@ static const uhs_config kBuiltinTemporal2F64[12] = {
     kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
 };
 
+// Plain FP32 (the DSL's float mode): the 7-point star also advances two
+// steps per launch, 727 against 627 GSt/s at 512^3 (T = 40).  The
+// compensated form and the radius-2 and radius-3 stars were faster with one
+// step per launch (3d1r_star compensated 630 against 636, 3d2r_star 537
+// against 586, 3d3r_star 327 against 440 GSt/s).
+static const uhs_config kBuiltinTemporal2F32[12] = {
+    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+    {62, 14, 32, 2, UHS_CFG_TEMPORAL2}, // 3d1r_star
+    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+};
+
 uhs_config uhs_heuristic_config(const Plan &pl, bool one_step)
 {
     uhs_config c{};
@@ -280,9 +291,10 @@ uhs_config uhs_heuristic_config(const Plan &pl, bool one_step)
     if (pl.family == FAM_GATHER) return c;
     c = a.dtype == DT_F64 ? kBuiltinDefaults[pl.prog] : kBuiltinDefaultsF32[pl.prog];
     char why[128];
-    if (!one_step && a.dtype == DT_F64 && kBuiltinTemporal2F64[pl.prog].flags &&
-        uhs_check_config(pl, kBuiltinTemporal2F64[pl.prog], why, sizeof why) == UHS_OK)
-        c = kBuiltinTemporal2F64[pl.prog];
+    const uhs_config *two = a.dtype == DT_F64 ? &kBuiltinTemporal2F64[pl.prog]
+                            : a.dtype == DT_F32 ? &kBuiltinTemporal2F32[pl.prog]
+                                                : nullptr;
+    if (!one_step && two && two->flags && uhs_check_config(pl, *two, why, sizeof why) == UHS_OK) c = *two;
     if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) {
         // e.g. an FP32 form whose vec variant is not compiled
         c.vec = 1;

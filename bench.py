@@ -150,19 +150,17 @@ def batched(u, T=50, probe_T=8):
     half 3d2r_box (125 points), seeded normalised coefficients, T steps: the
     plain per-grid sweeps against the batched graph with the configuration
     the batch autotuner picks (timed on a probe_T-step run)."""
-    rng = np.random.default_rng(2024)
-    choices = list(range(64, 385, 32))
+    from uxxxxxxxxh_amd.workloads import uneven_batch
     grids, pts = [], 0
-    for i in range(64):
-        case = "3d1r_star" if i < 32 else "3d2r_box"
-        st = u.builtin(case)
-        st = st.with_coeffs(u.make_coeffs(st, 500 + i))
-        ext = [int(rng.choice(choices)) for _ in range(3)]
+    for g in uneven_batch():
+        st = u.builtin(g["case"])
+        st = st.with_coeffs(u.make_coeffs(st, g["coeff_seed"]))
+        ext = g["ext"]
         h = st.radius
         A = u.DeviceArray(u.grid_shape(ext, h), np.float64)
         B = u.DeviceArray(u.grid_shape(ext, h), np.float64)
-        u.fill_uniform(A, ext, h, 7 + i)
-        u.fill_uniform(B, ext, h, 1007 + i)
+        u.fill_uniform(A, ext, h, g["seed_a"])
+        u.fill_uniform(B, ext, h, g["seed_b"])
         grids.append((A, B, ext, h, st))
         pts += int(np.prod(ext))
 
@@ -184,6 +182,13 @@ def batched(u, T=50, probe_T=8):
     b0 = u.Batch(grids, T, u.batch_default_config())
     ms_plain = timed(b0.run_each)
     b0.close()
+
+    def each_default():
+        # Each grid through uhs_sweep_f64 with the library's per-grid default
+        # (two steps per launch for the 7-point star grids).
+        for A, B, ext, h, st in grids:
+            u.sweep_f64(A, B, ext, h, st, T)
+    ms_default = timed(each_default)
     b = u.Batch(grids, T, best)
     ms_batch = timed(b.run)
     info = b.info()
@@ -211,6 +216,8 @@ def batched(u, T=50, probe_T=8):
             "plain_ms": round(ms_plain, 3), "plain_GStencils_s": round(upd / ms_plain / 1e6, 2),
             "batched_ms": round(ms_batch, 3), "batched_GStencils_s": round(upd / ms_batch / 1e6, 2),
             "speedup": round(ms_plain / ms_batch, 3),
+            "per_grid_default_ms": round(ms_default, 3),
+            "speedup_vs_per_grid_default": round(ms_default / ms_batch, 3),
             "config": {"star_depth": best.star_depth, "star_tile": [best.star_tile_x, best.star_tile_y, best.star_tile_z],
                        "box_tile": [best.box_tile_x, best.box_tile_y, best.box_tile_z], "flags": best.flags},
             "tuning_candidates": len(cands), "tuning_s": round(t_tune, 2), "launches": info["launches"],
@@ -323,8 +330,22 @@ def fp32_kahan(u):
             "errors": r["checkpoints"]}
 
 
-def cpu_baseline(ext, case="3d1r_star", budget_s=10.0):
-    """Single-thread oracle sweep (plain mode) on the same workload shape."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ext, case="3d1r_star", budget_s=10.0, oracle_budget_s=5.0):
+    """The oracle's restatement of the reference's serial sweep (cpu_verify,
+    plain multiply-then-add in term order) on the same workload shape: one
+    pinned thread (the baseline), then every host core this process may use
+    (the OpenMP oracle, bitwise the same per point; reported beside it)."""
     from tests.oracle_py import Oracle, PLAIN
     import json as _json
     orc = Oracle()
@@ -336,23 +357,34 @@ def cpu_baseline(ext, case="3d1r_star", budget_s=10.0):
     B = np.zeros_like(A)
     try:
         cpu = sorted(os.sched_getaffinity(0))
-        os.sched_setaffinity(0, {cpu[0]})
     except (AttributeError, OSError):
         cpu = None
-    sweeps, t0 = 0, time.perf_counter()
-    while True:
-        src, dst = (A, B) if sweeps % 2 == 0 else (B, A)
-        orc.sweep(src, dst, ext, h, s["offsets"], c, PLAIN, nthreads=1)
-        sweeps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
+
+    def run(nthreads, budget):
+        sweeps, t0 = 0, time.perf_counter()
+        while True:
+            src, dst = (A, B) if sweeps % 2 == 0 else (B, A)
+            orc.sweep(src, dst, ext, h, s["offsets"], c, PLAIN, nthreads=nthreads)
+            sweeps += 1
+            el = time.perf_counter() - t0
+            if el >= budget:
+                return sweeps, el
+
+    if cpu:
+        os.sched_setaffinity(0, {cpu[0]})
+    sweeps, el = run(1, budget_s)
     if cpu:
         os.sched_setaffinity(0, set(cpu))
     n = float(np.prod(ext))
+    threads = len(cpu) if cpu else (os.cpu_count() or 1)
+    osw, oel = run(threads, oracle_budget_s)
     return {"value": n * sweeps / el / 1e9, "unit": "GStencils/s", "cores": 1, "kind": "port",
             "sample": "%s FP64 %s, %d sweeps (%.1f s), oracle plain mode, 1 pinned thread"
-                      % (case, "x".join(map(str, ext)), sweeps, el)}
+                      % (case, "x".join(map(str, ext)), sweeps, el),
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "cpus_usable": threads,
+            "oracle_all_cores": {"value": n * osw / oel / 1e9, "unit": "GStencils/s", "threads": threads,
+                                 "sample": "%d sweeps (%.1f s), same grid, OpenMP over rows" % (osw, oel),
+                                 "label": "oracle (not the baseline)"}}
 
 
 def kernel_name(cfg):
@@ -412,6 +444,8 @@ def main():
                     help="skip one GPU's share of the slab decomposition")
     ap.add_argument("--programs-only", nargs="*", default=None, metavar="NAME",
                     help="only the fused multi-kernel programs (all, or those named)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched leg (profiling runs)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the copy kernel (traffic-counter calibration: 2 GiB read + written)")
     args = ap.parse_args()
@@ -433,6 +467,9 @@ def main():
         print(json.dumps({"case": case, "GStencils_s": round(float(np.prod(e)) / (ms_w * 1e-3) / 1e9, 2),
                           "ms_per_sweep": round(ms_w, 5), "tile": list(c.tuple()),
                           "copy_GBps": round(copy_bandwidth(u, reps=2), 1)}))
+        return
+    if args.batch_only:
+        print(json.dumps(batched(u)))
         return
     if args.programs_only is not None:
         print(json.dumps(programs(u, names=args.programs_only)))
@@ -489,6 +526,14 @@ def main():
                      "algorithmic_bytes_per_launch": int(alg_bytes),
                      "avg_launch_ms": round(ms_launch, 5)},
     }
+    if traffic:
+        # The DRAM bytes the launch really moves (traffic counters, committed
+        # summary) over the same launch time: with two steps per launch the
+        # kernel moves fewer bytes than the 16 per update counted above.
+        dram = traffic["bytes_per_launch"] / (ms_launch * 1e-3) / 1e9
+        out["roofline"]["dram_GBps"] = round(dram, 1)
+        out["roofline"]["dram_frac"] = round(dram / PEAK_HBM_GBPS, 4)
+        out["roofline"]["dram_bytes_per_update"] = round(traffic["bytes_per_launch"] / (updates * per_launch), 3)
     if not getattr(args, "no_probes", False):
         probes(u, out)
     if not args.no_shapes:

@@ -238,7 +238,12 @@ typedef struct uhs_batch *uhs_batch_t;
  * other buffer holds an earlier state.  cfg NULL = the default. */
 int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps,
                      const uhs_batch_config *cfg, uhs_batch_t *out);
-/* Launches the captured graph on `stream` (asynchronous; reusable). */
+/* Launches the captured graph on `stream` (asynchronous).  Reusable: every
+ * run advances each grid by T more steps, continuing from the state the
+ * previous run left, with the ping-pong convention carried on (after r runs
+ * the state r*T is in A when r*T is even, in B when odd; for odd T the
+ * batch holds a second graph with the roles of A and B swapped and runs
+ * alternate between the two).  Runs of one batch must not overlap. */
 int uhs_batch_run(uhs_batch_t b, uhs_stream_t stream);
 int uhs_batch_destroy(uhs_batch_t b);
 int uhs_batch_info(uhs_batch_t b, int *launches, long long *tasks,

@@ -41,7 +41,6 @@ def check_batch(u, orc, grids, T, cfg):
         orc.iterate(A, B, ext, h, st.offsets, st.coeffs, T, FUSED)
         res, ref = (dA.download(), A) if T % 2 == 0 else (dB.download(), B)
         assert np.array_equal(res, ref), (case, ext, h, T, cfg)
-    # the graph is reusable: a second run continues from the current state
     b.close()
     for dA, dB, _, _, _ in dev:
         dA.close()
@@ -116,3 +115,31 @@ def test_batch_default_config_and_rejections(gpu, orc):
         u.Batch([(dA, dB, ext, h, two_d)], 2)
     dA.close()
     dB.close()
+
+
+@pytest.mark.parametrize("T", [3, 4, 5])
+@pytest.mark.parametrize("depth", [1, 2, 4])
+def test_batch_rerun_continues(gpu, orc, T, depth):
+    """Runs of one batch continue from the state the last run left: three
+    runs equal 3T single sweeps, also for odd T (whose odd-numbered runs
+    start from B, through the second graph with the roles swapped)."""
+    u = gpu
+    grids = make_grids(u, orc, ["3d1r_star", "3d2r_box", "3d2r_star"], seed=50 + T + depth, halo_extra=1)
+    dev = [(u.DeviceArray.from_host(A0), u.DeviceArray.from_host(B0), ext, h, st)
+           for case, st, ext, h, A0, B0 in grids]
+    cfg = u.batch_default_config()
+    cfg.star_depth = depth
+    cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = 16, 8, 7
+    b = u.Batch(dev, T, cfg)
+    runs = 3
+    for _ in range(runs):
+        b.run()
+    u.synchronize()
+    b.close()
+    for (case, st, ext, h, A0, B0), (dA, dB, _, _, _) in zip(grids, dev):
+        A, B = A0.copy(), B0.copy()
+        orc.iterate(A, B, ext, h, st.offsets, st.coeffs, runs * T, FUSED)
+        res, ref = (dA.download(), A) if (runs * T) % 2 == 0 else (dB.download(), B)
+        assert np.array_equal(res, ref), (case, ext, h, T, depth)
+        dA.close()
+        dB.close()

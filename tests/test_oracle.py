@@ -87,6 +87,43 @@ def test_left_to_right_term_order(orc, case, mode):
         assert b[p] == expect, (case, big_term, b[p], expect)
 
 
+def _residual(c, v, dt):
+    """c*v - round(c*v), exactly (the error of a rounded product is
+    representable in the same format, so the conversion is exact)."""
+    from fractions import Fraction
+    prod = dt(c) * dt(v)
+    r = Fraction(float(dt(c))) * Fraction(float(dt(v))) - Fraction(float(prod))
+    assert r != 0 and float(dt(float(r))) == r
+    return float(r), float(prod)
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_fused_mode_contracts(orc, case, dt):
+    """FUSED evaluates term t as one fused multiply-add; PLAIN rounds the
+    product first.  Term 0 contributes s = -round(c1*v) exactly (coefficient
+    1), term 1 then adds c1*v with c1*v inexact: fma(c1, v, s) is the exact
+    rounding error of the product (nonzero), the unfused sum is 0.  Later
+    terms read zeros.  A silently unfused FUSED mode, or a contracted PLAIN
+    mode, fails here."""
+    s = SHAPES[case]
+    d, r, P = s["dims"], s["radius"], s["npoints"]
+    ext = [2 * r + 3] * d
+    p = tuple(e // 2 + r for e in ext)
+    c1, v = 0.1, 3.0
+    res, prod = _residual(c1, v, dt)
+    a = np.zeros([e + 2 * r for e in ext], dt)
+    off = s["offsets"]
+    a[tuple(pi + oi for pi, oi in zip(p, off[0]))] = -prod
+    a[tuple(pi + oi for pi, oi in zip(p, off[1]))] = v
+    c = np.full(P, 0.5)
+    c[0], c[1] = 1.0, c1
+    for mode, expect in ((FUSED, res), (PLAIN, 0.0)):
+        b = np.zeros_like(a)
+        orc.sweep(a, b, ext, r, off, c, mode)
+        assert float(b[p]) == expect, (case, mode, float(b[p]), expect)
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_constant_field_preserved(orc, case):
     s = SHAPES[case]

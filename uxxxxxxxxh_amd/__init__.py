@@ -420,12 +420,28 @@ def _cfg(cfg):
     return ctypes.byref(Config(*cfg))
 
 
+def _check_grids(ext, halo, dtype, *arrays):
+    """Every device grid must be the halo-inclusive shape of ext and halo, in
+    the precision the call computes in: the kernels index the grid from those
+    extents, so a smaller allocation would be read and written out of
+    bounds."""
+    want = grid_shape(ext, halo)
+    for a in arrays:
+        if tuple(a.shape) != want:
+            raise ValueError("grid shape %s does not match extents %s with halo %d (expected %s)"
+                             % (tuple(a.shape), list(ext), halo, want))
+        if np.dtype(a.dtype) != np.dtype(dtype):
+            raise ValueError("grid dtype %s, expected %s" % (np.dtype(a.dtype), np.dtype(dtype)))
+
+
 def sweep_f64(A, B, ext, halo, stencil, timesteps, cfg=None, stream=None):
+    _check_grids(ext, halo, np.float64, A, B)
     check(lib().uhs_sweep_f64(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
                               timesteps, _cfg(cfg), stream))
 
 
 def sweep_f32(A, B, ext, halo, stencil, timesteps, compensated=False, cfg=None, stream=None):
+    _check_grids(ext, halo, np.float32, A, B)
     check(lib().uhs_sweep_f32(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
                               timesteps, 1 if compensated else 0, _cfg(cfg), stream))
 
@@ -453,11 +469,18 @@ def one_step_config(ext, halo, stencil, dtype=DT_F64):
 
 
 def fill_uniform(arr, ext, halo, seed, stream=None):
+    if np.dtype(arr.dtype) not in (np.dtype(np.float64), np.dtype(np.float32)):
+        raise ValueError("fill_uniform takes float64 or float32 grids")
+    _check_grids(ext, halo, arr.dtype, arr)
     f = lib().uhs_fill_uniform_f64 if arr.dtype == np.float64 else lib().uhs_fill_uniform_f32
     check(f(arr.ptr, len(ext), ext_array(ext), halo, seed, stream))
 
 
 def error_norms(a, b, ext, halo, stream=None):
+    if np.dtype(a.dtype) not in (np.dtype(np.float64), np.dtype(np.float32)):
+        raise ValueError("error_norms compares a float64 or float32 grid with a float64 one")
+    _check_grids(ext, halo, a.dtype, a)
+    _check_grids(ext, halo, np.float64, b)
     out = (ctypes.c_double * 3)()
     f = lib().uhs_error_norms_f32_f64 if a.dtype == np.float32 else lib().uhs_error_norms_f64
     check(f(a.ptr, b.ptr, len(ext), ext_array(ext), halo, out, stream))
@@ -579,6 +602,7 @@ def tune_space(ext, halo, stencil, dtype=DT_F64):
 
 
 def tune_sweep(A, B, ext, halo, stencil, dtype=DT_F64, reps=3, db=None, model=None, topk=5, stream=None):
+    _check_grids(ext, halo, np.float64 if dtype == DT_F64 else np.float32, A, B)
     r = TuneResult()
     dbp = db.encode() if db else None
     if model is None:
@@ -591,6 +615,7 @@ def tune_sweep(A, B, ext, halo, stencil, dtype=DT_F64, reps=3, db=None, model=No
 
 
 def tune_list(A, B, ext, halo, stencil, cands, dtype=DT_F64, reps=2, db=None, stream=None):
+    _check_grids(ext, halo, np.float64 if dtype == DT_F64 else np.float32, A, B)
     arr = (Config * len(cands))(*[c if isinstance(c, Config) else Config(*c) for c in cands])
     r = TuneResult()
     check(lib().uhs_tune_sweep_list(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,
@@ -599,6 +624,7 @@ def tune_list(A, B, ext, halo, stencil, cands, dtype=DT_F64, reps=2, db=None, st
 
 
 def tune_time(A, B, ext, halo, stencil, cfg, dtype=DT_F64, reps=3, stream=None):
+    _check_grids(ext, halo, np.float64 if dtype == DT_F64 else np.float32, A, B)
     ms = ctypes.c_double()
     c = cfg if isinstance(cfg, Config) else Config(*cfg)
     check(lib().uhs_tune_time(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,

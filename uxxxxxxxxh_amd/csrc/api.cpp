@@ -61,11 +61,11 @@ int uhs_match_builtin(int dims, int npoints, const int *off)
     return -1;
 }
 
-std::string uhs_config_key(int family, int prog, int dtype, int dims, const int64_t *ext)
+std::string uhs_config_key(int family, int prog, int dtype, int dims, const int64_t *ext, int halo)
 {
     char buf[128];
-    snprintf(buf, sizeof buf, "%d/%d/%d/%lldx%lldx%lld", family, prog, dtype, (long long)ext[0],
-             (long long)ext[1], dims == 3 ? (long long)ext[2] : 1LL);
+    snprintf(buf, sizeof buf, "%d/%d/%d/%lldx%lldx%lld/h%d", family, prog, dtype, (long long)ext[0],
+             (long long)ext[1], dims == 3 ? (long long)ext[2] : 1LL, halo);
     return buf;
 }
 
@@ -367,12 +367,15 @@ int uhs_make_plan(int dims, const int64_t *ext, int halo, const uhs_stencil *st,
     }
 
     uhs_config c;
+    char why[256] = "";
     if (cfg) {
         c = *cfg;
-    } else if (!uhs_lookup_tuned(uhs_config_key(pl->family, pl->prog, dtype, dims, ext), &c)) {
+    } else if (!uhs_lookup_tuned(uhs_config_key(pl->family, pl->prog, dtype, dims, ext, halo), &c) ||
+               uhs_check_config(*pl, c, why, sizeof why) != UHS_OK) {
+        // No tuned entry, or one this build's kernels no longer accept (a
+        // database written by another version): the heuristic choice.
         c = uhs_heuristic_config(*pl);
     }
-    char why[256] = "";
     if (uhs_check_config(*pl, c, why, sizeof why) != UHS_OK)
         return fail(UHS_E_CONFIG, "config {%d,%d,%d,%d,%d} rejected: %s", c.tile_x, c.tile_y,
                     c.tile_z, c.vec, c.flags, why);

@@ -84,15 +84,21 @@ int berr(int code, const std::string &m)
 
 } // namespace
 
+// A batch keeps one captured graph per buffer parity.  Run r (r = 0, 1, ...)
+// advances every grid from global step r*T; when T is odd, the state after an
+// odd number of runs sits in B, so that run reads the grids with the roles of
+// A and B swapped (grid records n..2n-1, task list ntasks..2*ntasks-1), which
+// is exactly a fresh run on the swapped pair.  Even T needs graph 0 only.
 struct uhs_batch {
-    BGrid *d_grids = nullptr;
-    BTask *d_tasks = nullptr;
+    BGrid *d_grids = nullptr;   // [2n]: as given, then with A and B swapped
+    BTask *d_tasks = nullptr;   // [2 * ntasks] (the second half only for odd T)
     unsigned *d_counters = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     std::vector<LaunchRec> launches;
     long long ntasks = 0;
     int timesteps = 0;
+    int phase = 0;              // graph the next run launches
 };
 
 static std::vector<int> star_schedule(int T, int K)
@@ -367,6 +373,27 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         B->launches.push_back(r);
     }
     B->ntasks = (long long)all.size();
+    const bool odd = timesteps % 2 != 0;
+    const size_t nt = all.size();
+    // The swapped-role copies for the odd-numbered runs of an odd T.
+    for (int i = 0; i < n; i++) {
+        BGrid g = hg[i];
+        std::swap(g.A, g.B);
+        hg.push_back(g);
+    }
+    if (odd) {
+        all.reserve(2 * nt);
+        for (size_t k = 0; k < nt; k++) {
+            BTask t = all[k];
+            const BGrid &g = hg[t.grid];
+            const long long origin = (long long)g.halo * g.sz + (long long)g.halo * g.sy + g.halo;
+            const double *a = g.A + origin, *b = g.B + origin;
+            t.src = t.src == a ? b : a;
+            t.dst = const_cast<double *>(t.dst == b ? a : b);
+            t.grid += n;
+            all.push_back(t);
+        }
+    }
 
     auto fail_hip = [&](hipError_t e, const char *what) {
         std::string m = std::string(what) + ": " + hipGetErrorString(e);
@@ -374,11 +401,11 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         return berr(UHS_E_HIP, m);
     };
     hipError_t e;
-    if ((e = hipMalloc(&B->d_grids, sizeof(BGrid) * n)) != hipSuccess) return fail_hip(e, "grid table");
+    if ((e = hipMalloc(&B->d_grids, sizeof(BGrid) * hg.size())) != hipSuccess) return fail_hip(e, "grid table");
     if ((e = hipMalloc(&B->d_tasks, sizeof(BTask) * std::max<size_t>(1, all.size()))) != hipSuccess)
         return fail_hip(e, "task table");
     if ((e = hipMalloc(&B->d_counters, sizeof(unsigned) * L)) != hipSuccess) return fail_hip(e, "counters");
-    if ((e = hipMemcpy(B->d_grids, hg.data(), sizeof(BGrid) * n, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpy(B->d_grids, hg.data(), sizeof(BGrid) * hg.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return fail_hip(e, "grid upload");
     if (!all.empty() &&
         (e = hipMemcpy(B->d_tasks, all.data(), sizeof(BTask) * all.size(), hipMemcpyHostToDevice)) != hipSuccess)
@@ -397,42 +424,48 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         r.workgroups = (int)std::max<long long>(1, std::min<long long>(wg, (long long)r.count));
     }
 
-    // Capture: zero the counters with a kernel (not a memset node), then the launches.
-    hipStream_t cs;
-    if ((e = hipStreamCreate(&cs)) != hipSuccess) return fail_hip(e, "capture stream");
-    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_zero_u32, dim3((L + 255) / 256), dim3(256), 0, cs, B->d_counters, L);
-        for (int l = 0; l < L; l++) {
-            const LaunchRec &r = B->launches[l];
-            if (!r.count) continue;
-            BLaunch bl{B->d_tasks + r.first, B->d_grids, B->d_counters + l, (int)r.count};
-            hipLaunchKernelGGL(r.fn, dim3(r.workgroups), dim3(kBatchThreads), r.lds, cs, bl);
+    // Capture: zero the counters with a kernel (not a memset node), then the
+    // launches; one graph per buffer parity that a run can start from.
+    for (int ph = 0; ph < (odd ? 2 : 1); ph++) {
+        hipStream_t cs;
+        if ((e = hipStreamCreate(&cs)) != hipSuccess) return fail_hip(e, "capture stream");
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_zero_u32, dim3((L + 255) / 256), dim3(256), 0, cs, B->d_counters, L);
+            for (int l = 0; l < L; l++) {
+                const LaunchRec &r = B->launches[l];
+                if (!r.count) continue;
+                BLaunch bl{B->d_tasks + ph * nt + r.first, B->d_grids, B->d_counters + l, (int)r.count};
+                hipLaunchKernelGGL(r.fn, dim3(r.workgroups), dim3(kBatchThreads), r.lds, cs, bl);
+            }
+            hipError_t e2 = hipStreamEndCapture(cs, &B->graph[ph]);
+            if (e == hipSuccess) e = e2;
         }
-        hipError_t e2 = hipStreamEndCapture(cs, &B->graph);
-        if (e == hipSuccess) e = e2;
+        (void)hipStreamDestroy(cs);
+        if (e != hipSuccess) return fail_hip(e, "graph capture");
+        if ((e = hipGraphInstantiate(&B->exec[ph], B->graph[ph], nullptr, nullptr, 0)) != hipSuccess)
+            return fail_hip(e, "graph instantiate");
     }
-    (void)hipStreamDestroy(cs);
-    if (e != hipSuccess) return fail_hip(e, "graph capture");
-    if ((e = hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0)) != hipSuccess)
-        return fail_hip(e, "graph instantiate");
     *out = B;
     return UHS_OK;
 }
 
 int uhs_batch_run(uhs_batch_t b, uhs_stream_t stream)
 {
-    if (!b || !b->exec) return berr(UHS_E_INVAL, "null batch");
-    hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    if (!b || !b->exec[0]) return berr(UHS_E_INVAL, "null batch");
+    hipError_t e = hipGraphLaunch(b->exec[b->phase], (hipStream_t)stream);
     if (e != hipSuccess) return berr(UHS_E_HIP, std::string("graph launch: ") + hipGetErrorString(e));
+    if (b->timesteps % 2) b->phase ^= 1;
     return UHS_OK;
 }
 
 int uhs_batch_destroy(uhs_batch_t b)
 {
     if (!b) return UHS_OK;
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int ph = 0; ph < 2; ph++) {
+        if (b->exec[ph]) (void)hipGraphExecDestroy(b->exec[ph]);
+        if (b->graph[ph]) (void)hipGraphDestroy(b->graph[ph]);
+    }
     if (b->d_grids) (void)hipFree(b->d_grids);
     if (b->d_tasks) (void)hipFree(b->d_tasks);
     if (b->d_counters) (void)hipFree(b->d_counters);

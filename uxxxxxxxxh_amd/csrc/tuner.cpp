@@ -261,7 +261,7 @@ int uhs_search(Plan &pl, void *A, void *B, int dims, const int64_t *ext, int hal
     }
     r.seconds = now_s() - t0;
     if (r.evaluated == 0) return uhs_set_error(UHS_E_CONFIG, "no configuration could be timed");
-    uhs_store_tuned(uhs_config_key(pl.family, pl.prog, dtype, dims, ext), r.best);
+    uhs_store_tuned(uhs_config_key(pl.family, pl.prog, dtype, dims, ext, halo), r.best);
     if (res) *res = r;
     return UHS_OK;
 }
@@ -338,10 +338,15 @@ int uhs_tune_load_db(const char *db_path)
         if (best >= 1e30) continue;
         const int prog = uhs_builtin_find(k.stencil.c_str());
         if (prog < 0) continue;
-        const int fam = k.dims == 2 ? FAM_WIN2D : (kBuiltins[prog].star ? FAM_STAR3D : FAM_BOX3D);
+        const BuiltinStencil &b = kBuiltins[prog];
         const int64_t ext[3] = {k.e0, k.e1, k.e2};
         uhs_config c{std::get<0>(bc), std::get<1>(bc), std::get<2>(bc), std::get<3>(bc), std::get<4>(bc)};
-        uhs_store_tuned(uhs_config_key(fam, prog, k.dtype, k.dims, ext), c);
+        // Only entries this build can still run: the database outlives
+        // builds whose kernels and legality rules differ.
+        const uhs_stencil st{b.dims, b.npoints, b.off, b.coef};
+        Plan pl;
+        if (k.dims != b.dims || uhs_make_plan(k.dims, ext, k.halo, &st, k.dtype, &c, &pl) != UHS_OK) continue;
+        uhs_store_tuned(uhs_config_key(pl.family, pl.prog, k.dtype, k.dims, ext, k.halo), c);
         loaded++;
     }
     return loaded;

@@ -37,7 +37,7 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c);
 long long uhs_config_blocks(const SweepArgs &a, const uhs_config &c, int radius);
 
 // Tuned configurations, keyed by kernel family, stencil, precision and extents.
-std::string uhs_config_key(int family, int prog, int dtype, int dims, const int64_t *ext);
+std::string uhs_config_key(int family, int prog, int dtype, int dims, const int64_t *ext, int halo);
 void uhs_store_tuned(const std::string &key, const uhs_config &c);
 bool uhs_lookup_tuned(const std::string &key, uhs_config *c);
 void uhs_clear_tuned();

@@ -34,26 +34,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_sweeps(u, case, ext, T, steps, warmup, seed=1, cfg=None):
+def time_sweeps(u, case, ext, T, steps, warmup, seed=1, cfg=None, dtype="f64"):
     """Returns (ms per step, ms per sweep, config) for uhs_sweep_f64 (cfg None:
-    the library's default configuration)."""
+    the library's default configuration); dtype "f32"/"f32k": uhs_sweep_f32,
+    plain or compensated."""
     st = u.builtin(case)
     st = st.with_coeffs(u.make_coeffs(st, seed))
     h = st.radius
     shape = u.grid_shape(ext, h)
-    A = u.DeviceArray(shape, np.float64)
-    B = u.DeviceArray(shape, np.float64)
+    npdt = np.float64 if dtype == "f64" else np.float32
+    A = u.DeviceArray(shape, npdt)
+    B = u.DeviceArray(shape, npdt)
     u.fill_uniform(A, ext, h, seed)
     u.fill_uniform(B, ext, h, seed + 1)
     run_cfg = cfg
-    cfg = cfg if cfg is not None else u.default_config(ext, h, st)
+    cdt = {"f64": u.DT_F64, "f32": u.DT_F32, "f32k": u.DT_F32K}[dtype]
+    cfg = cfg if cfg is not None else u.default_config(ext, h, st, cdt)
+
+    def sweep():
+        if dtype == "f64":
+            u.sweep_f64(A, B, ext, h, st, T, run_cfg)
+        else:
+            u.sweep_f32(A, B, ext, h, st, T, dtype == "f32k", run_cfg)
     for _ in range(warmup):
-        u.sweep_f64(A, B, ext, h, st, T, run_cfg)
+        sweep()
     u.synchronize()
     tm = u.Timer()
     tm.start()
     for _ in range(steps):
-        u.sweep_f64(A, B, ext, h, st, T, run_cfg)
+        sweep()
     tm.stop()
     ms = tm.ms()
     u.synchronize()
@@ -145,7 +154,7 @@ def copy_bandwidth(u, nbytes=2 << 30, reps=10):
     return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
 
 
-def batched(u, T=50, probe_T=8):
+def batched(u, T=50, probe_T=8, tune=True, kinds=None):
     """64 uneven 3D grids (extents from {64,96,...,384}), half 3d1r_star and
     half 3d2r_box (125 points), seeded normalised coefficients, T steps: the
     plain per-grid sweeps against the batched graph with the configuration
@@ -153,6 +162,8 @@ def batched(u, T=50, probe_T=8):
     from uxxxxxxxxh_amd.workloads import uneven_batch
     grids, pts = [], 0
     for g in uneven_batch():
+        if kinds and g["case"] not in kinds:
+            continue
         st = u.builtin(g["case"])
         st = st.with_coeffs(u.make_coeffs(st, g["coeff_seed"]))
         ext = g["ext"]
@@ -174,11 +185,14 @@ def batched(u, T=50, probe_T=8):
         tm.stop()
         return tm.ms() / reps
 
-    probe = u.Batch(grids, probe_T)
-    t0 = time.time()
-    best, _, cands = probe.tune()
-    t_tune = time.time() - t0
-    probe.close()
+    if tune:
+        probe = u.Batch(grids, probe_T)
+        t0 = time.time()
+        best, _, cands = probe.tune()
+        t_tune = time.time() - t0
+        probe.close()
+    else:
+        best, cands, t_tune = u.batch_default_config(), [], 0.0
     b0 = u.Batch(grids, T, u.batch_default_config())
     ms_plain = timed(b0.run_each)
     b0.close()
@@ -212,7 +226,7 @@ def batched(u, T=50, probe_T=8):
         A.close()
         B.close()
     upd = float(pts) * T
-    return {"grids": 64, "timesteps": T, "updates": upd,
+    return {"grids": len(grids), "timesteps": T, "updates": upd,
             "plain_ms": round(ms_plain, 3), "plain_GStencils_s": round(upd / ms_plain / 1e6, 2),
             "batched_ms": round(ms_batch, 3), "batched_GStencils_s": round(upd / ms_batch / 1e6, 2),
             "speedup": round(ms_plain / ms_batch, 3),
@@ -446,6 +460,12 @@ def main():
                     help="only the fused multi-kernel programs (all, or those named)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched leg (profiling runs)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batched leg with the default batch configuration (no tuning search)")
+    ap.add_argument("--batch-kinds", default=None,
+                    help="batched leg on the grids of these stencils only (comma-separated)")
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32", "f32k"],
+                    help="precision of --shape-only")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the copy kernel (traffic-counter calibration: 2 GiB read + written)")
     args = ap.parse_args()
@@ -463,13 +483,14 @@ def main():
         # One benchmark stencil at its per-shape size (profiling runs).
         case = args.shape_only
         e = [8192, 8192] if case.startswith("2d") else [512, 512, 512]
-        ms_s, ms_w, c = time_sweeps(u, case, e, args.timesteps, args.steps, args.warmup)
+        ms_s, ms_w, c = time_sweeps(u, case, e, args.timesteps, args.steps, args.warmup, dtype=args.dtype)
         print(json.dumps({"case": case, "GStencils_s": round(float(np.prod(e)) / (ms_w * 1e-3) / 1e9, 2),
                           "ms_per_sweep": round(ms_w, 5), "tile": list(c.tuple()),
                           "copy_GBps": round(copy_bandwidth(u, reps=2), 1)}))
         return
     if args.batch_only:
-        print(json.dumps(batched(u)))
+        kinds = args.batch_kinds.split(",") if args.batch_kinds else None
+        print(json.dumps(batched(u, tune=not args.no_tune, kinds=kinds)))
         return
     if args.programs_only is not None:
         print(json.dumps(programs(u, names=args.programs_only)))

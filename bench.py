@@ -238,6 +238,40 @@ def batched(u, T=50, probe_T=8, tune=True, kinds=None):
             "tasks": info["tasks"], "partition": partition}
 
 
+def batch_graph(u, T, kinds=None, runs=2):
+    """The benchmark batch's graph alone (default configuration): two runs,
+    each timed, with a progress line per run."""
+    from uxxxxxxxxh_amd.workloads import uneven_batch
+    grids, pts = [], 0
+    for g in uneven_batch():
+        if kinds and g["case"] not in kinds:
+            continue
+        st = u.builtin(g["case"])
+        st = st.with_coeffs(u.make_coeffs(st, g["coeff_seed"]))
+        h = st.radius
+        A = u.DeviceArray(u.grid_shape(g["ext"], h), np.float64)
+        B = u.DeviceArray(u.grid_shape(g["ext"], h), np.float64)
+        u.fill_uniform(A, g["ext"], h, g["seed_a"])
+        u.fill_uniform(B, g["ext"], h, g["seed_b"])
+        grids.append((A, B, g["ext"], h, st))
+        pts += int(np.prod(g["ext"]))
+    b = u.Batch(grids, T, u.batch_default_config())
+    log("batch graph", b.info())
+    ms = []
+    for r in range(runs):
+        tm = u.Timer()
+        tm.start()
+        b.run()
+        tm.stop()
+        ms.append(tm.ms())
+        log("batch graph run %d: %.3f ms" % (r, ms[-1]))
+    b.close()
+    for A, B, _, _, _ in grids:
+        A.close()
+        B.close()
+    return {"grids": len(grids), "timesteps": T, "ms": ms, "GStencils_s": pts * T / (min(ms) * 1e-3) / 1e9}
+
+
 # The reference's multi-kernel and one-sided example programs
 # (UHStencil/examples/2d5pt_nested/stencil_2d5pt_nested.dsl,
 # 3d7pt9pt_nested/stencil_3d7pt9pt_nested.dsl,
@@ -460,6 +494,9 @@ def main():
                     help="only the fused multi-kernel programs (all, or those named)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched leg (profiling runs)")
+    ap.add_argument("--batch-graph-only", action="store_true",
+                    help="only the batched graph, default configuration, --timesteps steps, two runs "
+                         "(counter runs: few dispatches, no per-grid legs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="batched leg with the default batch configuration (no tuning search)")
     ap.add_argument("--batch-kinds", default=None,
@@ -487,6 +524,10 @@ def main():
         print(json.dumps({"case": case, "GStencils_s": round(float(np.prod(e)) / (ms_w * 1e-3) / 1e9, 2),
                           "ms_per_sweep": round(ms_w, 5), "tile": list(c.tuple()),
                           "copy_GBps": round(copy_bandwidth(u, reps=2), 1)}))
+        return
+    if args.batch_graph_only:
+        kinds = args.batch_kinds.split(",") if args.batch_kinds else None
+        print(json.dumps(batch_graph(u, args.timesteps, kinds)))
         return
     if args.batch_only:
         kinds = args.batch_kinds.split(",") if args.batch_kinds else None

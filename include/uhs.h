@@ -195,6 +195,17 @@ int uhs_error_norms_f32_f64(const float *a, const double *b, int dims,
 int uhs_error_norms_f64(const double *a, const double *b, int dims,
                         const int64_t *ext, int halo, double out[3],
                         uhs_stream_t stream);
+/* uhs_error_norms_f32_f64 without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels and the same
+ * result, bit for bit; the partial sums live in a static per-device array of
+ * 3 * 1024 + 3 doubles (24.6 KB, held until the process ends), whose size
+ * does not depend on the grid.  Every call on a device shares that array:
+ * calls that overlap on one device (other threads or streams) race.
+ * Synchronous. */
+int uhs_error_norms_f32_f64_noalloc(const float *a, const double *b, int dims,
+                                    const int64_t *ext, int halo, double out[3],
+                                    uhs_stream_t stream);
 
 /* ---- batched grids: one hipGraph, persistent work-queue launches -------- */
 

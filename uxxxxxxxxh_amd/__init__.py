@@ -115,6 +115,7 @@ _SIGS = {
     "uhs_fill_uniform_f32": (_I, [_P, _I, _I64P, _I, ctypes.c_uint64, _P]),
     "uhs_make_coeffs": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     "uhs_error_norms_f32_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
+    "uhs_error_norms_f32_f64_noalloc": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_error_norms_f64": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_parse_dsl": (_I, [ctypes.c_char_p, ctypes.POINTER(DslInfo)]),
     "uhs_parse_dsl_program": (_I, [ctypes.c_char_p, ctypes.POINTER(DslProgram)]),

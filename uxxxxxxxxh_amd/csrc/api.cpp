@@ -646,7 +646,7 @@ static int norms(const void *a, int a_f32, const double *b, int dims, const int6
     GridShape s;
     if (int rc = grid_shape(dims, ext, halo, &s)) return rc;
     if (!a || !b || !out) return fail(UHS_E_INVAL, "null argument");
-    const int nparts = 1024;
+    const int nparts = kNormParts;
     double *dev = nullptr;
     hipError_t e = hipMalloc(&dev, sizeof(double) * (3 * nparts + 3));
     if (e != hipSuccess) return hip_fail(e, "norm workspace");
@@ -668,6 +668,30 @@ int uhs_error_norms_f64(const double *a, const double *b, int dims, const int64_
                         double out[3], uhs_stream_t stream)
 {
     return norms(a, 0, b, dims, ext, halo, out, (hipStream_t)stream);
+}
+
+int uhs_error_norms_f32_f64_noalloc(const float *a, const double *b, int dims, const int64_t *ext,
+                                    int halo, double out[3], uhs_stream_t stream)
+{
+    GridShape s;
+    if (int rc = grid_shape(dims, ext, halo, &s)) return rc;
+    if (!a || !b || !out) return fail(UHS_E_INVAL, "null argument");
+    // The reduction's scratch need is fixed (kNormParts partial triples and
+    // the result), whatever the grid size; it must fit the static array.
+    const int nparts = kNormParts;
+    if (3 * nparts + 3 > kNormScratchDoubles)
+        return fail(UHS_E_NOMEM, "norm scratch of %d doubles exceeds the static %d", 3 * nparts + 3,
+                    kNormScratchDoubles);
+    double *scr = nullptr;
+    hipError_t e = uhs_norms_scratch(&scr);
+    if (e != hipSuccess) return hip_fail(e, "norm scratch address");
+    hipStream_t st = (hipStream_t)stream;
+    // Every k_norms workgroup writes its own triple and k_norms_final writes
+    // the result, so nothing left by an earlier call needs clearing.
+    e = uhs_launch_norms(a, 1, b, s, scr, nparts, scr + 3 * nparts, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, scr + 3 * nparts, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? UHS_OK : hip_fail(e, "error norms");
 }
 
 int uhs_copy_f64(double *dst, const double *src, size_t n, uhs_stream_t stream)

@@ -2,6 +2,8 @@
 #include "uhs_internal.h"
 #include "uhs_util.h"
 
+#include <mutex>
+
 namespace {
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
@@ -172,4 +174,35 @@ hipError_t uhs_launch_norms(const void *a, int a_is_f32, const double *b, const 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(256), 0, st, partial, nparts, out3);
     return hipGetLastError();
+}
+
+// The partial-norm triples and the result of the allocation-free norm entry
+// (uhs_error_norms_f32_f64_noalloc): one copy per device, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: only the address hipGetSymbolAddress returns for the
+// current device is ever passed to a kernel or a copy.
+static __device__ __align__(256) double scratch[kNormScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+} // namespace
+
+hipError_t uhs_norms_scratch(double **ptr)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        // First call on this device (it is current): look its copy up once.
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = (double *)p;
+    }
+    *ptr = g_scratch[dev];
+    return hipSuccess;
 }

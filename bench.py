@@ -157,23 +157,16 @@ def copy_bandwidth(u, nbytes=2 << 30, reps=10):
 def batched(u, T=50, probe_T=8, tune=True, kinds=None):
     """64 uneven 3D grids (extents from {64,96,...,384}), half 3d1r_star and
     half 3d2r_box (125 points), seeded normalised coefficients, T steps: the
-    plain per-grid sweeps against the batched graph with the configuration
-    the batch autotuner picks (timed on a probe_T-step run)."""
+    plain per-grid sweeps against the batched graph.  The batch configuration
+    is the learned cost model's pick: the model is trained on timings of
+    random other batches, predicts every candidate for this batch, and the
+    top 5 are timed on a probe_T-step run; the exhaustive timed search over
+    all candidates is run beside it as the judge of that pick."""
+    from uxxxxxxxxh_amd import costmodel as cm
     from uxxxxxxxxh_amd.workloads import uneven_batch
-    grids, pts = [], 0
-    for g in uneven_batch():
-        if kinds and g["case"] not in kinds:
-            continue
-        st = u.builtin(g["case"])
-        st = st.with_coeffs(u.make_coeffs(st, g["coeff_seed"]))
-        ext = g["ext"]
-        h = st.radius
-        A = u.DeviceArray(u.grid_shape(ext, h), np.float64)
-        B = u.DeviceArray(u.grid_shape(ext, h), np.float64)
-        u.fill_uniform(A, ext, h, g["seed_a"])
-        u.fill_uniform(B, ext, h, g["seed_b"])
-        grids.append((A, B, ext, h, st))
-        pts += int(np.prod(ext))
+    specs = [g for g in uneven_batch() if not kinds or g["case"] in kinds]
+    grids = cm.batch_grids(specs)
+    pts = sum(int(np.prod(g["ext"])) for g in specs)
 
     def timed(fn, reps=2):
         fn()
@@ -185,14 +178,18 @@ def batched(u, T=50, probe_T=8, tune=True, kinds=None):
         tm.stop()
         return tm.ms() / reps
 
+    model_info = None
     if tune:
-        probe = u.Batch(grids, probe_T)
-        t0 = time.time()
-        best, _, cands = probe.tune()
-        t_tune = time.time() - t0
-        probe.close()
+        X, y, t_collect = cm.batch_collect()
+        model, loss, t_train = cm.batch_train(X, y)
+        ev = cm.batch_evaluate(model, grids, probe_T)
+        model.close()
+        best, ex_best = ev.pop("_model_best"), ev.pop("_exhaustive_best")
+        model_info = dict(ev, training_rows=int(len(X)), collect_s=round(t_collect, 1),
+                          train_s=round(t_train, 1), final_train_loss=round(loss, 4))
+        log("batch cost model:", json.dumps(model_info))
     else:
-        best, cands, t_tune = u.batch_default_config(), [], 0.0
+        best = ex_best = u.batch_default_config()
     b0 = u.Batch(grids, T, u.batch_default_config())
     ms_plain = timed(b0.run_each)
     b0.close()
@@ -207,6 +204,12 @@ def batched(u, T=50, probe_T=8, tune=True, kinds=None):
     ms_batch = timed(b.run)
     info = b.info()
     b.close()
+    if bytes(ex_best) != bytes(best):
+        b = u.Batch(grids, T, ex_best)
+        ms_ex = timed(b.run)
+        b.close()
+    else:
+        ms_ex = ms_batch
     # Splitting the batch over 8 GPUs (a model; this run uses one GPU):
     # longest-processing-time-first packing on each grid's measured plain
     # sweep time, and on the library's estimate (points x work per point)
@@ -222,20 +225,22 @@ def batched(u, T=50, probe_T=8, tune=True, kinds=None):
                  "max_over_mean_measured_costs": round(max(load_m) / mean, 4),
                  "max_over_mean_estimated_costs": round(max(load_e) / mean, 4),
                  "grids_per_gpu": [gpu_m.count(k) for k in range(8)]}
-    for A, B, _, _, _ in grids:
-        A.close()
-        B.close()
+    cm.close_grids(grids)
     upd = float(pts) * T
+    cfgd = lambda c: {"star_depth": c.star_depth, "star_tile": [c.star_tile_x, c.star_tile_y, c.star_tile_z],
+                      "box_tile": [c.box_tile_x, c.box_tile_y, c.box_tile_z], "flags": c.flags}
     return {"grids": len(grids), "timesteps": T, "updates": upd,
             "plain_ms": round(ms_plain, 3), "plain_GStencils_s": round(upd / ms_plain / 1e6, 2),
             "batched_ms": round(ms_batch, 3), "batched_GStencils_s": round(upd / ms_batch / 1e6, 2),
             "speedup": round(ms_plain / ms_batch, 3),
+            "tuned_by": "cost model top-5" if tune else "default configuration",
             "per_grid_default_ms": round(ms_default, 3),
             "speedup_vs_per_grid_default": round(ms_default / ms_batch, 3),
-            "config": {"star_depth": best.star_depth, "star_tile": [best.star_tile_x, best.star_tile_y, best.star_tile_z],
-                       "box_tile": [best.box_tile_x, best.box_tile_y, best.box_tile_z], "flags": best.flags},
-            "tuning_candidates": len(cands), "tuning_s": round(t_tune, 2), "launches": info["launches"],
-            "tasks": info["tasks"], "partition": partition}
+            "exhaustive_pick_batched_ms": round(ms_ex, 3),
+            "speedup_exhaustive_pick": round(ms_plain / ms_ex, 3),
+            "config": cfgd(best), "exhaustive_config": cfgd(ex_best),
+            "cost_model": model_info,
+            "launches": info["launches"], "tasks": info["tasks"], "partition": partition}
 
 
 def batch_graph(u, T, kinds=None, runs=2):
@@ -370,6 +375,37 @@ def programs(u, reps=3, names=None):
     return out
 
 
+def costmodel_leg(u, problems_per_shape=6, configs_per_problem=150, epochs=300, topk=5, seed=11):
+    """The learned cost model of the per-grid tiling: timings of random
+    problems of all 12 stencils (FP64, and FP32 plain and compensated),
+    trained on the GPU, then judged on the two held-out problems (3d3r_box FP64
+    384x512x640, 2d3r_star FP64 12288x6144) against the exhaustive timed
+    search, the hand-set default and random-k."""
+    import tempfile
+    from uxxxxxxxxh_amd import costmodel as cm
+    with tempfile.TemporaryDirectory() as d:
+        db = os.path.join(d, "tune_db.csv")
+        t0 = time.time()
+        n = cm.collect(db, problems_per_shape, configs_per_problem, seed)
+        t_collect = time.time() - t0
+        X, y = cm.load_rows(db)
+    idx = np.random.default_rng(seed).permutation(len(X))
+    n_val = max(1, len(X) // 10)
+    val, tr = idx[:n_val], idx[n_val:]
+    m = u.MLP(seed)
+    t1 = time.time()
+    loss = m.train(X[tr], y[tr], epochs=epochs, batch=256, lr=1e-3, fit_norm=True, seed=seed)
+    t_train = time.time() - t1
+    pv = m.predict(X[val])
+    rk = lambda v: np.argsort(np.argsort(v))
+    out = {"timed_configs": n, "rows": int(len(X)), "collect_s": round(t_collect, 1), "train_s": round(t_train, 1),
+           "val_rmse_log_ms": round(float(np.sqrt(np.mean((pv - y[val]) ** 2))), 4),
+           "val_rank_corr": round(float(np.corrcoef(rk(pv), rk(y[val]))[0, 1]), 4),
+           "held_out": cm.evaluate(m, topk, seed)}
+    m.close()
+    return out
+
+
 def fp32_kahan(u):
     from uxxxxxxxxh_amd.fp32study import study
     r = study("3d2r_star", [512, 512, 512], [10, 100, 1000])
@@ -480,6 +516,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-costmodel", action="store_true")
+    ap.add_argument("--costmodel-only", action="store_true", help="only the per-grid cost-model leg")
     ap.add_argument("--no-one-step", action="store_true")
     ap.add_argument("--one-step-only", action="store_true",
                     help="only the one-step-per-launch sweep (its traffic-counter run)")
@@ -525,6 +563,9 @@ def main():
                           "ms_per_sweep": round(ms_w, 5), "tile": list(c.tuple()),
                           "copy_GBps": round(copy_bandwidth(u, reps=2), 1)}))
         return
+    if args.costmodel_only:
+        print(json.dumps(costmodel_leg(u)))
+        return
     if args.batch_graph_only:
         kinds = args.batch_kinds.split(",") if args.batch_kinds else None
         print(json.dumps(batch_graph(u, args.timesteps, kinds)))
@@ -541,7 +582,7 @@ def main():
         return
     if args.headline_only:
         for k in ("no_shapes", "no_one_step", "no_programs", "no_shard", "no_batch", "no_fp32", "no_cpu",
-                  "no_probes"):
+                  "no_probes", "no_costmodel"):
             setattr(args, k, True)
     name = __import__("ctypes").create_string_buffer(256)
     u.check(u.lib().uhs_device_name(name, 256))
@@ -632,6 +673,9 @@ def main():
     if not args.no_batch:
         out["batched"] = batched(u)
         log("batched", out["batched"])
+    if not args.no_costmodel:
+        out["costmodel"] = costmodel_leg(u)
+        log("costmodel", json.dumps(out["costmodel"]))
     if not args.no_fp32:
         out["fp32_kahan"] = fp32_kahan(u)
         log("fp32", out["fp32_kahan"])

@@ -75,17 +75,19 @@ int uhs_batch_tune(const uhs_grid_desc *grids, int n, int timesteps,
 
 typedef struct uhs_mlp *uhs_mlp_t;
 
-/* Feature vector of (problem, configuration): dims one-hot (2), radius
- * one-hot (3), star/box one-hot (2), log2 extents (3, axis 0 first, 0 for a
- * missing axis), log2 tile x/y/z (3), log2 threads per workgroup (1),
- * points per thread (1), temporal depth one-hot {1,2,4} (3; UHS_CFG_TEMPORAL2
- * counts as depth 2), non-temporal store flag (1), kernel variant (1: 0
- * LDS-staged, 1 deeper prefetch (two planes for stars and two rows for 2D,
- * three cells for boxes), 2 four rows of prefetch (2D), 3 register-tiled, 4 two
- * rows per thread, 5 two planes of box loads in flight, 6 both 4 and 5).
- * Threads per workgroup follow the variant (ghost zone included for
- * UHS_CFG_TEMPORAL2). */
-int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext,
+/* Feature vector of (problem, configuration), 20 inputs: 3D flag (1),
+ * radius one-hot (3), star flag (1), log2 extents (3, axis 0 first, 0 for a
+ * missing axis), log2 tile x/y/z (3), log2 threads per workgroup (1), points
+ * per thread (1), temporal depth one-hot {1,2,4} (3; UHS_CFG_TEMPORAL2
+ * counts as depth 2), log2 of the stencil's term count (1; 2 dims r + 1 for a
+ * star, (2r+1)^dims for a box), precision (1: 0 FP64, 1 FP32, 2 FP32
+ * compensated), non-temporal store flag (1), kernel variant (1: 0 LDS-staged,
+ * 1 deeper prefetch (two planes for stars and two rows for 2D, three cells
+ * for boxes), 2 four rows of prefetch (2D), 3 register-tiled, 4 two rows per
+ * thread, 5 two planes of box loads in flight, 6 both 4 and 5, 7 a register
+ * ring of rows (2D)).  Threads per workgroup follow the variant (ghost zone
+ * included for UHS_CFG_TEMPORAL2). */
+int uhs_mlp_features(int dims, int radius, int star, int dtype, const int64_t *ext,
                      const uhs_config *cfg, int depth, float *out20);
 
 int uhs_mlp_create(uint64_t seed, uhs_mlp_t *out);
@@ -122,6 +124,35 @@ int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *ext,
                          int halo, const uhs_stencil *st, int dtype,
                          uhs_mlp_t model, int topk, int reps,
                          const char *db_path, uhs_tune_result *out,
+                         uhs_stream_t stream);
+
+/* ---- batch configurations: candidate space, timed search, model --------- */
+
+/* The candidate batch configurations uhs_batch_tune times: returns their
+ * number and copies up to cap of them into out (out may be NULL). */
+int uhs_batch_space(uhs_batch_config *out, int cap);
+/* Timed search over an explicit list of batch configurations (each planned,
+ * captured, run once to warm up and once timed for `timesteps` steps; grid
+ * contents are overwritten).  ms_out (optional, ncand entries) receives each
+ * candidate's time, < 0 for candidates that could not be built. */
+int uhs_batch_tune_list(const uhs_grid_desc *grids, int n, int timesteps,
+                        const uhs_batch_config *cands, int ncand, uhs_batch_config *best,
+                        double *best_ms, double *ms_out, uhs_stream_t stream);
+/* Feature vector of (batch, configuration) for the cost model, 20 inputs:
+ * star depth one-hot {1,2,4} (3), log2 star tile x/y/z (3), log2 box tile
+ * x/y/z (3), box two-rows flag (1), box two-planes flag (1), log2 interior
+ * points of the star grids and of the box grids (2, 0 when none), log2
+ * number of grids (1), mean log2 extent per axis (3), spread (standard
+ * deviation) of the log2 extents (1), log2 (1 + mean terms per point of the
+ * star grids + of the box grids) (1), star tile takes the register-queue
+ * two-step form (1).  The model maps it to log(ms per step). */
+int uhs_batch_features(const uhs_grid_desc *grids, int n, const uhs_batch_config *cfg,
+                       float *out20);
+/* Model-pruned batch search: predicts every candidate of uhs_batch_space,
+ * times the topk predicted fastest (as uhs_batch_tune_list) and returns the
+ * fastest; *seconds (optional) is the wall time including the inference. */
+int uhs_batch_tune_model(const uhs_grid_desc *grids, int n, int timesteps, uhs_mlp_t model,
+                         int topk, uhs_batch_config *best, double *best_ms, double *seconds,
                          uhs_stream_t stream);
 
 /* Timed search over the streaming program kernel's tiles

@@ -20,16 +20,20 @@ def test_numpy_backprop_matches_finite_differences():
 
 
 def test_feature_encoding(uhs):
+    L = np.log2
+    # 3D radius-2 box (125 terms), two planes of prefetch, non-temporal stores, depth 4
     f = uhs.mlp_features(3, 2, False, [64, 128, 256], (64, 8, 32, 2, 3), depth=4)
-    assert f.tolist() == [0, 1, 0, 1, 0, 0, 1, 6, 7, 8, 6, 3, 5, 8, 2, 0, 0, 1, 1, 1]
+    assert np.allclose(f, [1, 0, 1, 0, 0, 6, 7, 8, 6, 3, 5, 8, 2, 0, 0, 1, L(125), 0, 1, 1])
+    # 2D 5-point star
     f = uhs.mlp_features(2, 1, True, [1024, 2048], (512, 1, 64, 1, 0))
-    assert f.tolist() == [1, 0, 1, 0, 0, 1, 0, 10, 11, 0, 9, 0, 6, 9, 1, 1, 0, 0, 0, 0]
+    assert np.allclose(f, [0, 1, 0, 0, 1, 10, 11, 0, 9, 0, 6, 9, 1, 1, 0, 0, L(5), 0, 0, 0])
     # two steps per launch: depth 2, threads over the ghost-zone tile (64/2 * 8)
     f = uhs.mlp_features(3, 2, True, [64, 128, 256], (60, 4, 64, 2, 33))
-    assert np.allclose(f, [0, 1, 0, 1, 0, 1, 0, 6, 7, 8, np.log2(60), 2, 6, 8, 2, 0, 1, 0, 1, 0])
-    # two rows per thread + two planes in flight: 64 * 16 / 2 threads, variant 6
-    f = uhs.mlp_features(3, 1, False, [64, 128, 256], (64, 16, 128, 1, 80))
-    assert f.tolist() == [0, 1, 1, 0, 0, 0, 1, 6, 7, 8, 6, 4, 7, 9, 1, 1, 0, 0, 0, 6]
+    assert np.allclose(f, [1, 0, 1, 0, 1, 6, 7, 8, L(60), 2, 6, 8, 2, 0, 1, 0, L(13), 0, 1, 0])
+    # two rows per thread + two planes in flight: 64 * 16 / 2 threads, variant 6;
+    # FP32 compensated
+    f = uhs.mlp_features(3, 1, False, [64, 128, 256], (64, 16, 128, 1, 80), dtype=2)
+    assert np.allclose(f, [1, 1, 0, 0, 0, 6, 7, 8, 6, 4, 7, 9, 1, 1, 0, 0, L(27), 2, 0, 6])
 
 
 def test_parameter_count(uhs):

@@ -144,6 +144,14 @@ _SIGS = {
                                   ctypes.POINTER(TuneResult), _P]),
     "uhs_tune_sweep_list": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), _I, _I,
                                  ctypes.c_char_p, ctypes.POINTER(TuneResult), _P]),
+    "uhs_batch_space": (_I, [ctypes.POINTER(BatchConfig), _I]),
+    "uhs_batch_tune_list": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(BatchConfig), _I,
+                                 ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_double), _P]),
+    "uhs_batch_features": (_I, [ctypes.POINTER(GridDesc), _I, ctypes.POINTER(BatchConfig),
+                                ctypes.POINTER(ctypes.c_float)]),
+    "uhs_batch_tune_model": (_I, [ctypes.POINTER(GridDesc), _I, _I, _P, _I, ctypes.POINTER(BatchConfig),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_batch_tune": (_I, [ctypes.POINTER(GridDesc), _I, _I, ctypes.POINTER(BatchConfig),
                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_double),
                             _I, ctypes.POINTER(_I), _P]),
@@ -152,7 +160,7 @@ _SIGS = {
                            ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_tune_load_db": (_I, [ctypes.c_char_p]),
     "uhs_tune_clear": (_I, []),
-    "uhs_mlp_features": (_I, [_I, _I, _I, _I64P, ctypes.POINTER(Config), _I, ctypes.POINTER(ctypes.c_float)]),
+    "uhs_mlp_features": (_I, [_I, _I, _I, _I, _I64P, ctypes.POINTER(Config), _I, ctypes.POINTER(ctypes.c_float)]),
     "uhs_mlp_create": (_I, [ctypes.c_uint64, ctypes.POINTER(_P)]),
     "uhs_mlp_destroy": (_I, [_P]),
     "uhs_mlp_num_params": (_I, []),
@@ -566,6 +574,45 @@ class Batch:
             pass
 
 
+def _grid_descs(grids):
+    """GridDesc array for (A, B, ext, halo, StencilSpec) or (ext, halo,
+    StencilSpec) tuples (the latter carry no device memory: only for calls
+    that touch none)."""
+    arr = (GridDesc * len(grids))()
+    for i, g in enumerate(grids):
+        ext, halo, st = g[-3], g[-2], g[-1]
+        A = g[0].ptr.value if len(g) == 5 else None
+        B = g[1].ptr.value if len(g) == 5 else None
+        arr[i] = GridDesc(A, B, 3, (ctypes.c_int64 * 3)(*ext), halo, st.c)
+    return arr
+
+
+def batch_features(grids, cfg):
+    """Cost-model features of (batch, configuration) (uhs_batch_features)."""
+    out = np.zeros(20, np.float32)
+    check(lib().uhs_batch_features(_grid_descs(grids), len(grids), ctypes.byref(cfg), _fp(out)))
+    return out
+
+
+def batch_tune_list(grids, timesteps, cands, stream=None):
+    """Times batch configurations on (A, B, ext, halo, StencilSpec) grids
+    (contents overwritten); returns (best, best_ms, [ms per candidate])."""
+    arr = (BatchConfig * len(cands))(*cands)
+    best, bms = BatchConfig(), ctypes.c_double()
+    ms = (ctypes.c_double * len(cands))()
+    check(lib().uhs_batch_tune_list(_grid_descs(grids), len(grids), timesteps, arr, len(cands),
+                                    ctypes.byref(best), ctypes.byref(bms), ms, stream))
+    return best, bms.value, list(ms)
+
+
+def batch_tune_model(grids, timesteps, model, topk=5, stream=None):
+    """Model-pruned batch search; returns (best, best_ms, seconds)."""
+    best, bms, sec = BatchConfig(), ctypes.c_double(), ctypes.c_double()
+    check(lib().uhs_batch_tune_model(_grid_descs(grids), len(grids), timesteps, model.h, topk, ctypes.byref(best),
+                                     ctypes.byref(bms), ctypes.byref(sec), stream))
+    return best, bms.value, sec.value
+
+
 def batch_partition(grids, ngpus, cost=None):
     """Longest-processing-time-first split of a batch over ngpus GPUs
     (uhs_batch_partition).  grids: (ext, halo, StencilSpec) or (A, B, ext,
@@ -582,6 +629,13 @@ def batch_partition(grids, ngpus, cost=None):
     c = (ctypes.c_double * n)(*cost) if cost is not None else None
     check(lib().uhs_batch_partition(arr, n, ngpus, c, gpu, load))
     return list(gpu), list(load)[:ngpus]
+
+
+def batch_space():
+    n = check(lib().uhs_batch_space(None, 0))
+    arr = (BatchConfig * n)()
+    check(lib().uhs_batch_space(arr, n))
+    return [arr[i] for i in range(n)]
 
 
 def batch_default_config():
@@ -633,11 +687,12 @@ def tune_time(A, B, ext, halo, stencil, cfg, dtype=DT_F64, reps=3, stream=None):
     return ms.value
 
 
-def mlp_features(dims, radius, star, ext, cfg, depth=1):
+def mlp_features(dims, radius, star, ext, cfg, depth=1, dtype=DT_F64):
     out = np.zeros(20, np.float32)
     c = cfg if isinstance(cfg, Config) else Config(*cfg)
     e = list(ext) + [0] * (3 - len(ext))
-    check(lib().uhs_mlp_features(dims, radius, 1 if star else 0, ext_array(e), ctypes.byref(c), depth, _fp(out)))
+    check(lib().uhs_mlp_features(dims, radius, 1 if star else 0, dtype, ext_array(e), ctypes.byref(c), depth,
+                                 _fp(out)))
     return out
 
 

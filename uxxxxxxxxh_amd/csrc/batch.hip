@@ -235,9 +235,22 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         if (b.star) {
             g.TX = cfg.star_tile_x; g.TY = cfg.star_tile_y;
             const int R = b.radius, bx = g.TX + 2 * R, by = g.TY + 2 * R;
-            if (g.TX * g.TY == 2 * kBatchThreads && g.TX % 2 == 0 &&
-                2 * R * (g.TX + g.TY) <= kStarHaloPerThread * kBatchThreads)
+            // Single steps: the star tile when it is a register-queue tile;
+            // in a temporally blocked batch, whose only single steps are the
+            // few leading ones that keep the result in the right grid, a
+            // 64 x 8 register-queue tile otherwise (the LDS-ring form is
+            // several times slower per step).
+            auto rq1 = [&](int tx, int ty) {
+                return tx * ty == 2 * kBatchThreads && tx % 2 == 0 &&
+                       2 * R * (tx + ty) <= kStarHaloPerThread * kBatchThreads;
+            };
+            if (rq1(g.TX, g.TY)) {
                 g.mode |= kModeRq1;
+                g.TX1 = g.TX; g.TY1 = g.TY;
+            } else if (K > 1 && rq1(64, 8)) {
+                g.mode |= kModeRq1;
+                g.TX1 = 64; g.TY1 = 8;
+            }
             if (K == 2 && bx % 2 == 0 && bx / 2 * by <= kBatchThreads &&
                 2 * R * (bx + by) <= kBatchRingCells * kBatchThreads)
                 g.mode |= kModeRq2;
@@ -287,9 +300,12 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
             const int depth = b.star ? ssched[l] : 1;
             const double *src = (l & 1) ? g.B : g.A;
             const double *dst = (l & 1) ? g.A : g.B;
+            // The tile this launch's tasks of the grid cover.
+            const bool t1 = b.star && depth == 1 && (g.mode & kModeRq1);
+            const int TX = t1 ? g.TX1 : g.TX, TY = t1 ? g.TY1 : g.TY;
             for (int z = 0; z < g.n0; z += tz)
-                for (int y = 0; y < g.n1; y += g.TY)
-                    for (int x = 0; x < g.n2; x += g.TX) {
+                for (int y = 0; y < g.n1; y += TY)
+                    for (int x = 0; x < g.n2; x += TX) {
                         BTask t{};
                         t.src = src + origin;
                         t.dst = const_cast<double *>(dst) + origin;
@@ -299,7 +315,7 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
                         t.k0 = k0;
                         const int gl = b.star ? slot[l] : l;
                         per[gl].push_back(t);
-                        const double pts = (double)std::min(g.TX, g.n2 - x) * std::min(g.TY, g.n1 - y) *
+                        const double pts = (double)std::min(TX, g.n2 - x) * std::min(TY, g.n1 - y) *
                                            (t.ze - t.zs);
                         cost[gl].push_back(pts * (b.star ? 1.0 : b.npoints / 8.0) * depth);
                     }
@@ -353,7 +369,7 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
                 const size_t wc = (size_t)(g.TX + 2 * t.depth * R) * (g.TY + 2 * t.depth * R);
                 need = sizeof(double) * t.depth * (2 * R + 1) * wc;
                 if (t.depth == 1 && (g.mode & kModeRq1)) // star3d_tile: two staged planes
-                    need = 2 * sizeof(double) * (size_t)(g.TX + 2 * R) * (g.TY + 2 * R);
+                    need = 2 * sizeof(double) * (size_t)(g.TX1 + 2 * R) * (g.TY1 + 2 * R);
                 if (t.depth == 2 && (g.mode & kModeRq2)) { // star3d_tb2_tile
                     const size_t bx = g.TX + 2 * R, by = g.TY + 2 * R;
                     need = 2 * sizeof(double) * ((bx + 2 * R) * (by + 2 * R) + bx * by);
@@ -489,76 +505,6 @@ int uhs_batch_info(uhs_batch_t b, int *launches, long long *tasks, int *max_work
     if (tasks) *tasks = b->ntasks;
     if (max_workgroups) *max_workgroups = wg;
     if (max_lds) *max_lds = lds;
-    return UHS_OK;
-}
-
-int uhs_batch_tune(const uhs_grid_desc *grids, int n, int timesteps, uhs_batch_config *best, double *best_ms,
-                   uhs_batch_config *cands_out, double *ms_out, int ncand_cap, int *ncand, uhs_stream_t stream)
-{
-    if (!grids || n < 1 || !best) return berr(UHS_E_INVAL, "batch tuning needs grids and an output");
-    // Star tiles per depth: 512-point tiles (register-queue single steps),
-    // tiles whose ghost-zone extension fits 256 threads (register-queue two
-    // steps), and for depth 4 the LDS-ring form.  Box tiles: 256 points, or
-    // 512 with two rows per thread; with one or two planes in flight.
-    static const int s1[][2] = {{64, 8}, {128, 4}, {32, 16}};
-    static const int s2[][2] = {{62, 6}, {30, 14}, {32, 12}, {32, 8}, {64, 4}};
-    static const int s4[][2] = {{32, 8}};
-    static const int stz[] = {64, 512};
-    static const int btiles[][3] = {{32, 8, 0},
-                                    {64, 4, 0},
-                                    {32, 16, UHS_BATCH_BOX_ROWS2},
-                                    {64, 8, UHS_BATCH_BOX_ROWS2},
-                                    {16, 32, UHS_BATCH_BOX_ROWS2}};
-    static const int btz[] = {64, 128};
-    static const int bpl[] = {0, UHS_BATCH_BOX_PLANES2};
-    std::vector<uhs_batch_config> cand;
-    auto add = [&](int d, const int (*tiles)[2], int nt) {
-        for (int i = 0; i < nt; i++)
-            for (int sz : stz)
-                for (auto &b : btiles)
-                    for (int bz : btz)
-                        for (int pl : bpl)
-                            cand.push_back(uhs_batch_config{d, tiles[i][0], tiles[i][1], sz, b[0], b[1], bz, b[2] | pl});
-    };
-    add(1, s1, 3);
-    add(2, s2, 5);
-    add(4, s4, 1);
-    double bms = 1e30;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-        return berr(UHS_E_HIP, "events");
-    for (size_t i = 0; i < cand.size(); i++) {
-        double ms = -1.0;
-        uhs_batch_t b = nullptr;
-        if (uhs_batch_create(grids, n, timesteps, &cand[i], &b) == UHS_OK) {
-            float t = 0.f;
-            bool ok = uhs_batch_run(b, stream) == UHS_OK; // warm-up
-            ok = ok && hipEventRecord(e0, (hipStream_t)stream) == hipSuccess;
-            ok = ok && uhs_batch_run(b, stream) == UHS_OK;
-            ok = ok && hipEventRecord(e1, (hipStream_t)stream) == hipSuccess;
-            ok = ok && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess;
-            uhs_batch_destroy(b);
-            if (!ok) {
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                return berr(UHS_E_HIP, "batch run failed during tuning");
-            }
-            ms = t;
-            if (ms < bms) {
-                bms = ms;
-                *best = cand[i];
-            }
-        }
-        if ((int)i < ncand_cap) {
-            if (cands_out) cands_out[i] = cand[i];
-            if (ms_out) ms_out[i] = ms;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (ncand) *ncand = (int)cand.size();
-    if (bms >= 1e30) return berr(UHS_E_CONFIG, "no batch configuration could be built");
-    if (best_ms) *best_ms = bms;
     return UHS_OK;
 }
 

@@ -31,7 +31,7 @@ struct BGrid {
     int kind;             // built-in stencil id (kBuiltins index)
     int TX, TY;           // tile
     int mode;             // kMode* bits: which tile code runs this grid's tasks
-    int pad;
+    int TX1, TY1;         // stars: the tile of single-step tasks (kModeRq1)
     double coef[UHS_MAX_POINTS]; // term order (stars) or box_pack_index order (boxes)
 };
 
@@ -48,7 +48,9 @@ struct BTask {
 };
 
 // BGrid::mode bits.  Stars: kModeRq1 = single steps use the register-queue
-// tile (star3d_tile, 2 points per thread, tile = 512 points); kModeRq2 = two-
+// tile (star3d_tile, 2 points per thread, tile TX1 x TY1 = 512 points: the
+// batch's star tile when it has 512 points, 64 x 8 for the leading single
+// steps of a temporally blocked batch otherwise); kModeRq2 = two-
 // step tasks use the register-queue two-step tile (star3d_tb2_tile, 2 columns
 // per thread over the tile plus its ghost zone).  Boxes: kModeRows2 = two
 // rows per thread (tile = 512 points), kModePlanes2 = two planes of tile
@@ -174,7 +176,7 @@ __device__ __forceinline__ void run_task(const BGrid &g, const BTask &t, double 
             // buffer) and the one-step tiles may read halos from it.
             if constexpr ((F & kFormRq1) != 0) {
                 if (g.mode & kModeRq1) {
-                    const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, g.TX / 2, false};
+                    const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX1, g.TY1, g.TX1 / 2, false};
                     star3d_tile<double, S, 2, false, 1>(t.src, t.dst, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
                     return;
                 }
@@ -187,9 +189,13 @@ __device__ __forceinline__ void run_task(const BGrid &g, const BTask &t, double 
                     const long long origin = (long long)g.halo * g.sz + (long long)g.halo * g.sy + g.halo;
                     const double *hA = g.A + origin, *hB = g.B + origin;
                     const TileGeom geo{g.n0, g.n1, g.n2, g.sy, g.sz, g.TX, g.TY, (g.TX + 2 * R) / 2, false};
-                    star3d_tb2_tile<double, S, 2, false, 1, kBatchRingCells>(t.src, t.dst, (t.k0 & 1) ? hB : hA,
-                                                             (t.k0 & 1) ? hA : hB, geo, t.x0, t.y0, t.zs, t.ze,
-                                                             lds, cf);
+                    const double *h0 = (t.k0 & 1) ? hB : hA, *h1 = (t.k0 & 1) ? hA : hB;
+                    if (star3d_tb2_full<R>(t.x0, t.y0, t.zs, t.ze, g.TX, g.TY, g.n0, g.n1, g.n2))
+                        star3d_tb2_tile<double, S, 2, false, 1, kBatchRingCells, true>(
+                            t.src, t.dst, h0, h1, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
+                    else
+                        star3d_tb2_tile<double, S, 2, false, 1, kBatchRingCells, false>(
+                            t.src, t.dst, h0, h1, geo, t.x0, t.y0, t.zs, t.ze, lds, cf);
                     return;
                 }
             }

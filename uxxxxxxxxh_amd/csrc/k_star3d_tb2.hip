@@ -24,8 +24,12 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d_tb2(TB2Params<T, S::P> t
     const T *h0 = (tp.k0 & 1) ? tp.hB : tp.hA;
     const T *h1 = (tp.k0 & 1) ? tp.hA : tp.hB;
     Coefs<T, S::P> cf(p);
-    star3d_tb2_tile<T, S, VX, KAHAN, PF, kTb2RingCells>(p.in, p.out, h0, h1, g, tile_x * p.TX, tile_y * p.TY, zs,
-                                     min(zs + p.ZC, p.n0), reinterpret_cast<T *>(smem_raw), cf);
+    const int x0 = tile_x * p.TX, y0 = tile_y * p.TY, ze = min(zs + p.ZC, p.n0);
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    if (star3d_tb2_full<S::R>(x0, y0, zs, ze, p.TX, p.TY, p.n0, p.n1, p.n2))
+        star3d_tb2_tile<T, S, VX, KAHAN, PF, kTb2RingCells, true>(p.in, p.out, h0, h1, g, x0, y0, zs, ze, lds, cf);
+    else
+        star3d_tb2_tile<T, S, VX, KAHAN, PF, kTb2RingCells, false>(p.in, p.out, h0, h1, g, x0, y0, zs, ze, lds, cf);
 }
 
 template <typename T, class S, int VX, bool KAHAN, int PF>

@@ -255,28 +255,33 @@ extern "C" {
 
 int uhs_mlp_num_params(void) { return NPARAM; }
 
-int uhs_mlp_features(int dims, int radius, int star, const int64_t *ext, const uhs_config *c, int depth,
-                     float *f)
+int uhs_mlp_features(int dims, int radius, int star, int dtype, const int64_t *ext, const uhs_config *c,
+                     int depth, float *f)
 {
-    if (!ext || !c || !f || (dims != 2 && dims != 3) || radius < 1 || radius > 3)
+    if (!ext || !c || !f || (dims != 2 && dims != 3) || radius < 1 || radius > 3 || dtype < 0 || dtype > 2)
         return uhs_set_error(UHS_E_INVAL, "bad feature request");
     memset(f, 0, sizeof(float) * IN);
-    f[dims == 2 ? 0 : 1] = 1.f;
-    f[2 + radius - 1] = 1.f;
-    f[star ? 5 : 6] = 1.f;
-    for (int k = 0; k < dims; k++) f[7 + k] = log2f((float)ext[k]);
-    f[10] = log2f((float)c->tile_x);
-    f[11] = log2f((float)c->tile_y);
-    f[12] = log2f((float)c->tile_z);
+    f[0] = dims == 3 ? 1.f : 0.f;
+    f[1 + radius - 1] = 1.f;
+    f[4] = star ? 1.f : 0.f;
+    for (int k = 0; k < dims; k++) f[5 + k] = log2f((float)ext[k]);
+    f[8] = log2f((float)c->tile_x);
+    f[9] = log2f((float)c->tile_y);
+    f[10] = log2f((float)c->tile_z);
     const int v = std::max(1, c->vec);
     int threads = c->tile_x / v * c->tile_y;
     if (c->flags & UHS_CFG_REGTILE) threads = 64 * (c->tile_y / v);
     else if (c->flags & UHS_CFG_ROWS2) threads = c->tile_x * c->tile_y / 2;
     else if (c->flags & UHS_CFG_TEMPORAL2) threads = (c->tile_x + 2 * radius) / v * (c->tile_y + 2 * radius);
-    f[13] = log2f((float)std::max(1, threads));
-    f[14] = (float)c->vec;
+    f[11] = log2f((float)std::max(1, threads));
+    f[12] = (float)c->vec;
     if (c->flags & UHS_CFG_TEMPORAL2) depth = std::max(depth, 2);
-    f[15 + (depth >= 4 ? 2 : depth >= 2 ? 1 : 0)] = 1.f;
+    f[13 + (depth >= 4 ? 2 : depth >= 2 ? 1 : 0)] = 1.f;
+    // Terms of the stencil: a full star has 2 dims r + 1, a full box (2r+1)^dims.
+    const int side = 2 * radius + 1;
+    const int P = star ? 2 * dims * radius + 1 : (dims == 3 ? side * side * side : side * side);
+    f[16] = log2f((float)P);
+    f[17] = (float)dtype;
     f[18] = (c->flags & UHS_CFG_NT_STORE) ? 1.f : 0.f;
     const bool rows2 = c->flags & UHS_CFG_ROWS2, planes2 = c->flags & UHS_CFG_PLANES2;
     f[19] = (c->flags & UHS_CFG_ROWRING) ? 7.f : rows2 && planes2 ? 6.f : planes2 ? 5.f : rows2 ? 4.f
@@ -530,7 +535,7 @@ extern "C" int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *e
     std::vector<float> feat(sp.size() * IN), pred(sp.size());
     const BuiltinStencil &b = kBuiltins[pl.prog];
     for (size_t i = 0; i < sp.size(); i++)
-        if ((rc = uhs_mlp_features(dims, b.radius, b.star, ext, &sp[i], 1, &feat[i * IN]))) return rc;
+        if ((rc = uhs_mlp_features(dims, b.radius, b.star, dtype, ext, &sp[i], 1, &feat[i * IN]))) return rc;
     if ((rc = uhs_mlp_predict(model, feat.data(), (int)sp.size(), pred.data()))) return rc;
     std::vector<size_t> ord(sp.size());
     std::iota(ord.begin(), ord.end(), 0);

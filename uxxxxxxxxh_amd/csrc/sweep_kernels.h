@@ -359,7 +359,21 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
 // never writes halos), passed as hA/hB with the global step k0 of the
 // input.  Every update is the product-then-fma chain in term order, so the
 // result is bitwise identical to two single sweeps.
-template <typename T, class S, int VX, bool KAHAN, int PF, int NH = kStarHaloPerThread, class CF>
+// FULL: every level-0 cell the tile reads (the extended tile, its ring and
+// the slab's 2R planes either side) lies in the interior, so no value comes
+// from a halo: the loads take the plane's base address plus a per-lane
+// offset, with no clamp or halo select, and no level-1 halo is loaded
+// (star3d_tb2_full tells whether a tile qualifies).
+template <int R>
+__device__ __forceinline__ bool star3d_tb2_full(int x0, int y0, int zs, int ze, int TX, int TY, int n0, int n1,
+                                                int n2)
+{
+    return x0 >= 2 * R && x0 + TX + 2 * R <= n2 && y0 >= 2 * R && y0 + TY + 2 * R <= n1 && zs >= 2 * R &&
+           ze + 2 * R <= n0;
+}
+
+template <typename T, class S, int VX, bool KAHAN, int PF, int NH = kStarHaloPerThread, bool FULL = false,
+          class CF>
 __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__restrict__ out,
                                                 const T *__restrict__ h0, const T *__restrict__ h1,
                                                 const TileGeom &g, int x0, int y0, int zs, int ze,
@@ -385,7 +399,7 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
 #pragma unroll
     for (int v = 0; v < VX; v++) {
         const int ex = tx + v * BXT, xg = ox + ex;
-        xyin[v] = col && xg >= 0 && xg < n2 && yg >= 0 && yg < n1;
+        xyin[v] = col && (FULL || (xg >= 0 && xg < n2 && yg >= 0 && yg < n1));
         core[v] = xyin[v] && ex >= R && ex < BX - R && ty >= R && ty < BY - R;
         coff[v] = clampi(yg, -R, n1 + R - 1) * sy + clampi(xg, -R, n2 + R - 1);
         l0c[v] = (ty + R) * LW + ex + R;
@@ -415,17 +429,24 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
             }
             const int gy = oy - R + ly, gx = ox - R + lx;
             hl[k] = ly * LW + lx;
-            hin[k] = gy >= 0 && gy < n1 && gx >= 0 && gx < n2;
-            hoff[k] = clampi(gy, -R, n1 + R - 1) * sy + clampi(gx, -R, n2 + R - 1);
+            hin[k] = FULL || (gy >= 0 && gy < n1 && gx >= 0 && gx < n2);
+            hoff[k] = FULL ? gy * sy + gx : clampi(gy, -R, n1 + R - 1) * sy + clampi(gx, -R, n2 + R - 1);
         }
     }
     // Level-0 value of a cell at plane z: the input inside the interior, the
     // level-0 halo outside it.  Planes beyond the allocation are clamped;
     // their values feed only level-1 cells outside the interior, which take
-    // the level-1 halo instead.
+    // the level-1 halo instead.  In a FULL tile every plane the values are
+    // used from is interior; only the prefetch past the slab's last plane is
+    // clamped (its value is never used).
     auto lev0 = [&](bool cin, int z, int off) -> T {
-        const long long a = (long long)clampi(z, -R, n0 + R - 1) * sz + off;
-        return (cin && z >= 0 && z < n0) ? in[a] : h0[a];
+        if constexpr (FULL) {
+            const T *pz = in + (long long)min(z, n0 + R - 1) * sz;
+            return pz[off];
+        } else {
+            const long long a = (long long)clampi(z, -R, n0 + R - 1) * sz + off;
+            return (cin && z >= 0 && z < n0) ? in[a] : h0[a];
+        }
     };
 
     const int z1s = zs - R;             // first level-1 plane
@@ -445,7 +466,7 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
     // takes it (outside the interior, or z1 outside it), loaded one
     // iteration ahead into a register of its own.
     T r0[VX][M0], rh[2][NH], r1[VX][M0], rh1[VX][2];
-    auto takes_halo1 = [&](int v, int z) { return !(xyin[v] && z >= 0 && z < n0); };
+    auto takes_halo1 = [&](int v, int z) { return !FULL && !(xyin[v] && z >= 0 && z < n0); };
     auto halo1 = [&](int v, int z) { return h1[(long long)clampi(z, -R, n0 + R - 1) * sz + coff[v]]; };
 #pragma unroll
     for (int v = 0; v < VX; v++) {
@@ -518,11 +539,11 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
 #pragma unroll
             for (int v = 0; v < VX; v++)
                 if (takes_halo1(v, z1 + 1)) rh1[v][(PH + 1) & 1] = halo1(v, z1 + 1);
-            const bool zin = z1 >= 0 && z1 < n0;
+            const bool zin = FULL || (z1 >= 0 && z1 < n0);
 #pragma unroll
             for (int v = 0; v < VX; v++) {
                 T v1;
-                if (zin && xyin[v]) {
+                if (FULL || (zin && xyin[v])) {
                     Acc<T, KAHAN> a;
                     static_for<P>([&](auto tc) {
                         constexpr int t = decltype(tc)::value;

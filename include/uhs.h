@@ -232,6 +232,18 @@ typedef struct uhs_grid_desc {
  *   UHS_BATCH_BOX_PLANES2 keeps two planes of tile loads in flight. */
 #define UHS_BATCH_BOX_ROWS2 1
 #define UHS_BATCH_BOX_PLANES2 2
+/* UHS_BATCH_SPLIT: star grids and box grids run in launches of their own
+ * instead of sharing each launch.  For a batch of 7-point stars and 125-point
+ * boxes (the benchmark mix) the star launches may then take star tiles that
+ * need 512-thread workgroups: the star tile rules above hold with 512 for
+ * 256 (single steps: a 1024-point tile; two steps: (tile_x + 2r) / 2 *
+ * (tile_y + 2r) <= 512), and such tiles get 512-thread workgroups.
+ * UHS_BATCH_CONCURRENT (with UHS_BATCH_SPLIT): the star launches form a
+ * second chain of the graph beside the box launches (forked after the
+ * counter reset, joined at the end), so HBM-bound star tiles and
+ * FP64-bound box tiles run at the same time. */
+#define UHS_BATCH_SPLIT 4
+#define UHS_BATCH_CONCURRENT 8
 typedef struct uhs_batch_config {
     int star_depth;
     int star_tile_x, star_tile_y, star_tile_z;
@@ -244,7 +256,8 @@ typedef struct uhs_batch *uhs_batch_t;
 /* Plans T timesteps of every grid, uploads the task lists and captures the
  * whole run as one hipGraph (a counter-reset node, then one persistent
  * launch per time block in which workgroups take tiles of any grid from a
- * device-side queue, largest first).  Grids keep the ping-pong convention:
+ * device-side queue, largest first; with UHS_BATCH_SPLIT the star and box
+ * grids have launches of their own).  Grids keep the ping-pong convention:
  * after the run each grid's result is in A for even T, in B for odd T; the
  * other buffer holds an earlier state.  cfg NULL = the default. */
 int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps,
@@ -259,6 +272,10 @@ int uhs_batch_run(uhs_batch_t b, uhs_stream_t stream);
 int uhs_batch_destroy(uhs_batch_t b);
 int uhs_batch_info(uhs_batch_t b, int *launches, long long *tasks,
                    int *max_workgroups, size_t *max_lds_bytes);
+/* The default for a batch of 7-point stars and 125-point boxes (the
+ * benchmark mix): {2, 62,14,32, 32,16,128, ROWS2 | PLANES2 | SPLIT |
+ * CONCURRENT}.  uhs_batch_create with cfg NULL uses it for such a batch and
+ * {2, 32,12,64, 32,16,128, ROWS2 | PLANES2} for other stencil sets. */
 int uhs_batch_default_config(uhs_batch_config *out);
 /* The unbatched baseline: uhs_sweep_f64 on each grid in turn, same stream,
  * one launch per timestep (the one-step kernel and its default tile). */

@@ -85,7 +85,7 @@ typedef struct uhs_mlp *uhs_mlp_t;
  * 1 deeper prefetch (two planes for stars and two rows for 2D, three cells
  * for boxes), 2 four rows of prefetch (2D), 3 register-tiled, 4 two rows per
  * thread, 5 two planes of box loads in flight, 6 both 4 and 5, 7 a register
- * ring of rows (2D)).  Threads per workgroup follow the variant (ghost zone
+ * ring of rows (2D), 8 four rows per thread, 9 four rows and two planes).  Threads per workgroup follow the variant (ghost zone
  * included for UHS_CFG_TEMPORAL2). */
 int uhs_mlp_features(int dims, int radius, int star, int dtype, const int64_t *ext,
                      const uhs_config *cfg, int depth, float *out20);
@@ -140,7 +140,9 @@ int uhs_batch_tune_list(const uhs_grid_desc *grids, int n, int timesteps,
                         double *best_ms, double *ms_out, uhs_stream_t stream);
 /* Feature vector of (batch, configuration) for the cost model, 20 inputs:
  * star depth one-hot {1,2,4} (3), log2 star tile x/y/z (3), log2 box tile
- * x/y/z (3), box two-rows flag (1), box two-planes flag (1), log2 interior
+ * x/y/z (3), box variant (1: two rows per thread + 1, two planes in flight
+ * + 2), split mode (1: 0 shared launches, 1 UHS_BATCH_SPLIT, 2 with
+ * UHS_BATCH_CONCURRENT), log2 interior
  * points of the star grids and of the box grids (2, 0 when none), log2
  * number of grids (1), mean log2 extent per axis (3), spread (standard
  * deviation) of the log2 extents (1), log2 (1 + mean terms per point of the

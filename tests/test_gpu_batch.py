@@ -78,6 +78,7 @@ def test_batch_star_tile_forms(gpu, orc, depth, star_tile):
     cfg = u.batch_default_config()
     cfg.star_depth = depth
     cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = star_tile + (11,)
+    cfg.flags = u.BATCH_BOX_ROWS2 | u.BATCH_BOX_PLANES2  # shared launches
     check_batch(u, orc, grids, 7, cfg)
 
 
@@ -89,13 +90,18 @@ def test_batch_all_3d_kinds_wide_halo(gpu, orc, depth):
     cfg = u.batch_default_config()
     cfg.star_depth = depth
     cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = 32, 4, 16
+    cfg.flags = u.BATCH_BOX_ROWS2 | u.BATCH_BOX_PLANES2  # shared launches
     check_batch(u, orc, grids, 6, cfg)
 
 
 def test_batch_default_config_and_rejections(gpu, orc):
     u = gpu
+    d = u.batch_default_config()
+    assert d.flags & u.BATCH_SPLIT and d.flags & u.BATCH_CONCURRENT
     grids = make_grids(u, orc, ["3d1r_star", "3d2r_box", "3d1r_star"], seed=3)
     check_batch(u, orc, grids, 4, None)
+    # other stencil sets: the shared-launch default
+    check_batch(u, orc, make_grids(u, orc, ["3d1r_star", "3d1r_box", "3d2r_box"], seed=4), 4, None)
     case, st, ext, h, A0, B0 = grids[0]
     dA = u.DeviceArray.from_host(A0)
     dB = u.DeviceArray.from_host(B0)
@@ -104,7 +110,8 @@ def test_batch_default_config_and_rejections(gpu, orc):
     with pytest.raises(u.UhsError) as e:
         u.Batch([(dA, dB, ext, h, st)], 2, bad)
     assert e.value.code == -3
-    for tile in ((32, 8, 9, 1), (32, 15, 9, 1), (32, 16, 9, 0), (32, 8, 9, 4)):
+    # flags 16: no such flag; 8: UHS_BATCH_CONCURRENT without UHS_BATCH_SPLIT
+    for tile in ((32, 8, 9, 1), (32, 15, 9, 1), (32, 16, 9, 0), (32, 8, 9, 16), (32, 8, 9, 8)):
         bad = u.batch_default_config()
         bad.box_tile_x, bad.box_tile_y, bad.box_tile_z, bad.flags = tile
         with pytest.raises(u.UhsError) as e:
@@ -143,3 +150,34 @@ def test_batch_rerun_continues(gpu, orc, T, depth):
         assert np.array_equal(res, ref), (case, ext, h, T, depth)
         dA.close()
         dB.close()
+
+
+@pytest.mark.parametrize("mode", ["split", "concurrent"])
+@pytest.mark.parametrize("depth,star_tile", [(1, (32, 16)), (1, (64, 16)), (2, (62, 14)), (2, (30, 14)),
+                                             (2, (32, 12)), (4, (32, 8))])
+def test_batch_split(gpu, orc, mode, depth, star_tile):
+    """Star grids in launches of their own (512-thread workgroups for the
+    benchmark mix), in one chain with the box launches or as a concurrent
+    second chain: bitwise the oracle's T single sweeps, odd and even T."""
+    u = gpu
+    kinds = ["3d1r_star", "3d2r_box"] * 3
+    for T in (5, 6):
+        grids = make_grids(u, orc, kinds, seed=70 + depth + T)
+        cfg = u.batch_default_config()
+        cfg.star_depth = depth
+        cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = star_tile + (9,)
+        cfg.flags = u.BATCH_BOX_ROWS2 | u.BATCH_BOX_PLANES2 | u.BATCH_SPLIT
+        if mode == "concurrent":
+            cfg.flags |= u.BATCH_CONCURRENT
+        check_batch(u, orc, grids, T, cfg)
+
+
+def test_batch_split_other_stencils(gpu, orc):
+    """Split batches of other stencil sets keep 256-thread star launches."""
+    u = gpu
+    kinds = ["3d1r_star", "3d2r_star", "3d1r_box", "3d2r_box"]
+    grids = make_grids(u, orc, kinds, seed=91, halo_extra=1)
+    cfg = u.batch_default_config()
+    cfg.star_tile_x, cfg.star_tile_y, cfg.star_tile_z = 16, 8, 7
+    cfg.flags = u.BATCH_BOX_ROWS2 | u.BATCH_SPLIT | u.BATCH_CONCURRENT
+    check_batch(u, orc, grids, 5, cfg)

@@ -13,9 +13,10 @@
 * The uneven batch exactly as the benchmark builds it (64 grids, extents
   64-384, half 7-point star and half 125-point box, T = 50): every grid of
   the batched graph bitwise equal to the per-grid one-step sweeps (the
-  default batch configuration and two others covering the other tile
-  forms), the smallest grids bitwise equal to the oracle, and every cell of
-  every grid exact on the linear field.
+  default batch configuration -- star and box grids as concurrent chains --
+  and three others covering shared launches and the other tile forms), the
+  smallest grids bitwise equal to the oracle, and every cell of every grid
+  exact on the linear field.
 * FP32 3d2r_star at 512^3, plain and compensated, bitwise equal to the
   oracle's FP32 modes at T = 10; its error against the FP64 sweep at steps
   10/100/1000 within the bound a normalised positive stencil allows.
@@ -181,7 +182,10 @@ def _batch_configs(u):
     four.star_depth = 4
     four.star_tile_x, four.star_tile_y, four.star_tile_z = 32, 8, 64
     four.box_tile_x, four.box_tile_y, four.box_tile_z, four.flags = 64, 8, 128, 1
-    return [("default", d), ("depth1", one), ("depth4", four)]
+    shared = u.batch_default_config()
+    shared.star_tile_x, shared.star_tile_y, shared.star_tile_z = 32, 12, 64
+    shared.flags = u.BATCH_BOX_ROWS2 | u.BATCH_BOX_PLANES2
+    return [("default", d), ("depth1", one), ("depth4", four), ("shared", shared)]
 
 
 @pytest.mark.timeout(600)

@@ -26,17 +26,17 @@ using MixFn = void (*)(BLaunch);
 template <int K, int SF> MixFn mix_box(int bf)
 {
     switch (bf) {
-    case kFormBoxStd | kFormPlanes1: return k_batch<K, SF | kFormBoxStd | kFormPlanes1, Prog_3d1r_star, Prog_3d2r_box>;
-    case kFormBoxStd | kFormPlanes2: return k_batch<K, SF | kFormBoxStd | kFormPlanes2, Prog_3d1r_star, Prog_3d2r_box>;
-    case kFormRows2 | kFormPlanes1: return k_batch<K, SF | kFormRows2 | kFormPlanes1, Prog_3d1r_star, Prog_3d2r_box>;
-    case kFormRows2 | kFormPlanes2: return k_batch<K, SF | kFormRows2 | kFormPlanes2, Prog_3d1r_star, Prog_3d2r_box>;
+    case kFormBoxStd | kFormPlanes1: return k_batch<K, SF | kFormBoxStd | kFormPlanes1, kBatchThreads, Prog_3d1r_star, Prog_3d2r_box>;
+    case kFormBoxStd | kFormPlanes2: return k_batch<K, SF | kFormBoxStd | kFormPlanes2, kBatchThreads, Prog_3d1r_star, Prog_3d2r_box>;
+    case kFormRows2 | kFormPlanes1: return k_batch<K, SF | kFormRows2 | kFormPlanes1, kBatchThreads, Prog_3d1r_star, Prog_3d2r_box>;
+    case kFormRows2 | kFormPlanes2: return k_batch<K, SF | kFormRows2 | kFormPlanes2, kBatchThreads, Prog_3d1r_star, Prog_3d2r_box>;
     }
     return nullptr;
 }
 template <int K> MixFn all_kernel()
 {
-    return k_batch<K, kFormAll, Prog_3d1r_star, Prog_3d2r_star, Prog_3d3r_star, Prog_3d1r_box, Prog_3d2r_box,
-                   Prog_3d3r_box>;
+    return k_batch<K, kFormAll, kBatchThreads, Prog_3d1r_star, Prog_3d2r_star, Prog_3d3r_star, Prog_3d1r_box,
+                   Prog_3d2r_box, Prog_3d3r_box>;
 }
 
 MixFn pick_kernel(bool mix, int K, int forms)
@@ -54,9 +54,9 @@ MixFn pick_kernel(bool mix, int K, int forms)
         if (K == 2 && sf == kFormStarLds) f = mix_box<2, kFormStarLds>(bf);
         if (K == 4 && sf == kFormStarLds) f = mix_box<4, kFormStarLds>(bf);
         if (f) return f;
-        if (K == 1) return k_batch<1, kFormAll, Prog_3d1r_star, Prog_3d2r_box>;
-        if (K == 2) return k_batch<2, kFormAll, Prog_3d1r_star, Prog_3d2r_box>;
-        if (K == 4) return k_batch<4, kFormAll, Prog_3d1r_star, Prog_3d2r_box>;
+        if (K == 1) return k_batch<1, kFormAll, kBatchThreads, Prog_3d1r_star, Prog_3d2r_box>;
+        if (K == 2) return k_batch<2, kFormAll, kBatchThreads, Prog_3d1r_star, Prog_3d2r_box>;
+        if (K == 4) return k_batch<4, kFormAll, kBatchThreads, Prog_3d1r_star, Prog_3d2r_box>;
         return nullptr;
     }
     switch (K) {
@@ -67,11 +67,29 @@ MixFn pick_kernel(bool mix, int K, int forms)
     return nullptr;
 }
 
+// Star launches of a split batch of the benchmark mix: star tile forms only,
+// kBatchStarThreads threads per workgroup.
+MixFn pick_star_kernel(int K, int forms)
+{
+    const int sf = forms & (kFormRq1 | kFormRq2 | kFormStarLds);
+    using S = Prog_3d1r_star;
+    using B = Prog_3d2r_box;
+    constexpr int N = kBatchStarThreads;
+    if (K == 1 && sf == kFormRq1) return k_batch<1, kFormRq1, N, S, B>;
+    if (K == 1 && sf == kFormStarLds) return k_batch<1, kFormStarLds, N, S, B>;
+    if (K == 2 && sf == kFormRq2) return k_batch<2, kFormRq2, N, S, B>;
+    if (K == 2 && sf == kFormStarLds) return k_batch<2, kFormStarLds, N, S, B>;
+    if (K == 4 && sf == kFormStarLds) return k_batch<4, kFormStarLds, N, S, B>;
+    return nullptr;
+}
+
 struct LaunchRec {
     int depth;
     size_t first, count; // slice of the task array
     size_t lds;
     int workgroups;
+    int threads;         // per workgroup
+    int chain;           // split batches: 0 = the star grids' launches, 1 = the box grids'
     MixFn fn;
 };
 
@@ -169,7 +187,12 @@ int uhs_batch_partition(const uhs_grid_desc *grids, int n, int ngpus, const doub
 int uhs_batch_default_config(uhs_batch_config *c)
 {
     if (!c) return berr(UHS_E_INVAL, "null config");
-    *c = uhs_batch_config{2, 32, 12, 64, 32, 16, 128, UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2};
+    // The benchmark mix's best measured configuration (64 uneven grids,
+    // 7-point stars and 125-point boxes): stars two steps per launch in
+    // 62 x 14 tiles (512-thread workgroups), boxes two rows per thread with
+    // two planes in flight, the two kinds as concurrent chains.
+    *c = uhs_batch_config{2, 62, 14, 32, 32, 16, 128,
+                          UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2 | UHS_BATCH_SPLIT | UHS_BATCH_CONCURRENT};
     return UHS_OK;
 }
 
@@ -179,12 +202,28 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
     if (!grids || n < 1 || !out) return berr(UHS_E_INVAL, "batch needs grids and an output handle");
     if (timesteps < 1) return berr(UHS_E_INVAL, "batch needs timesteps >= 1");
     uhs_batch_config cfg;
-    if (cfg_in) cfg = *cfg_in;
-    else uhs_batch_default_config(&cfg);
+    if (cfg_in) {
+        cfg = *cfg_in;
+    } else {
+        // Other stencil sets than the benchmark mix: shared launches with
+        // 32 x 12 star tiles in 256-thread workgroups.
+        bool mix = true;
+        for (int i = 0; i < n; i++) {
+            const int prog = grids[i].stencil.offsets && grids[i].dims == 3
+                                 ? uhs_match_builtin(3, grids[i].stencil.npoints, grids[i].stencil.offsets)
+                                 : -1;
+            if (prog != Prog_3d1r_star::ID && prog != Prog_3d2r_box::ID) mix = false;
+        }
+        uhs_batch_default_config(&cfg);
+        if (!mix) cfg = uhs_batch_config{2, 32, 12, 64, 32, 16, 128, UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2};
+    }
     const int K = cfg.star_depth;
     if (K != 1 && K != 2 && K != 4) return berr(UHS_E_CONFIG, "star_depth must be 1, 2 or 4");
-    if (cfg.flags & ~(UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2)) return berr(UHS_E_CONFIG, "unknown batch flags");
+    if (cfg.flags & ~(UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2 | UHS_BATCH_SPLIT | UHS_BATCH_CONCURRENT))
+        return berr(UHS_E_CONFIG, "unknown batch flags");
     const bool rows2 = cfg.flags & UHS_BATCH_BOX_ROWS2;
+    const bool split = cfg.flags & UHS_BATCH_SPLIT, concurrent = cfg.flags & UHS_BATCH_CONCURRENT;
+    if (concurrent && !split) return berr(UHS_E_CONFIG, "UHS_BATCH_CONCURRENT needs UHS_BATCH_SPLIT");
     if (cfg.box_tile_x < 1 || cfg.box_tile_y < 1 ||
         cfg.box_tile_x * cfg.box_tile_y != (rows2 ? 2 : 1) * kBatchThreads || (rows2 && cfg.box_tile_y % 2))
         return berr(UHS_E_CONFIG, "box tiles must hold exactly 256 points (tile_x * tile_y), or 512 with an even "
@@ -198,9 +237,29 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
         return berr(UHS_E_HIP, "cannot query the device");
 
+    // The benchmark mix (only 7-point stars and 125-point boxes) has its own
+    // kernels, and a split batch of it runs its star launches with
+    // kBatchStarThreads-thread workgroups.
+    bool mix = true;
+    for (int i = 0; i < n; i++) {
+        const int prog = grids[i].stencil.offsets && grids[i].dims == 3
+                             ? uhs_match_builtin(3, grids[i].stencil.npoints, grids[i].stencil.offsets)
+                             : -1;
+        if (prog != Prog_3d1r_star::ID && prog != Prog_3d2r_box::ID) mix = false;
+    }
+    // Star launches of a split batch take 512-thread workgroups when the star
+    // tile needs them (two steps: its ghost-zone extension at two columns per
+    // thread exceeds 256 threads; single steps: a 1024-point tile), else 256.
+    int NS = kBatchThreads; // threads of star tasks
+    if (split && mix) {
+        const int bx = cfg.star_tile_x + 2, by = cfg.star_tile_y + 2; // radius 1
+        const bool big = K == 2 ? bx / 2 * by > kBatchThreads
+                                : K == 1 && cfg.star_tile_x * cfg.star_tile_y == 2 * kBatchStarThreads;
+        if (big) NS = kBatchStarThreads;
+    }
+
     // Per-grid records and validation.
     std::vector<BGrid> hg(n);
-    bool mix = true;
     for (int i = 0; i < n; i++) {
         const uhs_grid_desc &d = grids[i];
         char msg[160];
@@ -235,24 +294,22 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         if (b.star) {
             g.TX = cfg.star_tile_x; g.TY = cfg.star_tile_y;
             const int R = b.radius, bx = g.TX + 2 * R, by = g.TY + 2 * R;
-            // Single steps: the star tile when it is a register-queue tile;
-            // in a temporally blocked batch, whose only single steps are the
-            // few leading ones that keep the result in the right grid, a
-            // 64 x 8 register-queue tile otherwise (the LDS-ring form is
-            // several times slower per step).
+            // Single steps: the star tile when it is a register-queue tile
+            // (two points per thread); in a temporally blocked batch, whose
+            // only single steps are the few leading ones that keep the
+            // result in the right grid, a 64-column register-queue tile
+            // otherwise (the LDS-ring form is several times slower per step).
             auto rq1 = [&](int tx, int ty) {
-                return tx * ty == 2 * kBatchThreads && tx % 2 == 0 &&
-                       2 * R * (tx + ty) <= kStarHaloPerThread * kBatchThreads;
+                return tx * ty == 2 * NS && tx % 2 == 0 && 2 * R * (tx + ty) <= kStarHaloPerThread * NS;
             };
             if (rq1(g.TX, g.TY)) {
                 g.mode |= kModeRq1;
                 g.TX1 = g.TX; g.TY1 = g.TY;
-            } else if (K > 1 && rq1(64, 8)) {
+            } else if (K > 1 && rq1(64, 2 * NS / 64)) {
                 g.mode |= kModeRq1;
-                g.TX1 = 64; g.TY1 = 8;
+                g.TX1 = 64; g.TY1 = 2 * NS / 64;
             }
-            if (K == 2 && bx % 2 == 0 && bx / 2 * by <= kBatchThreads &&
-                2 * R * (bx + by) <= kBatchRingCells * kBatchThreads)
+            if (K == 2 && bx % 2 == 0 && bx / 2 * by <= NS && 2 * R * (bx + by) <= kBatchRingCells * NS)
                 g.mode |= kModeRq2;
             for (int t = 0; t < b.npoints; t++) g.coef[t] = d.stencil.coeffs[t];
         } else {
@@ -272,21 +329,26 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
             if ((long long)(g.TX + 2 * b.radius) * (g.TY + 2 * b.radius) > kBatchBoxCells * kBatchThreads)
                 return berr(UHS_E_CONFIG, "box tile plus halo exceeds the per-thread staging");
         }
-        if (prog != Prog_3d1r_star::ID && prog != Prog_3d2r_box::ID) mix = false;
     }
 
     // Schedules and tasks.
     const std::vector<int> ssched = star_schedule(timesteps, K);
     const int nls = (int)ssched.size();
-    const int L = std::max<int>(nls, timesteps);
-    // Star launch j runs in global launch slot[j]: spread evenly over the
+    // Star launch j runs in box launch slot[j]'s time: spread evenly over the
     // box grids' one-step launches, so every part of the run has HBM-bound
     // star tiles beside the VALU-bound box tiles (each grid's own launches
-    // stay in order).
+    // stay in order).  Unsplit, both share that launch; split, the star
+    // launches are launches of their own (indices 0 .. nls-1, box launch l
+    // is nls + l), captured right after box launch slot[j] in one chain, or
+    // as a second chain beside the box chain with UHS_BATCH_CONCURRENT.
+    const int Lm = std::max<int>(nls, timesteps);
+    const int L = split ? nls + timesteps : Lm;
     std::vector<int> slot(nls);
-    for (int j = 0; j < nls; j++) slot[j] = (int)((long long)j * L / nls);
-    std::vector<int> ldepth(L, 1);
-    for (int j = 0; j < nls; j++) ldepth[slot[j]] = ssched[j];
+    for (int j = 0; j < nls; j++) slot[j] = (int)((long long)j * Lm / nls);
+    std::vector<int> ldepth(L, 1), lchain(L, 0);
+    for (int j = 0; j < nls; j++) ldepth[split ? j : slot[j]] = ssched[j];
+    if (split)
+        for (int l = 0; l < timesteps; l++) lchain[nls + l] = 1;
     std::vector<std::vector<BTask>> per(L);
     std::vector<std::vector<double>> cost(L);
     for (int i = 0; i < n; i++) {
@@ -313,7 +375,7 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
                         t.x0 = x; t.y0 = y; t.zs = z; t.ze = std::min(z + tz, g.n0);
                         t.depth = depth;
                         t.k0 = k0;
-                        const int gl = b.star ? slot[l] : l;
+                        const int gl = b.star ? (split ? l : slot[l]) : (split ? nls + l : l);
                         per[gl].push_back(t);
                         const double pts = (double)std::min(TX, g.n2 - x) * std::min(TY, g.n1 - y) *
                                            (t.ze - t.zs);
@@ -385,7 +447,14 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
             return berr(UHS_E_CONFIG, "tile and depth need more LDS than a workgroup may use");
         }
         r.lds = lds;
-        r.fn = pick_kernel(mix, r.depth, forms);
+        r.chain = lchain[l];
+        const bool star_launch = split && lchain[l] == 0;
+        r.threads = star_launch ? NS : kBatchThreads;
+        r.fn = star_launch && NS != kBatchThreads ? pick_star_kernel(r.depth, forms) : pick_kernel(mix, r.depth, forms);
+        if (!r.fn && r.count) {
+            delete B;
+            return berr(UHS_E_CONFIG, "no batch kernel for this combination of tile forms");
+        }
         B->launches.push_back(r);
     }
     B->ntasks = (long long)all.size();
@@ -428,36 +497,73 @@ int uhs_batch_create(const uhs_grid_desc *grids, int n, int timesteps, const uhs
         return fail_hip(e, "task upload");
 
     for (auto &r : B->launches) {
+        if (!r.count) continue;
         if (r.lds > 64 * 1024 &&
             (e = hipFuncSetAttribute((const void *)r.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)r.lds)) != hipSuccess)
             return fail_hip(e, "LDS limit");
         int per_cu = 0;
-        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)r.fn, kBatchThreads,
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)r.fn, r.threads,
                                                               r.lds)) != hipSuccess)
             return fail_hip(e, "occupancy");
         const long long wg = (long long)std::max(1, per_cu) * cus;
         r.workgroups = (int)std::max<long long>(1, std::min<long long>(wg, (long long)r.count));
     }
 
+    // Capture order: unsplit, the launches in index order; split, star
+    // launch j right after box launch slot[j] in one chain, or the star
+    // launches as a second chain forked off after the counter reset and
+    // joined before the end (UHS_BATCH_CONCURRENT).
+    std::vector<int> order;
+    if (!split) {
+        for (int l = 0; l < L; l++) order.push_back(l);
+    } else {
+        int j = 0;
+        for (int l = 0; l < timesteps; l++) {
+            order.push_back(nls + l);
+            for (; j < nls && slot[j] <= l; j++) order.push_back(j);
+        }
+        for (; j < nls; j++) order.push_back(j);
+    }
     // Capture: zero the counters with a kernel (not a memset node), then the
     // launches; one graph per buffer parity that a run can start from.
     for (int ph = 0; ph < (odd ? 2 : 1); ph++) {
-        hipStream_t cs;
+        hipStream_t cs = nullptr, cs2 = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
         if ((e = hipStreamCreate(&cs)) != hipSuccess) return fail_hip(e, "capture stream");
+        if (concurrent && ((e = hipStreamCreate(&cs2)) != hipSuccess ||
+                           (e = hipEventCreateWithFlags(&fork, hipEventDisableTiming)) != hipSuccess ||
+                           (e = hipEventCreateWithFlags(&join, hipEventDisableTiming)) != hipSuccess)) {
+            (void)hipStreamDestroy(cs);
+            if (cs2) (void)hipStreamDestroy(cs2);
+            if (fork) (void)hipEventDestroy(fork);
+            return fail_hip(e, "capture stream");
+        }
         e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_zero_u32, dim3((L + 255) / 256), dim3(256), 0, cs, B->d_counters, L);
-            for (int l = 0; l < L; l++) {
+            if (concurrent) {
+                if (e == hipSuccess) e = hipEventRecord(fork, cs);
+                if (e == hipSuccess) e = hipStreamWaitEvent(cs2, fork, 0);
+            }
+            for (int l : order) {
                 const LaunchRec &r = B->launches[l];
                 if (!r.count) continue;
+                hipStream_t ls = concurrent && r.chain == 0 ? cs2 : cs;
                 BLaunch bl{B->d_tasks + ph * nt + r.first, B->d_grids, B->d_counters + l, (int)r.count};
-                hipLaunchKernelGGL(r.fn, dim3(r.workgroups), dim3(kBatchThreads), r.lds, cs, bl);
+                hipLaunchKernelGGL(r.fn, dim3(r.workgroups), dim3(r.threads), r.lds, ls, bl);
+            }
+            if (concurrent) {
+                if (e == hipSuccess) e = hipEventRecord(join, cs2);
+                if (e == hipSuccess) e = hipStreamWaitEvent(cs, join, 0);
             }
             hipError_t e2 = hipStreamEndCapture(cs, &B->graph[ph]);
             if (e == hipSuccess) e = e2;
         }
         (void)hipStreamDestroy(cs);
+        if (cs2) (void)hipStreamDestroy(cs2);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
         if (e != hipSuccess) return fail_hip(e, "graph capture");
         if ((e = hipGraphInstantiate(&B->exec[ph], B->graph[ph], nullptr, nullptr, 0)) != hipSuccess)
             return fail_hip(e, "graph instantiate");

@@ -17,6 +17,9 @@
 #include "sweep_kernels.h"
 
 constexpr int kBatchThreads = 256;
+// Workgroup size of the star launches of a split batch (UHS_BATCH_SPLIT):
+// the per-grid two-step kernel's tile shapes need twice the threads.
+constexpr int kBatchStarThreads = 512;
 // Staging registers per thread of the batch's tiles (box: plane-tile cells,
 // two-step star: ring cells); the host rejects tiles that need more.
 constexpr int kBatchBoxCells = 4, kBatchRingCells = 2;
@@ -246,9 +249,9 @@ __host__ __device__ inline int task_forms(const BGrid &g, const BTask &t, bool s
 // The persistent kernel: each workgroup takes the next task index from the
 // launch's counter until the list is exhausted (every workgroup reaches the
 // exit).  Progs lists the stencils this instantiation can run; a grid's kind
-// selects one.
-template <int K, int F, class... Progs>
-__global__ void __launch_bounds__(kBatchThreads) k_batch(BLaunch L)
+// selects one.  NTB: threads per workgroup (the launch uses exactly that).
+template <int K, int F, int NTB, class... Progs>
+__global__ void __launch_bounds__(NTB) k_batch(BLaunch L)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double *lds = reinterpret_cast<double *>(smem_raw);

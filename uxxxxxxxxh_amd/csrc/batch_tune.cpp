@@ -58,6 +58,21 @@ std::vector<uhs_batch_config> batch_space()
     add(1, s1, 3);
     add(2, s2, 5);
     add(4, s4, 1);
+    // Split batches, star and box launches as two concurrent chains: two-step
+    // star tiles of 256 threads and of 512 (whose ghost-zone extension needs
+    // 512-thread workgroups).  The split without concurrency (one chain) is
+    // left out: it measured slower than shared launches for every tile.
+    static const int s2w[][2] = {{32, 12}, {30, 14}, {62, 6}, {62, 14}, {30, 30}, {126, 6}};
+    static const int stzw[] = {32, 64};
+    static const int btw[][3] = {{32, 16, UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2},
+                                 {64, 8, UHS_BATCH_BOX_ROWS2 | UHS_BATCH_BOX_PLANES2},
+                                 {32, 8, UHS_BATCH_BOX_PLANES2}};
+    for (auto &t : s2w)
+        for (int sz : stzw)
+            for (auto &b : btw)
+                for (int bz : btz)
+                    cand.push_back(uhs_batch_config{2, t[0], t[1], sz, b[0], b[1], bz,
+                                                    b[2] | UHS_BATCH_SPLIT | UHS_BATCH_CONCURRENT});
     return cand;
 }
 
@@ -154,8 +169,8 @@ int uhs_batch_features(const uhs_grid_desc *grids, int n, const uhs_batch_config
     f[6] = log2f((float)std::max(1, c->box_tile_x));
     f[7] = log2f((float)std::max(1, c->box_tile_y));
     f[8] = log2f((float)std::max(1, c->box_tile_z));
-    f[9] = (c->flags & UHS_BATCH_BOX_ROWS2) ? 1.f : 0.f;
-    f[10] = (c->flags & UHS_BATCH_BOX_PLANES2) ? 1.f : 0.f;
+    f[9] = (float)((c->flags & UHS_BATCH_BOX_ROWS2 ? 1 : 0) + (c->flags & UHS_BATCH_BOX_PLANES2 ? 2 : 0));
+    f[10] = (float)((c->flags & UHS_BATCH_SPLIT ? 1 : 0) + (c->flags & UHS_BATCH_CONCURRENT ? 1 : 0));
     double star_pts = 0, box_pts = 0, star_work = 0, box_work = 0;
     double le[3] = {0, 0, 0}, le2 = 0;
     int radius_star = 1;
@@ -196,7 +211,8 @@ int uhs_batch_features(const uhs_grid_desc *grids, int n, const uhs_batch_config
     f[18] = (float)std::log2(1.0 + (star_pts > 0 ? star_work / star_pts : 0.0) +
                              (box_pts > 0 ? box_work / box_pts : 0.0));
     const int bx = c->star_tile_x + 2 * radius_star, by = c->star_tile_y + 2 * radius_star;
-    f[19] = (c->star_depth == 2 && bx % 2 == 0 && bx / 2 * by <= 256) ? 1.f : 0.f;
+    const int nts = (c->flags & UHS_BATCH_SPLIT) ? 512 : 256;
+    f[19] = (c->star_depth == 2 && bx % 2 == 0 && bx / 2 * by <= nts) ? 1.f : 0.f;
     return UHS_OK;
 }
 

@@ -85,7 +85,7 @@ typedef struct uhs_mlp *uhs_mlp_t;
  * 1 deeper prefetch (two planes for stars and two rows for 2D, three cells
  * for boxes), 2 four rows of prefetch (2D), 3 register-tiled, 4 two rows per
  * thread, 5 two planes of box loads in flight, 6 both 4 and 5, 7 a register
- * ring of rows (2D), 8 four rows per thread, 9 four rows and two planes).  Threads per workgroup follow the variant (ghost zone
+ * ring of rows (2D)).  Threads per workgroup follow the variant (ghost zone
  * included for UHS_CFG_TEMPORAL2). */
 int uhs_mlp_features(int dims, int radius, int star, int dtype, const int64_t *ext,
                      const uhs_config *cfg, int depth, float *out20);

@@ -373,7 +373,7 @@ __device__ __forceinline__ bool star3d_tb2_full(int x0, int y0, int zs, int ze, 
 }
 
 template <typename T, class S, int VX, bool KAHAN, int PF, int NH = kStarHaloPerThread, bool FULL = false,
-          class CF>
+          int AHEAD = 2, class CF>
 __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__restrict__ out,
                                                 const T *__restrict__ h0, const T *__restrict__ h1,
                                                 const TileGeom &g, int x0, int y0, int zs, int ze,
@@ -453,33 +453,39 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
     const int N = (ze - zs) + 2 * R;    // level-1 planes
     // Register rings instead of shifting queues: level-0 plane p of the own
     // columns lives in slot (p - z1s + R) mod M0, the ring cells of plane p
-    // in slot (p - z1s) mod 2.  Every load is issued two iterations before
-    // the iteration that first reads it, at the end of an iteration, into
-    // the slot whose value that iteration read last: iteration i reads
-    // level-0 planes z1-R .. z1+R, plane z1+R+1 is in flight, and plane
-    // z1+R+2 is loaded into the slot of plane z1-R.  So the waits of an
-    // iteration count the previous iteration's loads as still in flight
-    // (the memory counter is in order) and wait only for older ones.  The
-    // loop below is unrolled by M0 by hand (the compiler does not unroll a
+    // in slot (p - z1s) mod 2.  AHEAD = 2: every load is issued two
+    // iterations before the iteration that first reads it, at the end of an
+    // iteration, into the slot whose value that iteration read last:
+    // iteration i reads level-0 planes z1-R .. z1+R, plane z1+R+1 is in
+    // flight, and plane z1+R+2 is loaded into the slot of plane z1-R.  So the
+    // waits of an iteration count the previous iteration's loads as still in
+    // flight (the memory counter is in order) and wait only for older ones.
+    // AHEAD = 1: each iteration loads, before its barrier, plane z1+R+1 into
+    // the slot of plane z1-R-1 and the ring (and level-1 halo) of plane z1+1
+    // -- fewer loads in flight, which the batched kernel, sharing the CUs
+    // with the box tiles, runs faster with (measured on the benchmark batch).
+    // The loop below is unrolled by M0 by hand (the compiler does not unroll a
     // loop with a barrier), so every slot index is a constant: a loaded
     // value stays in its register until its last use, and no copy makes the
     // loop wait for a load still in flight.  Level-1 plane p of the own
     // columns lives in slot (p - z1s) mod M0 (it may be a halo value loaded
     // from memory, so it must not be copied either).
     static_assert(PF == 1, "one plane in flight between iterations");
+    static_assert(AHEAD == 1 || AHEAD == 2, "loads one or two iterations ahead");
     constexpr int M0 = Q + 1;
     // rh1[v][i & 1]: the level-1 halo value of plane z1 for a column that
-    // takes it (outside the interior, or z1 outside it), loaded two
+    // takes it (outside the interior, or z1 outside it), loaded AHEAD
     // iterations ahead into a register of its own.
     T r0[VX][M0], rh[2][NH], r1[VX][M0], rh1[VX][2];
     auto takes_halo1 = [&](int v, int z) { return !FULL && !(xyin[v] && z >= 0 && z < n0); };
     auto halo1 = [&](int v, int z) { return h1[(long long)clampi(z, -R, n0 + R - 1) * sz + coff[v]]; };
-    // The older planes first, then the last two planes in the order an
-    // iteration issues its loads (level-1 halo, own columns, ring), so the
+    // Planes z1s-R .. z1s+R-1 first, then for g < AHEAD plane z1s+R+g (slot
+    // 2R+g) with the level-1 halo and the ring of plane z1s+g, in the order
+    // an iteration issues its loads (level-1 halo, own columns, ring), so the
     // waits at the loop's entry, which must hold for the first turn too,
     // still count the last group as in flight.
 #pragma unroll
-    for (int j = 0; j < M0 - 2; j++)
+    for (int j = 0; j < 2 * R; j++)
 #pragma unroll
         for (int v = 0; v < VX; v++) r0[v][j] = lev0(xyin[v], z1s - R + j, coff[v]);
 #pragma unroll
@@ -489,12 +495,12 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
         rh1[v][0] = rh1[v][1] = T(0);
     }
 #pragma unroll
-    for (int g = 0; g < 2; g++) {
+    for (int g = 0; g < AHEAD; g++) {
 #pragma unroll
         for (int v = 0; v < VX; v++)
             if (col && takes_halo1(v, z1s + g)) rh1[v][g] = halo1(v, z1s + g);
 #pragma unroll
-        for (int v = 0; v < VX; v++) r0[v][M0 - 2 + g] = lev0(xyin[v], z1s + R + g, coff[v]);
+        for (int v = 0; v < VX; v++) r0[v][2 * R + g] = lev0(xyin[v], z1s + R + g, coff[v]);
 #pragma unroll
         for (int k = 0; k < NH; k++) rh[g][k] = lev0(hin[k], z1s + g, hoff[k]);
     }
@@ -516,6 +522,15 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
 #pragma unroll
         for (int k = 0; k < NH; k++)
             if (hl[k] >= 0) S0[hl[k]] = rh[PH & 1][k];
+        if constexpr (AHEAD == 1) {
+            // Plane z1+R+1 into the slot of plane z1-R-1 (no longer used),
+            // the ring of plane z1+1 into the other ring slot.  Addresses are
+            // clamped, so the loads past the slab end are harmless.
+#pragma unroll
+            for (int v = 0; v < VX; v++) r0[v][(PH + M0 - 1) % M0] = lev0(xyin[v], z1 + R + 1, coff[v]);
+#pragma unroll
+            for (int k = 0; k < NH; k++) rh[(PH + 1) & 1][k] = lev0(hin[k], z1 + 1, hoff[k]);
+        }
         // Level-1 plane z1-1+d (d <= 0) is in slot (PH - 1 + d) mod M0.
         constexpr int L1S = PH - 1 - R + 2 * M0; // slot of plane z2 = z1-1-R, plus 2*M0
         if (col) {
@@ -547,8 +562,14 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
         }
         if (i < N && col) {
             // Level 1 at plane z1 for every extended column (the columns
-            // outside the interior take the level-1 halo), then the halo
-            // values they take at plane z1+2.
+            // outside the interior take the level-1 halo); the halo values
+            // they take at plane z1+AHEAD are loaded before (AHEAD = 1) or
+            // after (AHEAD = 2) it.
+            if constexpr (AHEAD == 1) {
+#pragma unroll
+                for (int v = 0; v < VX; v++)
+                    if (takes_halo1(v, z1 + 1)) rh1[v][(PH + 1) & 1] = halo1(v, z1 + 1);
+            }
             const bool zin = FULL || (z1 >= 0 && z1 < n0);
 #pragma unroll
             for (int v = 0; v < VX; v++) {
@@ -570,18 +591,22 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
                 }
                 r1[v][PH] = v1; // the slot of plane z1 - M0, last used above
             }
+            if constexpr (AHEAD == 2) {
 #pragma unroll
-            for (int v = 0; v < VX; v++)
-                if (takes_halo1(v, z1 + 2)) rh1[v][PH & 1] = halo1(v, z1 + 2);
+                for (int v = 0; v < VX; v++)
+                    if (takes_halo1(v, z1 + 2)) rh1[v][PH & 1] = halo1(v, z1 + 2);
+            }
         }
-        // Plane z1+R+2 into the slot of plane z1-R (last read by level 1
-        // above), the ring of plane z1+2 into the slot of plane z1 (staged
-        // above).  Addresses are clamped, so the loads past the slab end are
-        // harmless.
+        if constexpr (AHEAD == 2) {
+            // Plane z1+R+2 into the slot of plane z1-R (last read by level 1
+            // above), the ring of plane z1+2 into the slot of plane z1 (staged
+            // above).  Addresses are clamped, so the loads past the slab end
+            // are harmless.
 #pragma unroll
-        for (int v = 0; v < VX; v++) r0[v][PH] = lev0(xyin[v], z1 + R + 2, coff[v]);
+            for (int v = 0; v < VX; v++) r0[v][PH] = lev0(xyin[v], z1 + R + 2, coff[v]);
 #pragma unroll
-        for (int k = 0; k < NH; k++) rh[PH & 1][k] = lev0(hin[k], z1 + 2, hoff[k]);
+            for (int k = 0; k < NH; k++) rh[PH & 1][k] = lev0(hin[k], z1 + 2, hoff[k]);
+        }
     };
     // Whole ring turns without a branch around any step, so the waits at the
     // loop's entry count the loads still in flight instead of draining them;

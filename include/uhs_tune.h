@@ -50,6 +50,17 @@ int uhs_tune_space(int dims, const int64_t *ext, int halo, const uhs_stencil *st
 int uhs_tune_time(void *A, void *B, int dims, const int64_t *ext, int halo,
                   const uhs_stencil *st, int dtype, const uhs_config *cfg,
                   int reps, double *ms, uhs_stream_t stream);
+/* Mock timing backend.  With fn set, the searches (uhs_tune_sweep,
+ * uhs_tune_sweep_list, uhs_tune_sweep_model) call fn for every candidate
+ * that passes the configuration check, instead of timing sweeps on the GPU:
+ * fn returns a UHS_* status (UHS_E_HIP acts as a launch failure: the search
+ * records it and stops) and writes the milliseconds per sweep.  The search
+ * logic -- database reuse, skipping failed candidates, stopping on a launch
+ * failure, installing the winner -- then runs without a GPU and the grids are
+ * never touched, so tests can inject timings and failures.  NULL restores
+ * the GPU timer.  Process-wide; set it while no search runs. */
+typedef int (*uhs_tune_timer_fn)(const uhs_config *cfg, int reps, double *ms, void *user);
+int uhs_tune_set_timer(uhs_tune_timer_fn fn, void *user);
 /* Installs, for every built-in problem in the database measured on this
  * device, its fastest configuration as the default.  Returns the count. */
 int uhs_tune_load_db(const char *db_path);

@@ -161,6 +161,7 @@ _SIGS = {
                            ctypes.POINTER(ctypes.c_double), _P]),
     "uhs_tune_load_db": (_I, [ctypes.c_char_p]),
     "uhs_tune_clear": (_I, []),
+    "uhs_tune_set_timer": (_I, [_P, _P]),
     "uhs_mlp_features": (_I, [_I, _I, _I, _I, _I64P, ctypes.POINTER(Config), _I, ctypes.POINTER(ctypes.c_float)]),
     "uhs_mlp_create": (_I, [ctypes.c_uint64, ctypes.POINTER(_P)]),
     "uhs_mlp_destroy": (_I, [_P]),
@@ -668,6 +669,31 @@ def tune_sweep(A, B, ext, halo, stencil, dtype=DT_F64, reps=3, db=None, model=No
         check(lib().uhs_tune_sweep_model(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
                                          dtype, model.h, topk, reps, dbp, ctypes.byref(r), stream))
     return r
+
+
+# uhs_tune_timer_fn: int (*)(const uhs_config *, int reps, double *ms, void *user)
+TimerFn = ctypes.CFUNCTYPE(_I, ctypes.POINTER(Config), _I, ctypes.POINTER(ctypes.c_double), _P)
+_timer_ref = None
+
+
+def tune_set_timer(fn):
+    """Mock timing backend for the searches: fn(cfg, reps) -> (status, ms)
+    replaces the timed sweeps (None restores the GPU timer).  The search
+    then runs without a GPU and never touches its grids."""
+    global _timer_ref
+    if fn is None:
+        check(lib().uhs_tune_set_timer(None, None))
+        _timer_ref = None
+        return
+
+    def cb(cfg, reps, ms, user):
+        status, t = fn(Config(cfg[0].tile_x, cfg[0].tile_y, cfg[0].tile_z, cfg[0].vec, cfg[0].flags), reps)
+        ms[0] = t
+        return status
+
+    ref = TimerFn(cb)
+    check(lib().uhs_tune_set_timer(ctypes.cast(ref, _P), None))
+    _timer_ref = ref
 
 
 def tune_list(A, B, ext, halo, stencil, cands, dtype=DT_F64, reps=2, db=None, stream=None):

@@ -114,6 +114,11 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// The search's timer: the GPU (uhs_time_config), or a caller's function set
+// by uhs_tune_set_timer (a mock backend for tests of the search logic).
+uhs_tune_timer_fn g_timer = nullptr;
+void *g_timer_user = nullptr;
+
 } // namespace
 
 // The candidate space of a plan, filtered by uhs_check_config.
@@ -240,7 +245,14 @@ int uhs_search(Plan &pl, void *A, void *B, int dims, const int64_t *ext, int hal
             status = it->second.second;
             r.reused++;
         } else {
-            status = uhs_time_config(pl, A, B, c, reps, st, &ms);
+            if (g_timer) {
+                Plan p = pl;
+                char why[160];
+                status = uhs_check_config(p, c, why, sizeof why) != UHS_OK ? UHS_E_CONFIG
+                                                                          : g_timer(&c, reps, &ms, g_timer_user);
+            } else {
+                status = uhs_time_config(pl, A, B, c, reps, st, &ms);
+            }
             if (status == UHS_E_HIP) {
                 // A launch failure leaves the stream unusable: stop here.
                 (void)hipGetLastError();
@@ -267,6 +279,13 @@ int uhs_search(Plan &pl, void *A, void *B, int dims, const int64_t *ext, int hal
 }
 
 extern "C" {
+
+int uhs_tune_set_timer(uhs_tune_timer_fn fn, void *user)
+{
+    g_timer = fn;
+    g_timer_user = fn ? user : nullptr;
+    return UHS_OK;
+}
 
 int uhs_tune_sweep(void *A, void *B, int dims, const int64_t *ext, int halo, const uhs_stencil *st,
                    int dtype, int reps, const char *db_path, uhs_tune_result *out, uhs_stream_t stream)

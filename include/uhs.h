@@ -367,6 +367,14 @@ int uhs_copy_f64(double *dst, const double *src, size_t n, uhs_stream_t stream);
  * call, no memory traffic inside the loop: the attainable FP64 vector rate,
  * which the benchmark reports next to the FP64-ALU-bound stencils' roof. */
 int uhs_fma_f64(double *out, size_t n, double x, double y, int iters, uhs_stream_t stream);
+/* Debug mode, on the current device: with on != 0, every kernel that stages
+ * through LDS fills the LDS span of each tile with a NaN pattern (and waits
+ * for its workgroup) before staging anything, so a read of a cell a tile
+ * never wrote gives NaN instead of a stale value.  Results are otherwise
+ * bitwise unchanged.  Read by the kernels when they run (captured graphs
+ * included); off by default.  Returns the number of kernel source files
+ * whose flag was set. */
+int uhs_set_lds_poison(int on);
 const char *uhs_last_error(void);
 const char *uhs_version(void);
 int uhs_device_count(void);

@@ -162,6 +162,7 @@ _SIGS = {
     "uhs_tune_load_db": (_I, [ctypes.c_char_p]),
     "uhs_tune_clear": (_I, []),
     "uhs_tune_set_timer": (_I, [_P, _P]),
+    "uhs_set_lds_poison": (_I, [_I]),
     "uhs_mlp_features": (_I, [_I, _I, _I, _I, _I64P, ctypes.POINTER(Config), _I, ctypes.POINTER(ctypes.c_float)]),
     "uhs_mlp_create": (_I, [ctypes.c_uint64, ctypes.POINTER(_P)]),
     "uhs_mlp_destroy": (_I, [_P]),
@@ -500,6 +501,11 @@ def error_norms(a, b, ext, halo, stream=None):
 
 def synchronize():
     check(lib().uhs_synchronize())
+
+
+def set_lds_poison(on):
+    """Debug mode: kernels fill each tile's LDS span with NaN before staging."""
+    return check(lib().uhs_set_lds_poison(1 if on else 0))
 
 
 class Timer:

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -498,6 +499,20 @@ static int sweep_or_program(Plan &pl, void *A, void *B, int dims, const int64_t 
     return uhs_run_plan(pl, A, B, timesteps, stream);
 }
 
+// Setters of the LDS-poison flag, one per translation unit that has kernels
+// (each registers its own when the library loads).
+static std::vector<hipError_t (*)(int)> &lds_poison_setters()
+{
+    static std::vector<hipError_t (*)(int)> v;
+    return v;
+}
+
+int uhs_register_lds_poison_setter(hipError_t (*set)(int on))
+{
+    lds_poison_setters().push_back(set);
+    return 0;
+}
+
 extern "C" {
 
 int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
@@ -715,6 +730,15 @@ int uhs_fma_f64(double *out, size_t n, double x, double y, int iters, uhs_stream
 
 const char *uhs_last_error(void) { return g_err.c_str(); }
 const char *uhs_version(void) { return "uhs-mi355x 0.1 (gfx950)"; }
+
+int uhs_set_lds_poison(int on)
+{
+    for (auto set : lds_poison_setters()) {
+        const hipError_t e = set(on ? 1 : 0);
+        if (e != hipSuccess) return fail(UHS_E_HIP, hipGetErrorString(e));
+    }
+    return (int)lds_poison_setters().size();
+}
 
 int uhs_device_count(void)
 {

@@ -89,6 +89,7 @@ __device__ __forceinline__ void star_tb_tile(const BGrid &g, const BTask &t, dou
     const int NT = blockDim.x, tid = threadIdx.x;
     const int TX = g.TX, TY = g.TY;
     const int WX = TX + 2 * M, WY = TY + 2 * M, WC = WX * WY;
+    lds_poison(lds, K * Q * WC);
     const int ox = t.x0 - M, oy = t.y0 - M; // interior coordinates of LDS cell (0, 0)
     const int h = g.halo, n0 = g.n0, n1 = g.n1, n2 = g.n2, sy = g.sy;
     const long long sz = g.sz;

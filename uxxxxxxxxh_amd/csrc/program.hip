@@ -138,6 +138,7 @@ struct StreamParams {
     int nlo_a, span_a;             // input planes one step reads: [nlo_a, nlo_a + span_a]
     int NI;                        // input ring length
     int lead;                      // planes before the slab's first written plane
+    int lds_cells;                 // LDS cells the rings span
     int nprod, nout;
     int pring[kMaxProd];           // producer k's ring: first LDS cell, length
     int pN[kMaxProd];
@@ -215,6 +216,7 @@ __global__ void __launch_bounds__(kTileThreads) k_program_stream(StreamParams p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
+    lds_poison(lds, p.lds_cells);
     constexpr int waves = kTileThreads / 64;
     const int lane = threadIdx.x & 63, wave = uniform(threadIdx.x >> 6);
     const int t = xcd_remap(blockIdx.x, p.ntiles);
@@ -701,6 +703,7 @@ size_t build_stream(const uhs_program *pg, const Checked &c, const int tile[3], 
         }
     }
     p.pstart[p.nprod] = nterms;
+    p.lds_cells = (int)cells;
     // A barrier before a producer that reads one computed since the last one.
     p.pbar = 0;
     for (int k = 0, first = 0; k < p.nprod; k++) {

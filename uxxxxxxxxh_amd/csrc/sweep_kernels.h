@@ -72,6 +72,7 @@ __device__ __forceinline__ void star3d_tile(const T *__restrict__ in, T *__restr
     const int tx = tid % BXT, ty = tid / BXT;
     const long long sz = p.sz;
     const int sy = p.sy;
+    lds_poison(lds, 2 * LSZ);
 
     // This thread's points (x0 + tx + v*BXT, y0 + ty).
     int coff[VX], lc[VX];
@@ -200,6 +201,7 @@ __device__ __forceinline__ void star3d_tile_shift(const T *__restrict__ in, T *_
     const int tx = tid % BXT, ty = tid / BXT;
     const long long sz = p.sz;
     const int sy = p.sy;
+    lds_poison(lds, 2 * LSZ);
 
     // This thread's points (x0 + tx + v*BXT, y0 + ty).
     int coff[VX], lc[VX];
@@ -385,6 +387,7 @@ __device__ __forceinline__ void star3d_tb2_tile(const T *__restrict__ in, T *__r
     const int LW = BX + 2 * R, L0SZ = LW * (BY + 2 * R), L1SZ = BX * BY;
     T *L0 = lds;            // 2 x L0SZ: level-0 plane + ring
     T *L1 = lds + 2 * L0SZ; // 2 x L1SZ: level-1 plane over the extended tile
+    lds_poison(lds, 2 * (L0SZ + L1SZ));
     const int ox = x0 - R, oy = y0 - R; // interior coordinates of extended cell (0, 0)
     const int tx = tid % BXT, ty = tid / BXT;
     // Threads beyond the extended tile (a workgroup larger than it, as in
@@ -686,6 +689,7 @@ __device__ __forceinline__ void box3d_tile(const T *__restrict__ in, T *__restri
     const int tx = tid % BXT, ty = tid / BXT;
     const long long sz = p.sz;
     const int sy = p.sy;
+    lds_poison(lds, 2 * LSZ);
 
     int coff[VX], lc[VX];
     bool act[VX];
@@ -851,6 +855,7 @@ __device__ __forceinline__ void box3d_tile_rows2(const T *__restrict__ in, T *__
     const int tx = tid % TX, ty = 2 * (tid / TX);
     const long long sz = p.sz;
     const int sy = p.sy;
+    lds_poison(lds, 2 * LSZ);
 
     const int xg = x0 + tx, yg = y0 + ty;
     bool act[2];
@@ -1042,6 +1047,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d(KParams<T, S::P> p)
 
     const int tid = threadIdx.x;
     const int BXT = p.BXT, TX = p.TX, LW = TX + 2 * R;
+    lds_poison(lds, 2 * LW);
     const int bid = xcd_remap(blockIdx.x, p.nblocks);
     const int tile_x = bid % p.tiles_x, chunk = bid / p.tiles_x;
     const int x0 = tile_x * TX;

@@ -80,6 +80,36 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi)
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// LDS poisoning, a debug mode (uhs_set_lds_poison).  With the flag set, a
+// tile first fills the LDS span it uses with a NaN pattern and waits for its
+// whole workgroup, before it stages anything: a read of a cell the tile never
+// wrote then shows up as NaN in a result instead of as a stale value.  The
+// flag is read at run time (one uniform load per tile, graphs included).
+// Every translation unit has its own copy (no relocatable device code) and
+// registers a setter for it with the host library when the library loads.
+static __device__ int g_lds_poison;
+template <typename T> __device__ __forceinline__ T lds_poison_value()
+{
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, 0x7FF4DEAD7FF4DEADull);
+    else return __builtin_bit_cast(T, 0x7FA0DEADu);
+}
+template <typename T> __device__ __forceinline__ void lds_poison(T *lds, int cells)
+{
+    if (__builtin_expect(*(volatile int *)&g_lds_poison != 0, 0)) {
+        const T v = lds_poison_value<T>();
+        for (int i = threadIdx.x; i < cells; i += blockDim.x) lds[i] = v;
+        __syncthreads();
+    }
+}
+int uhs_register_lds_poison_setter(hipError_t (*set)(int on));
+namespace {
+hipError_t uhs_set_lds_poison_here(int on)
+{
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_lds_poison), &on, sizeof on);
+}
+[[maybe_unused]] const int g_lds_poison_registered = uhs_register_lds_poison_setter(uhs_set_lds_poison_here);
+} // namespace
+
 // Workgroups are dealt round-robin to the XCDs, each with its own L2.  Give
 // every XCD one contiguous range of logical tiles, so tiles that share halo
 // rows also share an L2.

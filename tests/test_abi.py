@@ -68,12 +68,25 @@ def test_plan_kinds_and_default_configs(uhs, case):
     uhs.default_config(small, s["radius"], st)
 
 
+# The 2D two-step defaults at 8192^2 (tile_x, tile_y, tile_z, vec, flags).
+_T2, _T2NT = 32, 33
+TWO_STEP_2D = {
+    0: {"2d1r_star": (510, 1, 32, 4, _T2), "2d2r_star": (252, 1, 32, 1, _T2NT), "2d3r_star": (122, 1, 64, 1, _T2),
+        "2d1r_box": (510, 1, 32, 2, _T2NT), "2d2r_box": (124, 1, 64, 1, _T2NT)},
+    1: {"2d1r_star": (510, 1, 16, 2, _T2NT), "2d2r_star": (252, 1, 32, 2, _T2NT), "2d3r_star": (122, 1, 64, 1, _T2NT),
+        "2d1r_box": (254, 1, 16, 4, _T2NT), "2d2r_box": (252, 1, 64, 1, _T2)},
+    2: {"2d1r_star": (254, 1, 32, 2, _T2NT), "2d2r_star": (124, 1, 64, 1, _T2), "2d3r_star": (122, 1, 64, 1, _T2NT),
+        "2d1r_box": (510, 1, 32, 2, _T2), "2d2r_box": (124, 1, 64, 1, _T2NT)},
+}
+
+
 @pytest.mark.parametrize("case", list(SHAPES))
 def test_one_step_configs(uhs, case):
-    """Only the FP64 radius-1 and radius-2 stars and the plain FP32 radius-1
-    star default to two steps per launch; their one-step configuration (leading steps, unbatched baseline) is the
-    one-step default tile, and every other default is its own one-step
-    configuration."""
+    """Defaults that advance two steps per launch: the FP64 radius-1 and
+    radius-2 3D stars, the plain FP32 radius-1 3D star, and the 2D stencils
+    but the radius-3 box in every precision.  Their one-step configuration
+    (leading steps, unbatched baseline) is the one-step default tile; every
+    other default is its own one-step configuration."""
     s = SHAPES[case]
     st = uhs.builtin(case)
     for ext in ([8192, 8192] if s["dims"] == 2 else [512, 512, 512],
@@ -93,6 +106,11 @@ def test_one_step_configs(uhs, case):
             elif case == "3d2r_star" and dt == uhs.DT_F64:
                 assert d.tuple() == (28, 12, 64 if big else 16, 2, uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE)
                 assert o.tuple() == (128, 16, 256 if big else 16, 8, uhs.CFG_NT_STORE)
+            elif s["dims"] == 2 and case != "2d3r_box":
+                tb = TWO_STEP_2D[dt][case]
+                assert d.flags == tb[4] and (d.tile_x, d.tile_y, d.vec) == (tb[0], 1, tb[3])
+                assert d.tile_z == tb[2] if ext[0] == 8192 else d.tile_z <= tb[2]
+                assert not o.flags & uhs.CFG_TEMPORAL2
             else:
                 assert d.tuple() == o.tuple()
 

@@ -1,9 +1,10 @@
 """The benchmark workloads themselves, at their own sizes, through the C ABI.
 
 * The headline sweep (3d1r_star FP64 512^3, the library's default: two
-  steps per launch) and the radius-2 star's two-step default, bitwise equal
-  to the oracle's fused mode at T = 4 (two two-step launches), and at T = 100
-  bitwise equal to one launch per step.
+  steps per launch), the radius-2 star's and the 2D stencils' two-step
+  defaults (8192^2), bitwise equal to the oracle's fused mode at T = 4 (two
+  two-step launches), and at T = 100 bitwise equal to one launch per step;
+  the FP32 two-step defaults (plain and compensated) the same at T = 4.
 * All 12 stencils at 512^3 / 8192^2 and T = 4 with dyadic symmetric
   coefficients on a linear field: the field is a fixed point of the sweep
   (every product and partial sum is exact, and a symmetric stencil whose
@@ -41,13 +42,20 @@ def normalised(u, case, seed):
     return st.with_coeffs(u.make_coeffs(st, seed))
 
 
-@pytest.mark.parametrize("case", ["3d1r_star", "3d2r_star"])
+TWO_STEP_F64 = ["3d1r_star", "3d2r_star", "2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box"]
+
+
+def full_ext(case):
+    return [8192, 8192] if case.startswith("2d") else [512, 512, 512]
+
+
+@pytest.mark.parametrize("case", TWO_STEP_F64)
 def test_headline_default_bitwise_vs_oracle_512(gpu, orc, case):
     u = gpu
     st = normalised(u, case, 1)
-    ext, h = [512, 512, 512], st.radius
+    ext, h = full_ext(case), st.radius
     cfg = u.default_config(ext, h, st)
-    log(case, "512^3 T=4 default", cfg)
+    log(case, ext, "T=4 default", cfg)
     assert cfg.flags & u.CFG_TEMPORAL2
     A0, B0 = orc.fill(ext, h, 1), orc.fill(ext, h, 2)
     dA, dB = u.DeviceArray.from_host(A0), u.DeviceArray.from_host(B0)
@@ -62,17 +70,17 @@ def test_headline_default_bitwise_vs_oracle_512(gpu, orc, case):
     assert np.array_equal(got, A0)
 
 
-@pytest.mark.parametrize("case", ["3d1r_star", "3d2r_star"])
+@pytest.mark.parametrize("case", TWO_STEP_F64)
 def test_headline_T100_two_step_equals_one_step(gpu, case):
     """The benchmark's own call (T = 100, default) against one launch per
     step (the kernel the oracle parity tests pin at every size)."""
     u = gpu
     st = normalised(u, case, 1)
-    ext, h = [512, 512, 512], st.radius
+    ext, h = full_ext(case), st.radius
     shape = u.grid_shape(ext, h)
     res = []
     for cfg in (None, u.one_step_config(ext, h, st)):
-        log(case, "512^3 T=100", cfg or u.default_config(ext, h, st))
+        log(case, ext, "T=100", cfg or u.default_config(ext, h, st))
         A, B = u.DeviceArray(shape, np.float64), u.DeviceArray(shape, np.float64)
         try:
             u.fill_uniform(A, ext, h, 1)
@@ -86,24 +94,29 @@ def test_headline_T100_two_step_equals_one_step(gpu, case):
     assert np.array_equal(res[0], res[1])
 
 
-def test_f32_two_step_default_vs_oracle_512(gpu, orc):
-    """The plain FP32 default of 3d1r_star advances two steps per launch."""
+@pytest.mark.parametrize("case,compensated", [("3d1r_star", False)] +
+                         [(c, k) for c in TWO_STEP_F64 if c.startswith("2d") for k in (False, True)])
+def test_f32_two_step_default_vs_oracle_512(gpu, orc, case, compensated):
+    """The FP32 defaults that advance two steps per launch: the plain
+    3d1r_star and the 2D stencils but the radius-3 box, plain and
+    compensated."""
     u = gpu
-    st = normalised(u, "3d1r_star", 3)
-    ext, h = [512, 512, 512], 1
-    cfg = u.default_config(ext, h, st, u.DT_F32)
-    log("3d1r_star f32 512^3 T=4", cfg)
+    st = normalised(u, case, 3)
+    ext, h = full_ext(case), st.radius
+    dt = u.DT_F32K if compensated else u.DT_F32
+    cfg = u.default_config(ext, h, st, dt)
+    log(case, "f32k" if compensated else "f32", ext, "T=4", cfg)
     assert cfg.flags & u.CFG_TEMPORAL2
     A0, B0 = orc.fill(ext, h, 5, np.float32), orc.fill(ext, h, 6, np.float32)
     dA, dB = u.DeviceArray.from_host(A0), u.DeviceArray.from_host(B0)
     try:
-        u.sweep_f32(dA, dB, ext, h, st, 4)
+        u.sweep_f32(dA, dB, ext, h, st, 4, compensated)
         u.synchronize()
         got = dA.download()
     finally:
         dA.close()
         dB.close()
-    orc.iterate(A0, B0, ext, h, st.offsets, st.coeffs, 4, FUSED)
+    orc.iterate(A0, B0, ext, h, st.offsets, st.coeffs, 4, KAHAN if compensated else FUSED)
     assert np.array_equal(got, A0)
 
 

@@ -123,6 +123,7 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
         return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0, planes);
     }
     case FAM_WIN2D:
+        if (c.flags & UHS_CFG_TEMPORAL2) return uhs_find_win2d_tb2(pl.prog, dt, c.vec);
         return uhs_find_win2d(pl.prog, dt, c.vec,
                               (c.flags & UHS_CFG_ROWRING)     ? 0
                               : (c.flags & UHS_CFG_PREFETCH4) ? 4
@@ -157,9 +158,28 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return bad("two planes of tile loads in flight exist for 3D boxes without deeper cell prefetch");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
     if (c.flags & UHS_CFG_TEMPORAL2) {
-        if (pl.family != FAM_STAR3D) return bad("two steps per launch exists for 3D star stencils only");
+        if (pl.family != FAM_STAR3D && pl.family != FAM_WIN2D)
+            return bad("two steps per launch exists for 3D star and 2D stencils only");
         if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE))
             return bad("two steps per launch combines with UHS_CFG_NT_STORE only");
+        if (pl.family == FAM_WIN2D) {
+            // Output columns tile_x plus an R-wide ghost zone each side, vec
+            // columns per thread; one row per step.
+            const long long bx = c.tile_x + 2LL * R;
+            if (c.tile_y != 1) return bad("2D stencils need tile_y = 1");
+            if (c.vec != 1 && c.vec != 2 && c.vec != 4)
+                return bad(str("two steps per launch needs vec 1, 2 or 4 (got %lld)", c.vec));
+            if (bx % c.vec) return bad(str("tile_x + 2r = %lld not a multiple of vec %lld", bx, c.vec));
+            const long long nt = bx / c.vec;
+            if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
+            if (nt < 2 * R) return bad(str("%lld threads cannot stage a radius-%lld ring", nt, R));
+            const size_t esz = a.dtype == DT_F64 ? 8 : 4;
+            const size_t lds = esz * (size_t)(2 * (bx + 2 * R) + 2 * bx + 2 * R);
+            if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
+            if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
+            if (!uhs_pick_fn(pl, c)) return bad(str("two-step 2D kernel not compiled for vec %lld", c.vec));
+            return UHS_OK;
+        }
         const long long bx = c.tile_x + 2LL * R, by = c.tile_y + 2LL * R;
         if (c.vec != 1 && c.vec != 2) return bad(str("two steps per launch needs vec 1 or 2 (got %lld)", c.vec));
         if (bx % c.vec) return bad(str("tile_x + 2r = %lld not a multiple of vec %lld", bx, c.vec));
@@ -258,31 +278,56 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
     {32, 32, 128, 1, UHS_CFG_ROWS2 | UHS_CFG_NT_STORE},      // 3d3r_box
 };
 
-// FP64 defaults that advance two steps per launch (UHS_CFG_TEMPORAL2),
-// where a timed sweep at 512^3 (T = 40) found them faster than the one-step
-// default: 3d1r_star 446 against 428 GSt/s for the earlier two-step default
-// and 337 for one step per launch (62x14 output tile, 64x16 with the ghost
-// zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327 against 320
-// (28x12 output tile, 64-plane slabs).  The radius-3 star stays one step per
-// launch (209 against 293 GSt/s for its best two-step tile).  The results
-// are bitwise those of the one-step kernels.
+// Defaults that advance two steps per launch (UHS_CFG_TEMPORAL2), where a
+// timed sweep at 512^3 / 8192^2 (T = 20 or 40) found them faster than the
+// one-step default.  The results are bitwise those of the one-step kernels.
+//   FP64: 3d1r_star 446 against 337 GSt/s one step (62x14 output tile, 64x16
+//   with the ghost zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327
+//   against 320 (28x12 output tile, 64-plane slabs); the 2D stencils 1.4-1.8x
+//   (2d1r_star 585 against 335, 2d2r_star 531 / 323, 2d3r_star 473 / 313,
+//   2d1r_box 602 / 335, 2d2r_box 431 / 311).  The radius-3 star stays one
+//   step per launch (209 against 293 GSt/s for its best two-step tile), and
+//   the 2D radius-3 box has no FP64 two-step kernel (its two windows do not
+//   fit the registers).
+//   FP32 plain: 3d1r_star 727 against 627; 2D 1.3-1.9x (2d1r_star 1118
+//   against 595, 2d2r_star 1040 / 619, 2d3r_star 802 / 601, 2d1r_box 1099 /
+//   605, 2d2r_box 818 / 559).
+//   FP32 compensated: 2D 1.1-1.7x (2d1r_star 1092 against 634, 2d2r_star 894
+//   / 582, 2d3r_star 605 / 532, 2d1r_box 982 / 590, 2d2r_box 464 / 418).
+// Slower with two steps per launch, so absent: the 3D radius-2 and radius-3
+// stars in FP32 (537 against 586, 327 against 440 GSt/s), the compensated
+// 3d1r_star (630 against 636) and the FP32 2D radius-3 box (299 against 424).
 static const uhs_config kTemporal2None = {0, 0, 0, 0, 0};
+constexpr int T2 = UHS_CFG_TEMPORAL2, T2NT = UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE;
 static const uhs_config kBuiltinTemporal2F64[12] = {
-    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
-    {62, 14, 32, 2, UHS_CFG_TEMPORAL2},                   // 3d1r_star
-    {28, 12, 64, 2, UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE}, // 3d2r_star
+    {510, 1, 32, 4, T2},   // 2d1r_star
+    {252, 1, 32, 1, T2NT}, // 2d2r_star
+    {122, 1, 64, 1, T2},   // 2d3r_star
+    {510, 1, 32, 2, T2NT}, // 2d1r_box
+    {124, 1, 64, 1, T2NT}, // 2d2r_box
+    kTemporal2None,        // 2d3r_box
+    {62, 14, 32, 2, T2},   // 3d1r_star
+    {28, 12, 64, 2, T2NT}, // 3d2r_star
     kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
 };
-
-// Plain FP32 (the DSL's float mode): the 7-point star also advances two
-// steps per launch, 727 against 627 GSt/s at 512^3 (T = 40).  The
-// compensated form and the radius-2 and radius-3 stars were faster with one
-// step per launch (3d1r_star compensated 630 against 636, 3d2r_star 537
-// against 586, 3d3r_star 327 against 440 GSt/s).
 static const uhs_config kBuiltinTemporal2F32[12] = {
-    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
-    {62, 14, 32, 2, UHS_CFG_TEMPORAL2}, // 3d1r_star
+    {510, 1, 16, 2, T2NT}, // 2d1r_star
+    {252, 1, 32, 2, T2NT}, // 2d2r_star
+    {122, 1, 64, 1, T2NT}, // 2d3r_star
+    {254, 1, 16, 4, T2NT}, // 2d1r_box
+    {252, 1, 64, 1, T2},   // 2d2r_box
+    kTemporal2None,        // 2d3r_box
+    {62, 14, 32, 2, T2},   // 3d1r_star
     kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+};
+static const uhs_config kBuiltinTemporal2F32K[12] = {
+    {254, 1, 32, 2, T2NT}, // 2d1r_star
+    {124, 1, 64, 1, T2},   // 2d2r_star
+    {122, 1, 64, 1, T2NT}, // 2d3r_star
+    {510, 1, 32, 2, T2},   // 2d1r_box
+    {124, 1, 64, 1, T2NT}, // 2d2r_box
+    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+    kTemporal2None,
 };
 
 uhs_config uhs_heuristic_config(const Plan &pl, bool one_step)
@@ -292,10 +337,10 @@ uhs_config uhs_heuristic_config(const Plan &pl, bool one_step)
     if (pl.family == FAM_GATHER) return c;
     c = a.dtype == DT_F64 ? kBuiltinDefaults[pl.prog] : kBuiltinDefaultsF32[pl.prog];
     char why[128];
-    const uhs_config *two = a.dtype == DT_F64 ? &kBuiltinTemporal2F64[pl.prog]
+    const uhs_config *two = a.dtype == DT_F64   ? &kBuiltinTemporal2F64[pl.prog]
                             : a.dtype == DT_F32 ? &kBuiltinTemporal2F32[pl.prog]
-                                                : nullptr;
-    if (!one_step && two && two->flags && uhs_check_config(pl, *two, why, sizeof why) == UHS_OK) c = *two;
+                                                : &kBuiltinTemporal2F32K[pl.prog];
+    if (!one_step && two->flags && uhs_check_config(pl, *two, why, sizeof why) == UHS_OK) c = *two;
     if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) {
         // e.g. an FP32 form whose vec variant is not compiled
         c.vec = 1;

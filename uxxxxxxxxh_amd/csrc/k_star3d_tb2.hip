@@ -4,12 +4,6 @@
 
 namespace {
 
-template <typename T, int P> struct TB2Params {
-    KParams<T, P> k;  // first member: the coefficient pack keeps its kernarg offset
-    const T *hA, *hB; // interior origins of the grids the ping-pong calls A and B
-    int k0;           // global step of the input state
-};
-
 template <typename T, class S, int VX, bool KAHAN, int PF>
 __global__ void __launch_bounds__(kMaxThreads) k_star3d_tb2(TB2Params<T, S::P> tp)
 {

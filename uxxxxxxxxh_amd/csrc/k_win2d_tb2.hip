@@ -1,0 +1,284 @@
+// win2d_tb2 kernel: two timesteps per launch for the 2D stencils (temporal
+// blocking with an R-column ghost zone), so each cell crosses HBM once per
+// two updates instead of once per update.  FP64, FP32 and FP32 compensated.
+//
+// A workgroup owns output columns [x0, x0 + tile_x) of rows [ys, ye) and
+// covers the columns extended by R on each side (BX = tile_x + 2R, VX
+// columns per thread along the unit-stride axis).  It streams rows along
+// axis 0.  At step s it stages level-0 row r0 = ys - 2R + s (the input) in
+// LDS, and level-1 row r0 - R - 1 (one step, computed at step s - 1):
+//   -- one barrier --
+//   each thread moves the 2R+1 cells of row r0 around each of its columns
+//     into its level-0 window (a register ring of 2R+1 rows), and the 2R+1
+//     level-1 cells of row r0 - R - 1 into its level-1 window;
+//   level 2 at row r0 - 2R - 1 for the tile's own columns, from the level-1
+//     window, is stored;
+//   level 1 at row r0 - R for every extended column, from the level-0
+//     window, is kept in a register until the next step stages it.
+// Buffer s&1 is rewritten two steps later, after the next barrier, so one
+// barrier per row suffices.  The windows are rings indexed by the step's
+// phase, with the row loop unrolled by hand by 2R+1, so no register copies
+// shift them; a star's window cells off its centre row and column are dead
+// and take no registers.
+//
+// Cells of a level outside the interior hold that level's halo: the halo of
+// the grid the one-step ping-pong holds that state in (h0 for the input, h1
+// for level 1).  Every update is the product-then-fma chain in term order,
+// so the result is bitwise that of two single sweeps.
+#include "sweep_kernels.h"
+
+namespace {
+
+template <int R> __device__ __forceinline__ bool win2d_tb2_full(int x0, int TX, int ys, int ye, int n0, int n2)
+{
+    return x0 >= 2 * R && x0 + TX + 2 * R <= n2 && ys >= 2 * R && ye + 2 * R <= n0;
+}
+
+// FULL: every cell whose value is used (level-0 columns [x0-2R, x0+TX+2R)
+// of rows [ys-2R, ye+2R)) lies in the interior: no halo selects, no clamps
+// but the prefetch's past the last row.
+template <typename T, class S, int VX, bool KAHAN, bool FULL, class CF>
+__device__ __forceinline__ void win2d_tb2_tile(const T *__restrict__ in, T *__restrict__ out,
+                                               const T *__restrict__ h0, const T *__restrict__ h1,
+                                               const KParams<T, S::P> &p, int x0, int ys, int ye, T *lds, CF &cf)
+{
+    constexpr int R = S::R, P = S::P, W = 2 * R + 1;
+    const int tid = threadIdx.x, BXT = p.BXT;
+    const int BX = p.TX + 2 * R, LW = BX + 2 * R;
+    T *L0 = lds;              // 2 x LW: level-0 row, columns x0-2R ..
+    T *L1 = lds + 2 * LW + R; // 2 x BX: level-1 row, columns x0-R .. (R spare cells either side)
+    lds_poison(lds, 2 * LW + 2 * BX + 2 * R);
+    const int n0 = p.n0, n2 = p.n2;
+    const long long sy = p.sy;
+    const bool nts = p.flags & UHS_CFG_NT_STORE;
+
+    // Extended column ex = tid + v*BXT is interior column x0 - R + ex.
+    int xcl[VX];
+    bool xin[VX], core[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int ex = tid + v * BXT, xg = x0 - R + ex;
+        xin[v] = FULL || (xg >= 0 && xg < n2);
+        core[v] = ex >= R && ex < BX - R && (FULL || xg < n2);
+        xcl[v] = FULL ? xg : clampi(xg, -R, n2 + R - 1);
+    }
+    // Threads 0 .. 2R-1 also stage the R level-0 cells beyond the extended
+    // columns on each side (LDS index hlx, interior column x0 - 2R + hlx);
+    // the others load a valid cell they never stage, so the loads stay
+    // unconditional.
+    const bool hthr = tid < 2 * R;
+    const int hlx = tid < R ? tid : BX + tid;
+    const int hxg = x0 - 2 * R + hlx;
+    const bool hin = FULL || (hxg >= 0 && hxg < n2);
+    const int hcl = FULL && hthr ? hxg : clampi(hxg, -R, n2 + R - 1);
+
+    // Level-0 value at row r: the input inside the interior, h0 outside it.
+    // Rows beyond the allocation are clamped; their values are never used.
+    auto lev0 = [&](bool cin, int r, int c) -> T {
+        if constexpr (FULL) {
+            return in[(long long)min(r, n0 + R - 1) * sy + c];
+        } else {
+            const T *base = (cin && r >= 0 && r < n0) ? in : h0;
+            return base[(long long)clampi(r, -R, n0 + R - 1) * sy + c];
+        }
+    };
+
+    // Prefetch ring: slot f holds the level-0 row staged at the next step
+    // with phase f, loaded 2R+1 rows ahead.
+    T nrow[W][VX], nh[W];
+#pragma unroll
+    for (int f = 0; f < W; f++) {
+#pragma unroll
+        for (int v = 0; v < VX; v++) nrow[f][v] = lev0(xin[v], ys - 2 * R + f, xcl[v]);
+        nh[f] = lev0(hin, ys - 2 * R + f, hcl);
+    }
+    // Windows: w0[v][k][b] is level-0 row r (staged at a step of phase k)
+    // at column x - R + b; w1 the same for level 1.
+    T w0[VX][W][W], w1[VX][W][W], v1[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        v1[v] = T(0);
+#pragma unroll
+        for (int a = 0; a < W; a++)
+#pragma unroll
+            for (int b = 0; b < W; b++) w0[v][a][b] = w1[v][a][b] = T(0);
+    }
+    const int N = (ye - ys) + 4 * R + 1; // steps
+
+    // One step s with phase PH = s mod W.  L1ON: level 1 is needed (s >= 2R);
+    // W1ON: the level-1 row read is one of them (s >= 2R+1); OUT: level 2 is
+    // stored (s >= 4R+1).
+    auto step = [&](auto phc, auto l1c, auto w1c, auto outc, int s) {
+        constexpr int PH = decltype(phc)::value;
+        constexpr bool L1ON = decltype(l1c)::value, W1ON = decltype(w1c)::value, OUT = decltype(outc)::value;
+        const int r0 = ys - 2 * R + s;
+        T *S0 = L0 + (s & 1) * LW;
+        T *S1 = L1 + (s & 1) * BX;
+#pragma unroll
+        for (int v = 0; v < VX; v++) S0[R + tid + v * BXT] = nrow[PH][v];
+        if (hthr) S0[hlx] = nh[PH];
+        if constexpr (W1ON) {
+#pragma unroll
+            for (int v = 0; v < VX; v++) S1[tid + v * BXT] = v1[v];
+        }
+        // Refill the slot with the row 2R+1 ahead (clamped past the end).
+#pragma unroll
+        for (int v = 0; v < VX; v++) nrow[PH][v] = lev0(xin[v], r0 + W, xcl[v]);
+        nh[PH] = lev0(hin, r0 + W, hcl);
+        __syncthreads();
+        cf.refresh();
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+#pragma unroll
+            for (int b = 0; b < W; b++) w0[v][PH][b] = S0[tid + v * BXT + b];
+        if constexpr (W1ON) {
+            // Columns near the tile edges read the spare cells or the other
+            // buffer; their level-2 sums are not stored.
+#pragma unroll
+            for (int v = 0; v < VX; v++)
+#pragma unroll
+                for (int b = 0; b < W; b++) w1[v][PH][b] = S1[tid + v * BXT - R + b];
+        }
+        // Row r of a window sits in the slot of the step that staged it: the
+        // level-0 row r0 - R + dy in slot (PH + R + 1 + dy) mod W, and the
+        // level-1 row r0 - 2R - 1 + dy the same.
+        if constexpr (OUT) {
+            const int y2 = r0 - 2 * R - 1;
+            T *orow = out + (long long)y2 * sy;
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                Acc<T, KAHAN> a;
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int dy = S::OFF[t][0], dx = S::OFF[t][1];
+                    const T val = w1[v][(PH + R + 1 + dy) % W][R + dx];
+                    if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                    else a.add(cf.template get<t>(), val);
+                });
+                if (core[v]) store_out(orow + xcl[v], a.value(), nts);
+            }
+        }
+        if constexpr (L1ON) {
+            const int y1 = r0 - R;
+            const bool yin = FULL || (y1 >= 0 && y1 < n0);
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                Acc<T, KAHAN> a;
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int dy = S::OFF[t][0], dx = S::OFF[t][1];
+                    const T val = w0[v][(PH + R + 1 + dy) % W][R + dx];
+                    if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                    else a.add(cf.template get<t>(), val);
+                });
+                v1[v] = a.value();
+                if (!FULL && !(yin && xin[v])) v1[v] = h1[(long long)clampi(y1, -R, n0 + R - 1) * sy + xcl[v]];
+            }
+        }
+    };
+    using B0 = std::false_type;
+    using B1 = std::true_type;
+    // Steps 0 .. 4R: no level-2 output yet (2W - 1 steps).
+    static_for<2 * W - 1>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        step(std::integral_constant<int, i % W>{}, std::integral_constant<bool, (i >= 2 * R)>{},
+             std::integral_constant<bool, (i >= 2 * R + 1)>{}, B0{}, i);
+    });
+    // Whole ring turns from step 2W - 1 (phase W - 1), then the rest.
+    int s0 = 2 * W - 1;
+    for (; s0 + W <= N; s0 += W)
+        static_for<W>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            step(std::integral_constant<int, (W - 1 + i) % W>{}, B1{}, B1{}, B1{}, s0 + i);
+        });
+    static_for<W - 1>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if (s0 + i < N) step(std::integral_constant<int, (W - 1 + i) % W>{}, B1{}, B1{}, B1{}, s0 + i);
+    });
+}
+
+template <typename T, class S, int VX, bool KAHAN>
+__global__ void __launch_bounds__(kMaxThreads) k_win2d_tb2(TB2Params<T, S::P> tp)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const KParams<T, S::P> &p = tp.k;
+    const int bid = xcd_remap(blockIdx.x, p.nblocks);
+    const int tile_x = bid % p.tiles_x, chunk = bid / p.tiles_x;
+    const int x0 = tile_x * p.TX, ys = chunk * p.ZC, ye = min(ys + p.ZC, p.n0);
+    const T *h0 = (tp.k0 & 1) ? tp.hB : tp.hA;
+    const T *h1 = (tp.k0 & 1) ? tp.hA : tp.hB;
+    Coefs<T, S::P> cf(p);
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    if (win2d_tb2_full<S::R>(x0, p.TX, ys, ye, p.n0, p.n2))
+        win2d_tb2_tile<T, S, VX, KAHAN, true>(p.in, p.out, h0, h1, p, x0, ys, ye, lds, cf);
+    else
+        win2d_tb2_tile<T, S, VX, KAHAN, false>(p.in, p.out, h0, h1, p, x0, ys, ye, lds, cf);
+}
+
+template <typename T, class S, int VX, bool KAHAN>
+hipError_t launch_win2d_tb2(const SweepArgs &a, hipStream_t st)
+{
+    TB2Params<T, S::P> tp;
+    tp.k = make_params<T, S::P>(a);
+    constexpr int R = S::R;
+    const int bx = a.tx + 2 * R;
+    tp.k.BXT = bx / VX;
+    const long long h = a.halo, origin = h * a.sy + h;
+    tp.hA = static_cast<const T *>(a.hA) + origin;
+    tp.hB = static_cast<const T *>(a.hB) + origin;
+    tp.k0 = a.k0;
+    const size_t lds = sizeof(T) * (2 * (size_t)(bx + 2 * R) + 2 * (size_t)bx + 2 * R);
+    hipLaunchKernelGGL((k_win2d_tb2<T, S, VX, KAHAN>), dim3(tp.k.nblocks), dim3(bx / VX), lds, st, tp);
+    return hipGetLastError();
+}
+
+// Compiled where the two windows fit the registers without spilling: the
+// windows hold 2(2R+1)^2 cells per column (a star's dead cells excepted), so
+// FP64 radius 2 and 3 stars take one column per thread and the FP64 radius-3
+// box has no two-step form.
+template <typename T, class S> constexpr int tb2_max_vec()
+{
+    constexpr bool f64 = sizeof(T) == 8;
+    if constexpr (S::R == 1) return 4;
+    else if constexpr (S::R == 2) return f64 ? 1 : 2;
+    else return (f64 && S::P != 4 * S::R + 1) ? 0 : 1; // a star has 4R+1 terms
+}
+template <typename T, bool K, class S> LaunchFn pick_vec(int vec)
+{
+    constexpr int M = tb2_max_vec<T, S>();
+    switch (vec) {
+    case 1:
+        if constexpr (M >= 1) return launch_win2d_tb2<T, S, 1, K>;
+        else return nullptr;
+    case 2:
+        if constexpr (M >= 2) return launch_win2d_tb2<T, S, 2, K>;
+        else return nullptr;
+    case 4:
+        if constexpr (M >= 4) return launch_win2d_tb2<T, S, 4, K>;
+        else return nullptr;
+    }
+    return nullptr;
+}
+template <class S> LaunchFn pick(int dtype, int vec)
+{
+    switch (dtype) {
+    case DT_F64: return pick_vec<double, false, S>(vec);
+    case DT_F32: return pick_vec<float, false, S>(vec);
+    case DT_F32K: return pick_vec<float, true, S>(vec);
+    }
+    return nullptr;
+}
+} // namespace
+
+LaunchFn uhs_find_win2d_tb2(int prog, int dtype, int vec)
+{
+    switch (prog) {
+    case Prog_2d1r_star::ID: return pick<Prog_2d1r_star>(dtype, vec);
+    case Prog_2d2r_star::ID: return pick<Prog_2d2r_star>(dtype, vec);
+    case Prog_2d3r_star::ID: return pick<Prog_2d3r_star>(dtype, vec);
+    case Prog_2d1r_box::ID: return pick<Prog_2d1r_box>(dtype, vec);
+    case Prog_2d2r_box::ID: return pick<Prog_2d2r_box>(dtype, vec);
+    case Prog_2d3r_box::ID: return pick<Prog_2d3r_box>(dtype, vec);
+    }
+    return nullptr;
+}

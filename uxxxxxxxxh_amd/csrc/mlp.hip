@@ -272,7 +272,8 @@ int uhs_mlp_features(int dims, int radius, int star, int dtype, const int64_t *e
     int threads = c->tile_x / v * c->tile_y;
     if (c->flags & UHS_CFG_REGTILE) threads = 64 * (c->tile_y / v);
     else if (c->flags & UHS_CFG_ROWS2) threads = c->tile_x * c->tile_y / 2;
-    else if (c->flags & UHS_CFG_TEMPORAL2) threads = (c->tile_x + 2 * radius) / v * (c->tile_y + 2 * radius);
+    else if (c->flags & UHS_CFG_TEMPORAL2)
+        threads = (c->tile_x + 2 * radius) / v * (dims == 3 ? c->tile_y + 2 * radius : 1);
     f[11] = log2f((float)std::max(1, threads));
     f[12] = (float)c->vec;
     if (c->flags & UHS_CFG_TEMPORAL2) depth = std::max(depth, 2);

@@ -366,6 +366,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_star3d(KParams<T, S::P> p)
 // from a halo: the loads take the plane's base address plus a per-lane
 // offset, with no clamp or halo select, and no level-1 halo is loaded
 // (star3d_tb2_full tells whether a tile qualifies).
+// Arguments of a two-step launch (3D stars and 2D stencils).
+template <typename T, int P> struct TB2Params {
+    KParams<T, P> k;  // first member: the coefficient pack keeps its kernarg offset
+    const T *hA, *hB; // interior origins of the grids the ping-pong calls A and B
+    int k0;           // global step of the input state
+};
+
 template <int R>
 __device__ __forceinline__ bool star3d_tb2_full(int x0, int y0, int zs, int ze, int TX, int TY, int n0, int n1,
                                                 int n2)

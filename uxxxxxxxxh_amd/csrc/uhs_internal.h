@@ -48,6 +48,7 @@ LaunchFn uhs_find_star3d_rt(int prog, int dtype, int rows);
 LaunchFn uhs_find_star3d_tb2(int prog, int dtype, int vec, int prefetch); // two steps per launch
 LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep, int planes);
 LaunchFn uhs_find_win2d(int prog, int dtype, int vec, int prefetch);
+LaunchFn uhs_find_win2d_tb2(int prog, int dtype, int vec); // two steps per launch
 LaunchFn uhs_find_gather(int dtype);
 
 // Limits baked into the kernels; the host rejects configurations beyond them.

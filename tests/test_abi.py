@@ -68,15 +68,20 @@ def test_plan_kinds_and_default_configs(uhs, case):
     uhs.default_config(small, s["radius"], st)
 
 
-# The 2D two-step defaults at 8192^2 (tile_x, tile_y, tile_z, vec, flags).
-_T2, _T2NT = 32, 33
+# The 2D two-step defaults at 8192^2 (tile_x, tile_y, tile_z, vec, flags):
+# UHS_CFG_TEMPORAL2 (32), with UHS_CFG_NT_STORE (1) and, for the window form,
+# UHS_CFG_ROWRING (128).
+_T2, _NT, _RR = 32, 1, 128
 TWO_STEP_2D = {
-    0: {"2d1r_star": (510, 1, 32, 4, _T2), "2d2r_star": (252, 1, 32, 1, _T2NT), "2d3r_star": (122, 1, 64, 1, _T2),
-        "2d1r_box": (510, 1, 32, 2, _T2NT), "2d2r_box": (124, 1, 64, 1, _T2NT)},
-    1: {"2d1r_star": (510, 1, 16, 2, _T2NT), "2d2r_star": (252, 1, 32, 2, _T2NT), "2d3r_star": (122, 1, 64, 1, _T2NT),
-        "2d1r_box": (254, 1, 16, 4, _T2NT), "2d2r_box": (252, 1, 64, 1, _T2)},
-    2: {"2d1r_star": (254, 1, 32, 2, _T2NT), "2d2r_star": (124, 1, 64, 1, _T2), "2d3r_star": (122, 1, 64, 1, _T2NT),
-        "2d1r_box": (510, 1, 32, 2, _T2), "2d2r_box": (124, 1, 64, 1, _T2NT)},
+    0: {"2d1r_star": (510, 1, 32, 4, _T2 | _RR), "2d2r_star": (508, 1, 16, 2, _T2 | _NT),
+        "2d3r_star": (506, 1, 64, 1, _T2), "2d1r_box": (510, 1, 32, 2, _T2 | _NT | _RR),
+        "2d2r_box": (124, 1, 64, 1, _T2 | _NT | _RR)},
+    1: {"2d1r_star": (510, 1, 16, 2, _T2 | _NT | _RR), "2d2r_star": (124, 1, 32, 2, _T2 | _NT | _RR),
+        "2d3r_star": (506, 1, 32, 2, _T2), "2d1r_box": (254, 1, 16, 4, _T2 | _NT | _RR),
+        "2d2r_box": (252, 1, 64, 1, _T2 | _RR)},
+    2: {"2d1r_star": (254, 1, 32, 2, _T2 | _NT | _RR), "2d2r_star": (252, 1, 32, 2, _T2 | _NT),
+        "2d3r_star": (250, 1, 32, 2, _T2), "2d1r_box": (510, 1, 32, 2, _T2 | _RR),
+        "2d2r_box": (124, 1, 64, 1, _T2 | _NT | _RR)},
 }
 
 

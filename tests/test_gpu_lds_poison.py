@@ -20,7 +20,7 @@ from tests.test_program import random_program
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 SHAPES = json.load(open(os.path.join(HERE, "golden", "shapes.json")))
-T2, NT, PF2, ROWS2, PL2 = 32, 1, 2, 16, 64
+T2, NT, PF2, ROWS2, PL2, RR = 32, 1, 2, 16, 64, 128
 
 
 def log(*a):
@@ -93,8 +93,9 @@ def test_two_step_tiles_under_poison(poisoned, orc, case):
 @pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box"])
 def test_two_step_tiles_2d_under_poison(poisoned, orc, case):
     R = SHAPES[case]["radius"]
-    for bx, tz, v, fl in ((64, 16, 1, T2), (16, 5, 1, T2 | NT), (128, 7, 2 if R == 1 else 1, T2)):
-        check(poisoned, orc, case, [150, 301], 5, (bx - 2 * R, 1, tz, v, fl))
+    for form in ((0, RR) if "star" in case else (RR,)):  # ring form (stars), window form
+        for bx, tz, v, fl in ((64, 16, 1, T2), (16, 5, 1, T2 | NT), (128, 7, 2 if R == 1 or not form else 1, T2)):
+            check(poisoned, orc, case, [150, 301], 5, (bx - 2 * R, 1, tz, v, fl | form))
 
 
 @pytest.mark.parametrize("case", ["3d1r_box", "3d2r_box", "3d3r_box", "3d1r_star", "3d2r_star"])

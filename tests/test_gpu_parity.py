@@ -207,45 +207,49 @@ def test_two_steps_per_launch(gpu, orc, case, dtype):
                 assert np.array_equal(gA[mask], A0[mask]) and np.array_equal(gB[mask], B0[mask])
 
 
-def two_step_vecs_2d(case, dtype):
-    """Columns per thread the 2D two-step kernel is compiled for (its two
-    windows must fit the registers)."""
+def two_step_forms_2d(case, dtype):
+    """(form flag, columns per thread) the 2D two-step kernel is compiled
+    for: the ring form (flag 0, stars only) at 1, 2 and 4 columns; the window
+    form (UHS_CFG_ROWRING) where its two windows fit the registers."""
     R = SHAPES[case]["radius"]
-    if R == 1:
-        return [1, 2, 4]
-    if R == 2:
-        return [1] if dtype == "f64" else [1, 2]
-    return ([1] if "star" in case else []) if dtype == "f64" else [1]
+    star = "star" in case
+    f64 = dtype == "f64"
+    win = [1, 2, 4] if R == 1 else ([1] if f64 else [1, 2]) if R == 2 else ([] if f64 and not star else [1])
+    out = [(128, v) for v in win]
+    if star:
+        out += [(0, v) for v in (1, 2, 4)]
+    return out
 
 
 @pytest.mark.parametrize("case", [c for c in CASES if c.startswith("2d")])
 @pytest.mark.parametrize("dtype", ["f64", "f32", "f32k"])
 def test_two_steps_per_launch_2d(gpu, orc, case, dtype):
-    """UHS_CFG_TEMPORAL2 for the 2D stencils, as for the 3D stars: the result
-    grid is bitwise the oracle's for T = 1..6 and 9, halos untouched and
-    every state's halo read from the grid that holds it, with edge tiles in
-    both axes, slabs of 3 to 64 rows and halos wider than the radius."""
+    """UHS_CFG_TEMPORAL2 for the 2D stencils, both forms, as for the 3D
+    stars: the result grid is bitwise the oracle's for T = 1..6 and 9, halos
+    untouched and every state's halo read from the grid that holds it, with
+    edge tiles in both axes, slabs of 3 to 64 rows and halos wider than the
+    radius."""
     u = gpu
     s = SHAPES[case]
     R = s["radius"]
     st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 16))
     dt = np.float64 if dtype == "f64" else np.float32
     T2, NT = u.CFG_TEMPORAL2, u.CFG_NT_STORE
-    vecs = two_step_vecs_2d(case, dtype)
-    if not vecs:
+    forms = two_step_forms_2d(case, dtype)
+    if not forms:
         with pytest.raises(u.UhsError):
-            u.plan_kind([67, 301], R, st, u.DT_F64, cfg=(64 - 2 * R, 1, 16, 1, T2))
+            u.plan_kind([67, 301], R, st, u.DT_F64, cfg=(64 - 2 * R, 1, 16, 1, T2 | u.CFG_ROWRING))
         return
     for ext, h in (([67, 301], R), ([23, 45], R + 1), ([150, 40], R)):
         A0, B0 = random_grids(ext, h, 5)
         A0, B0 = A0.astype(dt), B0.astype(dt)
         mask = np.ones(A0.shape, bool)
         mask[tuple(slice(h, n - h) for n in A0.shape)] = False
-        for v in vecs:
+        for form, v in forms:
             for bx, tz, fl in ((64, 16, T2), (16, 3, T2 | NT), (128, 5, T2), (256, 64, T2 | NT)):
-                if bx % v:
+                if bx % v or bx // v < 2 * R:  # the ring needs 2r staging threads
                     continue
-                cfg = (bx - 2 * R, 1, tz, v, fl)
+                cfg = (bx - 2 * R, 1, tz, v, fl | form)
                 for T in (1, 2, 3, 4, 5, 6, 9):
                     log(case, dtype, ext, h, cfg, "T", T)
                     gA, gB = run_gpu(u, st, A0, B0, ext, h, T, dt, dtype == "f32k", cfg=cfg)
@@ -258,21 +262,25 @@ def test_two_steps_per_launch_2d(gpu, orc, case, dtype):
 
 def test_two_steps_per_launch_config_checks_2d(gpu):
     u = gpu
-    T2 = u.CFG_TEMPORAL2
+    T2, RR = u.CFG_TEMPORAL2, u.CFG_ROWRING
     ext = [64, 96]
-    s1, s3 = u.builtin("2d1r_star"), u.builtin("2d3r_star")
+    s1, s3, b1 = u.builtin("2d1r_star"), u.builtin("2d3r_star"), u.builtin("2d1r_box")
     assert u.plan_kind(ext, 1, s1, cfg=(254, 1, 32, 4, T2)) == "win2d"
-    assert u.plan_kind(ext, 3, s3, cfg=(58, 1, 32, 1, T2)) == "win2d"
-    for st, bad in ((s1, (61, 1, 32, 2, T2)),            # tile_x + 2r odd with vec 2
-                    (s1, (62, 2, 32, 2, T2)),            # tile_y 2
-                    (s1, (1022, 1, 32, 1, T2)),          # 1024 threads
-                    (s1, (62, 1, 32, 2, T2 | u.CFG_ROWRING)),
-                    (s3, (58, 1, 32, 2, T2)),            # FP64 radius-3 star: one column per thread
-                    (u.builtin("2d3r_box"), (58, 1, 32, 1, T2))):  # no FP64 radius-3 box kernel
+    assert u.plan_kind(ext, 1, s1, cfg=(254, 1, 32, 4, T2 | RR)) == "win2d"
+    assert u.plan_kind(ext, 3, s3, cfg=(58, 1, 32, 1, T2 | RR)) == "win2d"
+    assert u.plan_kind(ext, 3, s3, cfg=(58, 1, 32, 4, T2)) == "win2d"
+    assert u.plan_kind(ext, 1, b1, cfg=(62, 1, 32, 2, T2 | RR)) == "win2d"
+    for st, bad in ((s1, (61, 1, 32, 2, T2)),                  # tile_x + 2r odd with vec 2
+                    (s1, (62, 2, 32, 2, T2)),                  # tile_y 2
+                    (s1, (1022, 1, 32, 1, T2)),                # 1024 threads
+                    (s1, (62, 1, 32, 2, T2 | u.CFG_PREFETCH2)),
+                    (s3, (58, 1, 32, 2, T2 | RR)),             # FP64 radius-3 window: one column per thread
+                    (b1, (62, 1, 32, 2, T2)),                  # a box has no ring form
+                    (u.builtin("2d3r_box"), (58, 1, 32, 1, T2 | RR))):  # no FP64 radius-3 box kernel
         with pytest.raises(u.UhsError) as e:
             u.plan_kind(ext, st.radius, st, cfg=bad)
         assert e.value.code == -3, bad
-    assert u.plan_kind(ext, 3, u.builtin("2d3r_box"), u.DT_F32, cfg=(58, 1, 32, 1, T2)) == "win2d"
+    assert u.plan_kind(ext, 3, u.builtin("2d3r_box"), u.DT_F32, cfg=(58, 1, 32, 1, T2 | RR)) == "win2d"
 
 
 def test_two_steps_per_launch_config_checks(gpu):

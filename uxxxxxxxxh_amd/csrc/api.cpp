@@ -123,7 +123,8 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
         return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0, planes);
     }
     case FAM_WIN2D:
-        if (c.flags & UHS_CFG_TEMPORAL2) return uhs_find_win2d_tb2(pl.prog, dt, c.vec);
+        if (c.flags & UHS_CFG_TEMPORAL2)
+            return uhs_find_win2d_tb2(pl.prog, dt, c.vec, (c.flags & UHS_CFG_ROWRING) ? 1 : 0);
         return uhs_find_win2d(pl.prog, dt, c.vec,
                               (c.flags & UHS_CFG_ROWRING)     ? 0
                               : (c.flags & UHS_CFG_PREFETCH4) ? 4
@@ -160,11 +161,17 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     if (c.flags & UHS_CFG_TEMPORAL2) {
         if (pl.family != FAM_STAR3D && pl.family != FAM_WIN2D)
             return bad("two steps per launch exists for 3D star and 2D stencils only");
-        if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE))
-            return bad("two steps per launch combines with UHS_CFG_NT_STORE only");
         if (pl.family == FAM_WIN2D) {
             // Output columns tile_x plus an R-wide ghost zone each side, vec
-            // columns per thread; one row per step.
+            // columns per thread; one row per step.  UHS_CFG_ROWRING selects
+            // the window form (2r+1 rows of loads in flight in a register
+            // ring, a (2r+1)^2 window per column and level); without it a
+            // star takes the ring form (2r+2 rows of each column in
+            // registers, loads two rows ahead).
+            if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE | UHS_CFG_ROWRING))
+                return bad("2D two steps per launch combines with UHS_CFG_NT_STORE and UHS_CFG_ROWRING only");
+            if (!(c.flags & UHS_CFG_ROWRING) && !kBuiltins[pl.prog].star)
+                return bad("a 2D box advances two steps per launch in the window form (UHS_CFG_ROWRING)");
             const long long bx = c.tile_x + 2LL * R;
             if (c.tile_y != 1) return bad("2D stencils need tile_y = 1");
             if (c.vec != 1 && c.vec != 2 && c.vec != 4)
@@ -180,6 +187,8 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
             if (!uhs_pick_fn(pl, c)) return bad(str("two-step 2D kernel not compiled for vec %lld", c.vec));
             return UHS_OK;
         }
+        if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE))
+            return bad("two steps per launch combines with UHS_CFG_NT_STORE only");
         const long long bx = c.tile_x + 2LL * R, by = c.tile_y + 2LL * R;
         if (c.vec != 1 && c.vec != 2) return bad(str("two steps per launch needs vec 1 or 2 (got %lld)", c.vec));
         if (bx % c.vec) return bad(str("tile_x + 2r = %lld not a multiple of vec %lld", bx, c.vec));
@@ -279,53 +288,56 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 };
 
 // Defaults that advance two steps per launch (UHS_CFG_TEMPORAL2), where a
-// timed sweep at 512^3 / 8192^2 (T = 20 or 40) found them faster than the
+// timed sweep at 512^3 / 8192^2 (T = 20 to 100) found them faster than the
 // one-step default.  The results are bitwise those of the one-step kernels.
 //   FP64: 3d1r_star 446 against 337 GSt/s one step (62x14 output tile, 64x16
 //   with the ghost zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327
 //   against 320 (28x12 output tile, 64-plane slabs); the 2D stencils 1.4-1.8x
-//   (2d1r_star 585 against 335, 2d2r_star 531 / 323, 2d3r_star 473 / 313,
+//   (2d1r_star 588 against 335, 2d2r_star 541 / 323, 2d3r_star 508 / 313,
 //   2d1r_box 602 / 335, 2d2r_box 431 / 311).  The radius-3 star stays one
 //   step per launch (209 against 293 GSt/s for its best two-step tile), and
 //   the 2D radius-3 box has no FP64 two-step kernel (its two windows do not
 //   fit the registers).
-//   FP32 plain: 3d1r_star 727 against 627; 2D 1.3-1.9x (2d1r_star 1118
-//   against 595, 2d2r_star 1040 / 619, 2d3r_star 802 / 601, 2d1r_box 1099 /
+//   FP32 plain: 3d1r_star 727 against 627; 2D 1.3-1.8x (2d1r_star 1080
+//   against 595, 2d2r_star 1029 / 619, 2d3r_star 909 / 601, 2d1r_box 1099 /
 //   605, 2d2r_box 818 / 559).
-//   FP32 compensated: 2D 1.1-1.7x (2d1r_star 1092 against 634, 2d2r_star 894
-//   / 582, 2d3r_star 605 / 532, 2d1r_box 982 / 590, 2d2r_box 464 / 418).
+//   FP32 compensated: 2D 1.1-1.7x (2d1r_star 1058 against 634, 2d2r_star 897
+//   / 582, 2d3r_star 747 / 532, 2d1r_box 982 / 590, 2d2r_box 464 / 418).
+// 2D stars take the window form (UHS_CFG_ROWRING) or the ring form, whichever
+// timed faster; the two are within 1-2% at radius 1 and 2, the ring form is
+// 4-17% faster at radius 3.
 // Slower with two steps per launch, so absent: the 3D radius-2 and radius-3
 // stars in FP32 (537 against 586, 327 against 440 GSt/s), the compensated
 // 3d1r_star (630 against 636) and the FP32 2D radius-3 box (299 against 424).
 static const uhs_config kTemporal2None = {0, 0, 0, 0, 0};
-constexpr int T2 = UHS_CFG_TEMPORAL2, T2NT = UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE;
+constexpr int T2 = UHS_CFG_TEMPORAL2, T2NT = UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE, RR = UHS_CFG_ROWRING;
 static const uhs_config kBuiltinTemporal2F64[12] = {
-    {510, 1, 32, 4, T2},   // 2d1r_star
-    {252, 1, 32, 1, T2NT}, // 2d2r_star
-    {122, 1, 64, 1, T2},   // 2d3r_star
-    {510, 1, 32, 2, T2NT}, // 2d1r_box
-    {124, 1, 64, 1, T2NT}, // 2d2r_box
-    kTemporal2None,        // 2d3r_box
-    {62, 14, 32, 2, T2},   // 3d1r_star
-    {28, 12, 64, 2, T2NT}, // 3d2r_star
+    {510, 1, 32, 4, T2 | RR},   // 2d1r_star
+    {508, 1, 16, 2, T2NT},      // 2d2r_star
+    {506, 1, 64, 1, T2},        // 2d3r_star
+    {510, 1, 32, 2, T2NT | RR}, // 2d1r_box
+    {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
+    kTemporal2None,             // 2d3r_box
+    {62, 14, 32, 2, T2},        // 3d1r_star
+    {28, 12, 64, 2, T2NT},      // 3d2r_star
     kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
 };
 static const uhs_config kBuiltinTemporal2F32[12] = {
-    {510, 1, 16, 2, T2NT}, // 2d1r_star
-    {252, 1, 32, 2, T2NT}, // 2d2r_star
-    {122, 1, 64, 1, T2NT}, // 2d3r_star
-    {254, 1, 16, 4, T2NT}, // 2d1r_box
-    {252, 1, 64, 1, T2},   // 2d2r_box
-    kTemporal2None,        // 2d3r_box
-    {62, 14, 32, 2, T2},   // 3d1r_star
+    {510, 1, 16, 2, T2NT | RR}, // 2d1r_star
+    {124, 1, 32, 2, T2NT | RR}, // 2d2r_star
+    {506, 1, 32, 2, T2},        // 2d3r_star
+    {254, 1, 16, 4, T2NT | RR}, // 2d1r_box
+    {252, 1, 64, 1, T2 | RR},   // 2d2r_box
+    kTemporal2None,             // 2d3r_box
+    {62, 14, 32, 2, T2},        // 3d1r_star
     kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
 };
 static const uhs_config kBuiltinTemporal2F32K[12] = {
-    {254, 1, 32, 2, T2NT}, // 2d1r_star
-    {124, 1, 64, 1, T2},   // 2d2r_star
-    {122, 1, 64, 1, T2NT}, // 2d3r_star
-    {510, 1, 32, 2, T2},   // 2d1r_box
-    {124, 1, 64, 1, T2NT}, // 2d2r_box
+    {254, 1, 32, 2, T2NT | RR}, // 2d1r_star
+    {252, 1, 32, 2, T2NT},      // 2d2r_star
+    {250, 1, 32, 2, T2},        // 2d3r_star
+    {510, 1, 32, 2, T2 | RR},   // 2d1r_box
+    {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
     kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
     kTemporal2None,
 };

@@ -1,12 +1,15 @@
 // win2d_tb2 kernel: two timesteps per launch for the 2D stencils (temporal
 // blocking with an R-column ghost zone), so each cell crosses HBM once per
 // two updates instead of once per update.  FP64, FP32 and FP32 compensated.
+// Two forms: the window form below (any 2D stencil; UHS_CFG_ROWRING) and the
+// ring form of win2d_tb2_star_tile (stars only).
 //
-// A workgroup owns output columns [x0, x0 + tile_x) of rows [ys, ye) and
-// covers the columns extended by R on each side (BX = tile_x + 2R, VX
-// columns per thread along the unit-stride axis).  It streams rows along
-// axis 0.  At step s it stages level-0 row r0 = ys - 2R + s (the input) in
-// LDS, and level-1 row r0 - R - 1 (one step, computed at step s - 1):
+// Window form.  A workgroup owns output columns [x0, x0 + tile_x) of rows
+// [ys, ye) and covers the columns extended by R on each side (BX = tile_x +
+// 2R, VX columns per thread along the unit-stride axis).  It streams rows
+// along axis 0.  At step s it stages level-0 row r0 = ys - 2R + s (the
+// input) in LDS, and level-1 row r0 - R - 1 (one step, computed at step
+// s - 1):
 //   -- one barrier --
 //   each thread moves the 2R+1 cells of row r0 around each of its columns
 //     into its level-0 window (a register ring of 2R+1 rows), and the 2R+1
@@ -19,7 +22,8 @@
 // barrier per row suffices.  The windows are rings indexed by the step's
 // phase, with the row loop unrolled by hand by 2R+1, so no register copies
 // shift them; a star's window cells off its centre row and column are dead
-// and take no registers.
+// and take no registers.  2R+1 rows of loads are in flight in a register
+// ring.
 //
 // Cells of a level outside the interior hold that level's halo: the halo of
 // the grid the one-step ping-pong holds that state in (h0 for the input, h1
@@ -197,7 +201,158 @@ __device__ __forceinline__ void win2d_tb2_tile(const T *__restrict__ in, T *__re
     });
 }
 
-template <typename T, class S, int VX, bool KAHAN>
+// The star form (every term on the centre row or the centre column): the
+// 3D two-step star tile without its y axis.  Each thread keeps its columns'
+// rows z1-R .. z1+R of level 0 in a register ring, loaded from memory two
+// iterations before their first use, and the rows of level 1 likewise;
+// only the centre rows go through LDS (level 0 at row z1, level 1 at row
+// z2 = z1-1-R), for the in-row neighbours.  Iteration i = z1 - (ys - R):
+//   stage both centre rows -- one barrier --
+//   level 2 at row z2 (own columns): column terms from the level-1 ring,
+//     row terms from LDS; stored;
+//   level 1 at row z1 (extended columns): column terms from the level-0
+//     ring, row terms from LDS; into the level-1 ring;
+//   load level-0 row z1+R+2 into the slot row z1-R held.
+// Registers per column: two rings of 2R+2 cells, instead of two windows.
+template <typename T, class S, int VX, bool KAHAN, bool FULL, class CF>
+__device__ __forceinline__ void win2d_tb2_star_tile(const T *__restrict__ in, T *__restrict__ out,
+                                                    const T *__restrict__ h0, const T *__restrict__ h1,
+                                                    const KParams<T, S::P> &p, int x0, int ys, int ye, T *lds,
+                                                    CF &cf)
+{
+    constexpr int R = S::R, P = S::P, M0 = 2 * R + 2;
+    const int tid = threadIdx.x, BXT = p.BXT;
+    const int BX = p.TX + 2 * R, LW = BX + 2 * R;
+    T *L0 = lds;              // 2 x LW: level-0 row z1, columns x0-2R ..
+    T *L1 = lds + 2 * LW + R; // 2 x BX: level-1 row z2, columns x0-R .. (R spare cells either side)
+    lds_poison(lds, 2 * LW + 2 * BX + 2 * R);
+    const int n0 = p.n0, n2 = p.n2;
+    const long long sy = p.sy;
+    const bool nts = p.flags & UHS_CFG_NT_STORE;
+
+    int xcl[VX];
+    bool xin[VX], core[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int ex = tid + v * BXT, xg = x0 - R + ex;
+        xin[v] = FULL || (xg >= 0 && xg < n2);
+        core[v] = ex >= R && ex < BX - R && (FULL || xg < n2);
+        xcl[v] = FULL ? xg : clampi(xg, -R, n2 + R - 1);
+    }
+    const bool hthr = tid < 2 * R;
+    const int hlx = tid < R ? tid : BX + tid;
+    const int hxg = x0 - 2 * R + hlx;
+    const bool hin = FULL || (hxg >= 0 && hxg < n2);
+    const int hcl = FULL && hthr ? hxg : clampi(hxg, -R, n2 + R - 1);
+    auto lev0 = [&](bool cin, int r, int c) -> T {
+        if constexpr (FULL) {
+            return in[(long long)min(r, n0 + R - 1) * sy + c];
+        } else {
+            const T *base = (cin && r >= 0 && r < n0) ? in : h0;
+            return base[(long long)clampi(r, -R, n0 + R - 1) * sy + c];
+        }
+    };
+    auto takes_halo1 = [&](int v, int r) { return !FULL && !(xin[v] && r >= 0 && r < n0); };
+    auto halo1 = [&](int v, int r) { return h1[(long long)clampi(r, -R, n0 + R - 1) * sy + xcl[v]]; };
+
+    const int z1s = ys - R;          // first level-1 row
+    const int N = (ye - ys) + 2 * R; // level-1 rows; iterations 0 .. N
+    // r0[v][(p - z1s + R) mod M0]: level-0 row p; r1[v][(p - z1s) mod M0]:
+    // level-1 row p; rh[(p - z1s) mod 2]: the ring cells of level-0 row p;
+    // rh1[v][(p - z1s) mod 2]: the level-1 halo value of row p.
+    T r0[VX][M0], r1[VX][M0], rh[2], rh1[VX][2];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+#pragma unroll
+        for (int j = 0; j < M0; j++) r1[v][j] = T(0);
+        rh1[v][0] = rh1[v][1] = T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < 2 * R; j++)
+#pragma unroll
+        for (int v = 0; v < VX; v++) r0[v][j] = lev0(xin[v], z1s - R + j, xcl[v]);
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+            if (takes_halo1(v, z1s + g)) rh1[v][g] = halo1(v, z1s + g);
+#pragma unroll
+        for (int v = 0; v < VX; v++) r0[v][2 * R + g] = lev0(xin[v], z1s + R + g, xcl[v]);
+        rh[g] = lev0(hin, z1s + g, hcl);
+    }
+
+    auto step = [&](auto phc, int i) {
+        constexpr int PH = decltype(phc)::value; // i mod M0 (M0 even: PH & 1 == i & 1)
+        const int z1 = z1s + i;
+        T *S0 = L0 + (PH & 1) * LW;
+        T *S1 = L1 + (PH & 1) * BX;
+        constexpr int L1S = PH - 1 - R + 2 * M0; // slot of level-1 row z2 = z1-1-R, plus 2*M0
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+            S0[R + tid + v * BXT] = r0[v][(PH + R) % M0];
+            S1[tid + v * BXT] = r1[v][L1S % M0];
+        }
+        if (hthr) S0[hlx] = rh[PH & 1];
+        __syncthreads();
+        cf.refresh();
+        const int z2 = z1 - 1 - R;
+        if (z2 >= ys) {
+            // Columns near the tile edges read the spare cells or the other
+            // buffer; their sums are not stored.
+            T *orow = out + (long long)z2 * sy;
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                Acc<T, KAHAN> a;
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int d0 = S::OFF[t][0], d1 = S::OFF[t][1];
+                    T val;
+                    if constexpr (d1 == 0) val = r1[v][(L1S + d0) % M0];
+                    else val = S1[tid + v * BXT + d1];
+                    if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                    else a.add(cf.template get<t>(), val);
+                });
+                if (core[v]) store_out(orow + xcl[v], a.value(), nts);
+            }
+        }
+        if (i < N) {
+            const bool zin = FULL || (z1 >= 0 && z1 < n0);
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                Acc<T, KAHAN> a;
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int d0 = S::OFF[t][0], d1 = S::OFF[t][1];
+                    T val;
+                    if constexpr (d1 == 0) val = r0[v][(PH + R + d0) % M0];
+                    else val = S0[R + tid + v * BXT + d1];
+                    if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                    else a.add(cf.template get<t>(), val);
+                });
+                r1[v][PH] = (FULL || (zin && xin[v])) ? a.value() : rh1[v][PH & 1];
+            }
+#pragma unroll
+            for (int v = 0; v < VX; v++)
+                if (takes_halo1(v, z1 + 2)) rh1[v][PH & 1] = halo1(v, z1 + 2);
+        }
+        // Row z1+R+2 into the slot of row z1-R (last read above), the ring
+        // cells of row z1+2 into the slot of row z1 (staged above).
+        // Addresses are clamped, so the loads past the slab end are harmless.
+#pragma unroll
+        for (int v = 0; v < VX; v++) r0[v][PH] = lev0(xin[v], z1 + R + 2, xcl[v]);
+        rh[PH & 1] = lev0(hin, z1 + 2, hcl);
+    };
+    int i0 = 0;
+    for (; i0 + M0 - 1 <= N; i0 += M0)
+        static_for<M0>([&](auto phc) { step(phc, i0 + decltype(phc)::value); });
+    static_for<M0>([&](auto phc) {
+        const int i = i0 + decltype(phc)::value;
+        if (i <= N) step(phc, i);
+    });
+}
+
+// WIN: the window form (any 2D stencil); otherwise the star ring form.
+template <typename T, class S, int VX, bool KAHAN, bool WIN>
 __global__ void __launch_bounds__(kMaxThreads) k_win2d_tb2(TB2Params<T, S::P> tp)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -209,13 +364,18 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d_tb2(TB2Params<T, S::P> tp
     const T *h1 = (tp.k0 & 1) ? tp.hA : tp.hB;
     Coefs<T, S::P> cf(p);
     T *lds = reinterpret_cast<T *>(smem_raw);
-    if (win2d_tb2_full<S::R>(x0, p.TX, ys, ye, p.n0, p.n2))
-        win2d_tb2_tile<T, S, VX, KAHAN, true>(p.in, p.out, h0, h1, p, x0, ys, ye, lds, cf);
-    else
-        win2d_tb2_tile<T, S, VX, KAHAN, false>(p.in, p.out, h0, h1, p, x0, ys, ye, lds, cf);
+    const bool full = win2d_tb2_full<S::R>(x0, p.TX, ys, ye, p.n0, p.n2);
+    if constexpr (WIN) {
+        if (full) win2d_tb2_tile<T, S, VX, KAHAN, true>(p.in, p.out, h0, h1, p, x0, ys, ye, lds, cf);
+        else win2d_tb2_tile<T, S, VX, KAHAN, false>(p.in, p.out, h0, h1, p, x0, ys, ye, lds, cf);
+    } else {
+        static_assert(S::STAR, "the ring form needs a star");
+        if (full) win2d_tb2_star_tile<T, S, VX, KAHAN, true>(p.in, p.out, h0, h1, p, x0, ys, ye, lds, cf);
+        else win2d_tb2_star_tile<T, S, VX, KAHAN, false>(p.in, p.out, h0, h1, p, x0, ys, ye, lds, cf);
+    }
 }
 
-template <typename T, class S, int VX, bool KAHAN>
+template <typename T, class S, int VX, bool KAHAN, bool WIN>
 hipError_t launch_win2d_tb2(const SweepArgs &a, hipStream_t st)
 {
     TB2Params<T, S::P> tp;
@@ -228,57 +388,63 @@ hipError_t launch_win2d_tb2(const SweepArgs &a, hipStream_t st)
     tp.hB = static_cast<const T *>(a.hB) + origin;
     tp.k0 = a.k0;
     const size_t lds = sizeof(T) * (2 * (size_t)(bx + 2 * R) + 2 * (size_t)bx + 2 * R);
-    hipLaunchKernelGGL((k_win2d_tb2<T, S, VX, KAHAN>), dim3(tp.k.nblocks), dim3(bx / VX), lds, st, tp);
+    hipLaunchKernelGGL((k_win2d_tb2<T, S, VX, KAHAN, WIN>), dim3(tp.k.nblocks), dim3(bx / VX), lds, st, tp);
     return hipGetLastError();
 }
 
-// Compiled where the two windows fit the registers without spilling: the
-// windows hold 2(2R+1)^2 cells per column (a star's dead cells excepted), so
-// FP64 radius 2 and 3 stars take one column per thread and the FP64 radius-3
-// box has no two-step form.
-template <typename T, class S> constexpr int tb2_max_vec()
+// Compiled where the registers hold the tile without spilling.  The ring
+// form (stars) keeps two rings of 2R+2 cells per column: 1, 2 or 4 columns
+// per thread.  The window form keeps two windows of (2R+1)^2 cells per
+// column (a star's dead cells excepted): 4 columns at radius 1; at radius 2
+// one (FP64) or two (FP32); at radius 3 one, and none for the FP64 box.
+template <typename T, class S, bool WIN> constexpr int tb2_max_vec()
 {
     constexpr bool f64 = sizeof(T) == 8;
-    if constexpr (S::R == 1) return 4;
+    if constexpr (!WIN) return S::STAR ? 4 : 0;
+    else if constexpr (S::R == 1) return 4;
     else if constexpr (S::R == 2) return f64 ? 1 : 2;
-    else return (f64 && S::P != 4 * S::R + 1) ? 0 : 1; // a star has 4R+1 terms
+    else return (f64 && !S::STAR) ? 0 : 1;
 }
-template <typename T, bool K, class S> LaunchFn pick_vec(int vec)
+template <typename T, bool K, class S, bool WIN> LaunchFn pick_vec(int vec)
 {
-    constexpr int M = tb2_max_vec<T, S>();
+    constexpr int M = tb2_max_vec<T, S, WIN>();
     switch (vec) {
     case 1:
-        if constexpr (M >= 1) return launch_win2d_tb2<T, S, 1, K>;
+        if constexpr (M >= 1) return launch_win2d_tb2<T, S, 1, K, WIN>;
         else return nullptr;
     case 2:
-        if constexpr (M >= 2) return launch_win2d_tb2<T, S, 2, K>;
+        if constexpr (M >= 2) return launch_win2d_tb2<T, S, 2, K, WIN>;
         else return nullptr;
     case 4:
-        if constexpr (M >= 4) return launch_win2d_tb2<T, S, 4, K>;
+        if constexpr (M >= 4) return launch_win2d_tb2<T, S, 4, K, WIN>;
         else return nullptr;
     }
     return nullptr;
 }
-template <class S> LaunchFn pick(int dtype, int vec)
+template <typename T, bool K, class S> LaunchFn pick_form(int vec, bool window)
+{
+    return window ? pick_vec<T, K, S, true>(vec) : pick_vec<T, K, S, false>(vec);
+}
+template <class S> LaunchFn pick(int dtype, int vec, bool window)
 {
     switch (dtype) {
-    case DT_F64: return pick_vec<double, false, S>(vec);
-    case DT_F32: return pick_vec<float, false, S>(vec);
-    case DT_F32K: return pick_vec<float, true, S>(vec);
+    case DT_F64: return pick_form<double, false, S>(vec, window);
+    case DT_F32: return pick_form<float, false, S>(vec, window);
+    case DT_F32K: return pick_form<float, true, S>(vec, window);
     }
     return nullptr;
 }
 } // namespace
 
-LaunchFn uhs_find_win2d_tb2(int prog, int dtype, int vec)
+LaunchFn uhs_find_win2d_tb2(int prog, int dtype, int vec, int window)
 {
     switch (prog) {
-    case Prog_2d1r_star::ID: return pick<Prog_2d1r_star>(dtype, vec);
-    case Prog_2d2r_star::ID: return pick<Prog_2d2r_star>(dtype, vec);
-    case Prog_2d3r_star::ID: return pick<Prog_2d3r_star>(dtype, vec);
-    case Prog_2d1r_box::ID: return pick<Prog_2d1r_box>(dtype, vec);
-    case Prog_2d2r_box::ID: return pick<Prog_2d2r_box>(dtype, vec);
-    case Prog_2d3r_box::ID: return pick<Prog_2d3r_box>(dtype, vec);
+    case Prog_2d1r_star::ID: return pick<Prog_2d1r_star>(dtype, vec, window);
+    case Prog_2d2r_star::ID: return pick<Prog_2d2r_star>(dtype, vec, window);
+    case Prog_2d3r_star::ID: return pick<Prog_2d3r_star>(dtype, vec, window);
+    case Prog_2d1r_box::ID: return pick<Prog_2d1r_box>(dtype, vec, window);
+    case Prog_2d2r_box::ID: return pick<Prog_2d2r_box>(dtype, vec, window);
+    case Prog_2d3r_box::ID: return pick<Prog_2d3r_box>(dtype, vec, window);
     }
     return nullptr;
 }

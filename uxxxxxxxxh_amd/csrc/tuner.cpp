@@ -174,7 +174,9 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
         for (int bx : {64, 128, 256, 512, 1024})
             for (int tz : {16, 32, 64, 128})
                 for (int v : {1, 2, 4})
-                    for (int fl : {UHS_CFG_TEMPORAL2, UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE}) {
+                    for (int fl : {UHS_CFG_TEMPORAL2, UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE,
+                                   UHS_CFG_TEMPORAL2 | UHS_CFG_ROWRING,
+                                   UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE | UHS_CFG_ROWRING}) {
                         uhs_config c{bx - 2 * R, 1, tz, v, fl};
                         if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) continue;
                         if (tz > 2 * pl.args.n0 && tz > 16) continue;

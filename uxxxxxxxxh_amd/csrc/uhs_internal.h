@@ -48,7 +48,9 @@ LaunchFn uhs_find_star3d_rt(int prog, int dtype, int rows);
 LaunchFn uhs_find_star3d_tb2(int prog, int dtype, int vec, int prefetch); // two steps per launch
 LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep, int planes);
 LaunchFn uhs_find_win2d(int prog, int dtype, int vec, int prefetch);
-LaunchFn uhs_find_win2d_tb2(int prog, int dtype, int vec); // two steps per launch
+// Two steps per launch, 2D: window = 1 the window form (any stencil), 0 the
+// ring form (stars).
+LaunchFn uhs_find_win2d_tb2(int prog, int dtype, int vec, int window);
 LaunchFn uhs_find_gather(int dtype);
 
 // Limits baked into the kernels; the host rejects configurations beyond them.

@@ -39,9 +39,12 @@ def test_costmodel_held_out_pair(gpu, tmp_path):
     assert set(ev) == {"3d3r_box_384x512x640", "2d3r_star_12288x6144"}
     for name, r in ev.items():
         log(name, r)
-        assert r["model_topk"]["slowdown_vs_exhaustive"] <= 0.10, name
-        assert r["model_topk"]["tuning_time_ratio"] >= 20.0, name
+        # The picks re-timed under one protocol (the search's own best is the
+        # minimum of many noisy timings and reads low), and the same timings
+        # of the exhaustive table for both.
+        assert r["retimed"]["model_topk_slowdown"] <= 0.10, name
         assert r["in_exhaustive_table"]["model_topk_slowdown"] <= 0.10, name
+        assert r["model_topk"]["tuning_time_ratio"] >= 20.0, name
 
 
 @pytest.mark.timeout(900)

@@ -82,7 +82,10 @@ typedef struct uhs_stencil {
  *           instead of one;
  *           UHS_CFG_PREFETCH4: 2D stencils keep four rows in flight;
  *           UHS_CFG_ROWRING: 2D stencils keep 2r+1 rows in flight in a
- *           register ring (not with UHS_CFG_PREFETCH2/4);
+ *           register ring (not with UHS_CFG_PREFETCH2/4); with
+ *           UHS_CFG_TEMPORAL2 it selects the window form of the 2D two-step
+ *           kernel (any 2D stencil), without it a 2D star takes the ring
+ *           form and a 2D box is rejected;
  *           UHS_CFG_ROWS2: 3D boxes compute two rows per thread (vec must
  *           be 1, tile_y even; tile_x * tile_y / 2 threads), halving the
  *           LDS reads per output;
@@ -92,12 +95,14 @@ typedef struct uhs_stencil {
  *           multiple of vec (tile_y / vec wavefronts per workgroup);
  *           UHS_CFG_PLANES2: 3D boxes keep two planes of tile loads in
  *           flight instead of one (not with UHS_CFG_PREFETCH2);
- *           UHS_CFG_TEMPORAL2: 3D stars and 2D stencils advance two
- *           timesteps per launch (temporal blocking, each cell read and
- *           written once per two updates): the workgroup covers the
- *           tile_x * tile_y output tile plus an r-wide ghost zone; 3D stars
- *           (tile_x + 2r) / vec * (tile_y + 2r) threads, vec 1 or 2 dividing
- *           tile_x + 2r; 2D stencils (tile_x + 2r) / vec threads, tile_y 1,
+ *           UHS_CFG_TEMPORAL2: 3D stars, the radius-1 3D box and 2D
+ *           stencils advance two timesteps per launch (temporal blocking,
+ *           each cell read and written once per two updates): the workgroup
+ *           covers the tile_x * tile_y output tile plus an r-wide ghost
+ *           zone; 3D stars (tile_x + 2r) / vec * (tile_y + 2r) threads, vec
+ *           1 or 2 dividing tile_x + 2r; the radius-1 box the same threads,
+ *           vec 1 or 2 (and 4 in FP32); 2D stencils (tile_x + 2r) / vec
+ *           threads, tile_y 1,
  *           vec 1, 2 or 4 dividing tile_x + 2r (FP64: vec 1 for radius 2
  *           and 3, no radius-3 box; FP32: vec <= 2 for radius 2, 1 for
  *           radius 3).  Up to three leading
@@ -125,9 +130,9 @@ typedef struct uhs_config {
 /* FP64 sweep, T timesteps.  ext[dims] = interior extents, halo >= the
  * stencil radius.  cfg NULL = the tuned (or default) configuration; the
  * FP64 defaults of the radius-1 and radius-2 3D stars (3d1r_star,
- * 3d2r_star) and of the 2D stencils but the radius-3 box advance two steps
- * per launch (UHS_CFG_TEMPORAL2), with results bitwise those of single
- * sweeps. */
+ * 3d2r_star), the radius-1 3D box (3d1r_box) and the 2D stencils but the
+ * radius-3 box advance two steps per launch (UHS_CFG_TEMPORAL2), with
+ * results bitwise those of single sweeps. */
 int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
                   const uhs_stencil *st, int timesteps, const uhs_config *cfg,
                   uhs_stream_t stream);
@@ -135,8 +140,8 @@ int uhs_sweep_f64(double *A, double *B, int dims, const int64_t *ext, int halo,
 /* FP32 storage and arithmetic.  compensated = 0: plain fused sum (the
  * reference's `float` DSL mode, Parser.h:86-91); compensated = 1:
  * Kahan-compensated sum of the terms.  cfg NULL = the tuned (or default)
- * configuration; the plain FP32 default of 3d1r_star and the FP32
- * defaults (plain and compensated) of the 2D stencils but the radius-3 box
+ * configuration; the plain FP32 defaults of 3d1r_star and 3d1r_box and the
+ * FP32 defaults (plain and compensated) of the 2D stencils but the radius-3 box
  * advance two steps per launch (UHS_CFG_TEMPORAL2), bitwise the same as
  * single sweeps. */
 int uhs_sweep_f32(float *A, float *B, int dims, const int64_t *ext, int halo,

@@ -88,8 +88,9 @@ TWO_STEP_2D = {
 @pytest.mark.parametrize("case", list(SHAPES))
 def test_one_step_configs(uhs, case):
     """Defaults that advance two steps per launch: the FP64 radius-1 and
-    radius-2 3D stars, the plain FP32 radius-1 3D star, and the 2D stencils
-    but the radius-3 box in every precision.  Their one-step configuration
+    radius-2 3D stars, the plain FP32 radius-1 3D star, the FP64 and plain
+    FP32 radius-1 3D box, and the 2D stencils but the radius-3 box in every
+    precision.  Their one-step configuration
     (leading steps, unbatched baseline) is the one-step default tile; every
     other default is its own one-step configuration."""
     s = SHAPES[case]
@@ -108,6 +109,9 @@ def test_one_step_configs(uhs, case):
                     assert o.tuple() == (128, 16, 256 if big else 8, 8, uhs.CFG_NT_STORE)
                 else:
                     assert o.tuple()[:2] == (256, 16) and o.flags == 0
+            elif case == "3d1r_box" and dt in (uhs.DT_F64, uhs.DT_F32):
+                assert d.tuple()[:2] == (30, 14) and d.vec == 2 and d.flags == uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE
+                assert not o.flags & uhs.CFG_TEMPORAL2 and o.flags & uhs.CFG_ROWS2
             elif case == "3d2r_star" and dt == uhs.DT_F64:
                 assert d.tuple() == (28, 12, 64 if big else 16, 2, uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE)
                 assert o.tuple() == (128, 16, 256 if big else 16, 8, uhs.CFG_NT_STORE)

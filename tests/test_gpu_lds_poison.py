@@ -3,7 +3,7 @@ span it uses with a NaN pattern before staging, so a read of a cell a kernel
 never wrote would turn into NaN in its output.  Under the mode, every kernel
 that stages through LDS must still give results bitwise equal to the
 oracle's fused mode: the one-step kernels of all 12 stencils (partial tiles,
-random halos), the two-step 3D star and 2D tiles (interior and edge paths), the box
+random halos), the two-step 3D star, 3D box and 2D tiles (interior and edge paths), the box
 tile forms, the FP32 forms, the batched work queue (register and LDS-ring
 star tiles, box tiles) and the fused-program streaming kernel."""
 import json
@@ -78,12 +78,13 @@ def check(u, orc, case, ext, T, cfg=None, dtype=np.float64, mode=FUSED, seed=5):
 @pytest.mark.parametrize("case", list(SHAPES))
 def test_default_kernels_under_poison(poisoned, orc, case):
     ext = [67, 301] if SHAPES[case]["dims"] == 2 else [19, 37, 70]
-    two_step = (case.startswith("3d") and "star" in case) or (case.startswith("2d") and case != "2d3r_box")
+    two_step = (case.startswith("3d") and "star" in case) or (case.startswith("2d") and case != "2d3r_box") or \
+        case == "3d1r_box"
     for T in ((1, 3, 4) if two_step else (1, 3)):
         check(poisoned, orc, case, ext, T)
 
 
-@pytest.mark.parametrize("case", ["3d1r_star", "3d2r_star", "3d3r_star"])
+@pytest.mark.parametrize("case", ["3d1r_star", "3d2r_star", "3d3r_star", "3d1r_box"])
 def test_two_step_tiles_under_poison(poisoned, orc, case):
     # Grids wide enough for interior tiles next to edge tiles.
     for cfg in ((10, 4, 5, 2, T2), (30, 6, 7, 1, T2 | NT), (14, 6, 3, 2, T2)):

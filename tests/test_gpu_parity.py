@@ -207,6 +207,40 @@ def test_two_steps_per_launch(gpu, orc, case, dtype):
                 assert np.array_equal(gA[mask], A0[mask]) and np.array_equal(gB[mask], B0[mask])
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32", "f32k"])
+def test_two_steps_per_launch_box3d(gpu, orc, dtype):
+    """UHS_CFG_TEMPORAL2 for the radius-1 3D box: running sums per pending
+    plane for both levels.  The result grid bitwise the oracle's for T =
+    1..6 and 9, halos untouched and every state's halo read from the grid
+    that holds it, on grids with interior and edge tiles and halos wider
+    than the radius."""
+    u = gpu
+    case = "3d1r_box"
+    s = SHAPES[case]
+    st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 16))
+    dt = np.float64 if dtype == "f64" else np.float32
+    T2, NT = u.CFG_TEMPORAL2, u.CFG_NT_STORE
+    for ext, h in (([19, 37, 70], 1), ([9, 14, 23], 2), ([40, 44, 70], 1)):
+        A0, B0 = random_grids(ext, h, 5)
+        A0, B0 = A0.astype(dt), B0.astype(dt)
+        mask = np.ones(A0.shape, bool)
+        mask[tuple(slice(h, n - h) for n in A0.shape)] = False
+        for cfg in ((10, 4, 5, 2, T2), (30, 6, 7, 1, T2 | NT), (62, 6, 64, 2, T2), (14, 6, 3, 2, T2),
+                    (30, 14, 9, 2, T2 | NT), (14, 6, 8, 4, T2)):
+            if cfg[3] == 4 and dtype == "f64":
+                with pytest.raises(u.UhsError):
+                    u.plan_kind(ext, h, st, u.DT_F64, cfg=cfg)
+                continue
+            for T in (1, 2, 3, 4, 5, 6, 9):
+                log(case, dtype, ext, h, cfg, "T", T)
+                gA, gB = run_gpu(u, st, A0, B0, ext, h, T, dt, dtype == "f32k", cfg=cfg)
+                A, B = A0.copy(), B0.copy()
+                orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, T, KAHAN if dtype == "f32k" else FUSED)
+                got, want = (gA, A) if T % 2 == 0 else (gB, B)
+                assert np.array_equal(got, want), (ext, h, cfg, T)
+                assert np.array_equal(gA[mask], A0[mask]) and np.array_equal(gB[mask], B0[mask])
+
+
 def two_step_forms_2d(case, dtype):
     """(form flag, columns per thread) the 2D two-step kernel is compiled
     for: the ring form (flag 0, stars only) at 1, 2 and 4 columns; the window

@@ -1,8 +1,8 @@
 """The benchmark workloads themselves, at their own sizes, through the C ABI.
 
 * The headline sweep (3d1r_star FP64 512^3, the library's default: two
-  steps per launch), the radius-2 star's and the 2D stencils' two-step
-  defaults (8192^2), bitwise equal to the oracle's fused mode at T = 4 (two
+  steps per launch), the radius-2 star's, the radius-1 box's and the 2D
+  stencils' two-step defaults (8192^2), bitwise equal to the oracle's fused mode at T = 4 (two
   two-step launches), and at T = 100 bitwise equal to one launch per step;
   the FP32 two-step defaults (plain and compensated) the same at T = 4.
 * All 12 stencils at 512^3 / 8192^2 and T = 4 with dyadic symmetric
@@ -42,7 +42,8 @@ def normalised(u, case, seed):
     return st.with_coeffs(u.make_coeffs(st, seed))
 
 
-TWO_STEP_F64 = ["3d1r_star", "3d2r_star", "2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box"]
+TWO_STEP_F64 = ["3d1r_star", "3d2r_star", "3d1r_box", "2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box",
+                "2d2r_box"]
 
 
 def full_ext(case):
@@ -94,12 +95,12 @@ def test_headline_T100_two_step_equals_one_step(gpu, case):
     assert np.array_equal(res[0], res[1])
 
 
-@pytest.mark.parametrize("case,compensated", [("3d1r_star", False)] +
+@pytest.mark.parametrize("case,compensated", [("3d1r_star", False), ("3d1r_box", False)] +
                          [(c, k) for c in TWO_STEP_F64 if c.startswith("2d") for k in (False, True)])
 def test_f32_two_step_default_vs_oracle_512(gpu, orc, case, compensated):
     """The FP32 defaults that advance two steps per launch: the plain
-    3d1r_star and the 2D stencils but the radius-3 box, plain and
-    compensated."""
+    3d1r_star and 3d1r_box, and the 2D stencils but the radius-3 box, plain
+    and compensated."""
     u = gpu
     st = normalised(u, case, 3)
     ext, h = full_ext(case), st.radius

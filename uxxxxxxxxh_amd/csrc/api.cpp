@@ -118,6 +118,7 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
     }
     case FAM_BOX3D:
     {
+        if (c.flags & UHS_CFG_TEMPORAL2) return uhs_find_box3d_tb2(pl.prog, dt, c.vec);
         const int planes = (c.flags & UHS_CFG_PLANES2) ? 2 : 1;
         if (c.flags & UHS_CFG_ROWS2) return c.vec == 1 ? uhs_find_box3d(pl.prog, dt, 0, 0, planes) : nullptr;
         return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0, planes);
@@ -159,8 +160,30 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return bad("two planes of tile loads in flight exist for 3D boxes without deeper cell prefetch");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
     if (c.flags & UHS_CFG_TEMPORAL2) {
+        if (pl.family == FAM_BOX3D) {
+            // Output tile plus an R-wide ghost zone: (tile_x + 2r) / vec *
+            // (tile_y + 2r) threads; the level-0 plane with its R-wide ring
+            // (corners included) and the level-1 plane double-buffered in LDS.
+            if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE))
+                return bad("two steps per launch combines with UHS_CFG_NT_STORE only");
+            const long long bx = c.tile_x + 2LL * R, by = c.tile_y + 2LL * R;
+            if (c.vec != 1 && c.vec != 2 && c.vec != 4)
+                return bad(str("two steps per launch needs vec 1, 2 or 4 (got %lld)", c.vec));
+            if (bx % c.vec) return bad(str("tile_x + 2r = %lld not a multiple of vec %lld", bx, c.vec));
+            const long long nt = bx / c.vec * by;
+            if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
+            const long long ring = 2LL * R * (bx + 2 * R) + 2LL * R * by;
+            if (ceil_div(ring, nt) > kBoxTb2RingCells)
+                return bad(str("ring of %lld cells needs more than %lld per thread", ring, kBoxTb2RingCells));
+            const size_t esz = a.dtype == DT_F64 ? 8 : 4;
+            const size_t lds = 2 * esz * (size_t)((bx + 2 * R) * (by + 2 * R) + bx * by);
+            if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
+            if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
+            if (!uhs_pick_fn(pl, c)) return bad("two steps per launch exists for the radius-1 3D box only");
+            return UHS_OK;
+        }
         if (pl.family != FAM_STAR3D && pl.family != FAM_WIN2D)
-            return bad("two steps per launch exists for 3D star and 2D stencils only");
+            return bad("two steps per launch exists for 3D stars, the radius-1 3D box and 2D stencils only");
         if (pl.family == FAM_WIN2D) {
             // Output columns tile_x plus an R-wide ghost zone each side, vec
             // columns per thread; one row per step.  UHS_CFG_ROWRING selects
@@ -292,13 +315,15 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 // one-step default.  The results are bitwise those of the one-step kernels.
 //   FP64: 3d1r_star 446 against 337 GSt/s one step (62x14 output tile, 64x16
 //   with the ghost zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327
-//   against 320 (28x12 output tile, 64-plane slabs); the 2D stencils 1.4-1.8x
+//   against 320 (28x12 output tile, 64-plane slabs); 3d1r_box 362-370
+//   against 310-322 (30x14 output tile); the 2D stencils 1.4-1.8x
 //   (2d1r_star 588 against 335, 2d2r_star 541 / 323, 2d3r_star 508 / 313,
 //   2d1r_box 602 / 335, 2d2r_box 431 / 311).  The radius-3 star stays one
 //   step per launch (209 against 293 GSt/s for its best two-step tile), and
 //   the 2D radius-3 box has no FP64 two-step kernel (its two windows do not
 //   fit the registers).
-//   FP32 plain: 3d1r_star 727 against 627; 2D 1.3-1.8x (2d1r_star 1080
+//   FP32 plain: 3d1r_star 727 against 627; 3d1r_box 545 against 452-483;
+//   2D 1.3-1.8x (2d1r_star 1080
 //   against 595, 2d2r_star 1029 / 619, 2d3r_star 909 / 601, 2d1r_box 1099 /
 //   605, 2d2r_box 818 / 559).
 //   FP32 compensated: 2D 1.1-1.7x (2d1r_star 1058 against 634, 2d2r_star 897
@@ -308,7 +333,9 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 // 4-17% faster at radius 3.
 // Slower with two steps per launch, so absent: the 3D radius-2 and radius-3
 // stars in FP32 (537 against 586, 327 against 440 GSt/s), the compensated
-// 3d1r_star (630 against 636) and the FP32 2D radius-3 box (299 against 424).
+// 3d1r_star (630 against 636) and 3d1r_box (346 against 361), and the FP32
+// 2D radius-3 box (299 against 424).  The 3D radius-2 and radius-3 boxes,
+// bound by FP64 FMAs, have no two-step kernel.
 static const uhs_config kTemporal2None = {0, 0, 0, 0, 0};
 constexpr int T2 = UHS_CFG_TEMPORAL2, T2NT = UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE, RR = UHS_CFG_ROWRING;
 static const uhs_config kBuiltinTemporal2F64[12] = {
@@ -320,7 +347,9 @@ static const uhs_config kBuiltinTemporal2F64[12] = {
     kTemporal2None,             // 2d3r_box
     {62, 14, 32, 2, T2},        // 3d1r_star
     {28, 12, 64, 2, T2NT},      // 3d2r_star
-    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+    kTemporal2None,             // 3d3r_star
+    {30, 14, 64, 2, T2NT},      // 3d1r_box
+    kTemporal2None, kTemporal2None,
 };
 static const uhs_config kBuiltinTemporal2F32[12] = {
     {510, 1, 16, 2, T2NT | RR}, // 2d1r_star
@@ -330,7 +359,9 @@ static const uhs_config kBuiltinTemporal2F32[12] = {
     {252, 1, 64, 1, T2 | RR},   // 2d2r_box
     kTemporal2None,             // 2d3r_box
     {62, 14, 32, 2, T2},        // 3d1r_star
-    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
+    kTemporal2None, kTemporal2None,
+    {30, 14, 32, 2, T2NT},      // 3d1r_box
+    kTemporal2None, kTemporal2None,
 };
 static const uhs_config kBuiltinTemporal2F32K[12] = {
     {254, 1, 32, 2, T2NT | RR}, // 2d1r_star

@@ -169,6 +169,20 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
                             out.push_back(c);
                         }
     }
+    if (pl.family == FAM_BOX3D && uhs_find_box3d_tb2(pl.prog, pl.args.dtype, 1)) { // two steps per launch
+        const int R = pl.radius;
+        for (int bx : {32, 64, 128})
+            for (int by : {4, 6, 8, 10, 12, 16})
+                for (int tz : {16, 32, 64, 128})
+                    for (int v : {1, 2, 4})
+                        for (int fl : {UHS_CFG_TEMPORAL2, UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE}) {
+                            if (by <= 2 * R) continue;
+                            uhs_config c{bx - 2 * R, by - 2 * R, tz, v, fl};
+                            if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) continue;
+                            if (tz > 2 * pl.args.n0 && tz > 16) continue;
+                            out.push_back(c);
+                        }
+    }
     if (pl.family == FAM_WIN2D) { // two steps per launch: output columns + ghost zone of 64 .. 1024 columns
         const int R = pl.radius;
         for (int bx : {64, 128, 256, 512, 1024})

@@ -47,6 +47,7 @@ LaunchFn uhs_find_star3d(int prog, int dtype, int vec, int prefetch, int ring_ce
 LaunchFn uhs_find_star3d_rt(int prog, int dtype, int rows);
 LaunchFn uhs_find_star3d_tb2(int prog, int dtype, int vec, int prefetch); // two steps per launch
 LaunchFn uhs_find_box3d(int prog, int dtype, int vec, int deep, int planes);
+LaunchFn uhs_find_box3d_tb2(int prog, int dtype, int vec); // two steps per launch (radius-1 box)
 LaunchFn uhs_find_win2d(int prog, int dtype, int vec, int prefetch);
 // Two steps per launch, 2D: window = 1 the window form (any stencil), 0 the
 // ring form (stars).
@@ -61,6 +62,7 @@ constexpr int kStarHaloPerThread = 4; // star3d: halo cells staged per thread pe
 // radius 2 and 3 keep 4, which frees 16-32 VGPRs for more waves.
 constexpr int box_cells_per_thread(int radius) { return radius == 1 ? 8 : 4; }
 constexpr int kTb2RingCells = 2;       // star3d_tb2: ring cells staged per thread per plane
+constexpr int kBoxTb2RingCells = 2;    // box3d_tb2: level-0 ring cells staged per thread per plane
 constexpr int kGatherMaxPoints = 256; // gather: terms passed in kernel arguments
 constexpr size_t kMaxLdsBytes = 64 * 1024;
 

@@ -110,7 +110,8 @@ def test_one_step_configs(uhs, case):
                 else:
                     assert o.tuple()[:2] == (256, 16) and o.flags == 0
             elif case == "3d1r_box" and dt in (uhs.DT_F64, uhs.DT_F32):
-                assert d.tuple()[:2] == (30, 14) and d.vec == 2 and d.flags == uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE
+                assert d.tuple()[:2] == ((62, 14) if dt == uhs.DT_F64 else (30, 14)) and d.vec == 2
+                assert d.flags == uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE
                 assert not o.flags & uhs.CFG_TEMPORAL2 and o.flags & uhs.CFG_ROWS2
             elif case == "3d2r_star" and dt == uhs.DT_F64:
                 assert d.tuple() == (28, 12, 64 if big else 16, 2, uhs.CFG_TEMPORAL2 | uhs.CFG_NT_STORE)

@@ -315,14 +315,14 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 // one-step default.  The results are bitwise those of the one-step kernels.
 //   FP64: 3d1r_star 446 against 337 GSt/s one step (62x14 output tile, 64x16
 //   with the ghost zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327
-//   against 320 (28x12 output tile, 64-plane slabs); 3d1r_box 362-370
-//   against 310-322 (30x14 output tile); the 2D stencils 1.4-1.8x
+//   against 320 (28x12 output tile, 64-plane slabs); 3d1r_box 373-376
+//   against 310-322 (62x14 output tile); the 2D stencils 1.4-1.8x
 //   (2d1r_star 588 against 335, 2d2r_star 541 / 323, 2d3r_star 508 / 313,
 //   2d1r_box 602 / 335, 2d2r_box 431 / 311).  The radius-3 star stays one
 //   step per launch (209 against 293 GSt/s for its best two-step tile), and
 //   the 2D radius-3 box has no FP64 two-step kernel (its two windows do not
 //   fit the registers).
-//   FP32 plain: 3d1r_star 727 against 627; 3d1r_box 545 against 452-483;
+//   FP32 plain: 3d1r_star 727 against 627; 3d1r_box 564 against 451-483;
 //   2D 1.3-1.8x (2d1r_star 1080
 //   against 595, 2d2r_star 1029 / 619, 2d3r_star 909 / 601, 2d1r_box 1099 /
 //   605, 2d2r_box 818 / 559).
@@ -333,7 +333,7 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 // 4-17% faster at radius 3.
 // Slower with two steps per launch, so absent: the 3D radius-2 and radius-3
 // stars in FP32 (537 against 586, 327 against 440 GSt/s), the compensated
-// 3d1r_star (630 against 636) and 3d1r_box (346 against 361), and the FP32
+// 3d1r_star (630 against 636) and 3d1r_box (347 against 351-361), and the FP32
 // 2D radius-3 box (299 against 424).  The 3D radius-2 and radius-3 boxes,
 // bound by FP64 FMAs, have no two-step kernel.
 static const uhs_config kTemporal2None = {0, 0, 0, 0, 0};
@@ -348,7 +348,7 @@ static const uhs_config kBuiltinTemporal2F64[12] = {
     {62, 14, 32, 2, T2},        // 3d1r_star
     {28, 12, 64, 2, T2NT},      // 3d2r_star
     kTemporal2None,             // 3d3r_star
-    {30, 14, 64, 2, T2NT},      // 3d1r_box
+    {62, 14, 32, 2, T2NT},      // 3d1r_box
     kTemporal2None, kTemporal2None,
 };
 static const uhs_config kBuiltinTemporal2F32[12] = {

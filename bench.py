@@ -375,9 +375,10 @@ def programs(u, reps=3, names=None):
     return out
 
 
-def costmodel_leg(u, problems_per_shape=6, configs_per_problem=150, epochs=300, topk=5, seed=11):
+def costmodel_leg(u, problems_per_shape=9, configs_per_problem=150, epochs=300, topk=5, seed=11):
     """The learned cost model of the per-grid tiling: timings of random
-    problems of all 12 stencils (FP64, and FP32 plain and compensated),
+    problems of all 12 stencils (FP64, and FP32 plain and compensated; about
+    20k timed configurations at the defaults),
     trained on the GPU, then judged on the two held-out problems (3d3r_box FP64
     384x512x640, 2d3r_star FP64 12288x6144) against the exhaustive timed
     search, the hand-set default and random-k."""

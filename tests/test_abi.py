@@ -167,3 +167,39 @@ def test_no_gpu_needed_for_planning(uhs):
     st = uhs.builtin("2d3r_box")
     c = uhs.default_config([8192, 8192], 3, st)
     assert c.tile_y == 1 and c.tile_x % c.vec == 0
+
+
+def test_two_step_config_checks_host_only(uhs):
+    """The two-step forms' legality rules, decided on the host without a
+    GPU: which stencils and precisions have a two-step kernel, and the
+    limits on columns per thread, threads and flags."""
+    T2, NT, RR, PF2 = uhs.CFG_TEMPORAL2, uhs.CFG_NT_STORE, uhs.CFG_ROWRING, uhs.CFG_PREFETCH2
+    F64, F32 = uhs.DT_F64, uhs.DT_F32
+    s1, s3, b1, b3 = (uhs.builtin(c) for c in ("2d1r_star", "2d3r_star", "2d1r_box", "2d3r_box"))
+    e2, e3 = [64, 96], [40, 44, 70]
+    ok = [(e2, s1, F64, (254, 1, 32, 4, T2)),            # 2D star, ring form
+          (e2, s1, F64, (254, 1, 32, 4, T2 | RR)),       # 2D star, window form
+          (e2, s3, F64, (58, 1, 32, 4, T2)),
+          (e2, s3, F64, (58, 1, 32, 1, T2 | RR | NT)),
+          (e2, b1, F64, (62, 1, 32, 2, T2 | RR)),
+          (e2, b3, F32, (58, 1, 32, 1, T2 | RR)),        # FP32 radius-3 box, window form
+          (e3, uhs.builtin("3d1r_box"), F64, (62, 14, 32, 2, T2 | NT)),
+          (e3, uhs.builtin("3d1r_box"), F32, (14, 6, 8, 4, T2)),
+          (e3, uhs.builtin("3d1r_star"), F64, (62, 14, 32, 2, T2))]
+    for ext, st, dt, cfg in ok:
+        assert uhs.plan_kind(ext, st.radius, st, dt, cfg=cfg) in ("win2d", "box3d", "star3d"), cfg
+    bad = [(e2, s1, F64, (61, 1, 32, 2, T2)),            # tile_x + 2r odd with vec 2
+           (e2, s1, F64, (62, 2, 32, 2, T2)),            # tile_y 2 in 2D
+           (e2, s1, F64, (1022, 1, 32, 1, T2)),          # 1024 threads
+           (e2, s1, F64, (62, 1, 32, 2, T2 | PF2)),      # no deeper prefetch in 2D
+           (e2, s3, F64, (58, 1, 32, 2, T2 | RR)),       # FP64 radius-3 window: one column per thread
+           (e2, b1, F64, (62, 1, 32, 2, T2)),            # a 2D box has no ring form
+           (e2, b3, F64, (58, 1, 32, 1, T2 | RR)),       # no FP64 radius-3 box two-step kernel
+           (e3, uhs.builtin("3d1r_box"), F64, (14, 6, 8, 4, T2)),          # FP64 box: vec <= 2
+           (e3, uhs.builtin("3d1r_box"), F64, (62, 14, 32, 2, T2 | RR)),   # box flags: NT only
+           (e3, uhs.builtin("3d2r_box"), F64, (28, 12, 32, 1, T2)),        # radius-2 box: none
+           (e3, uhs.builtin("3d1r_box"), F64, (126, 14, 32, 2, T2))]       # 1024 threads
+    for ext, st, dt, cfg in bad:
+        with pytest.raises(uhs.UhsError) as e:
+            uhs.plan_kind(ext, st.radius, st, dt, cfg=cfg)
+        assert e.value.code == -3, cfg

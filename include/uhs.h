@@ -85,7 +85,8 @@ typedef struct uhs_stencil {
  *           register ring (not with UHS_CFG_PREFETCH2/4); with
  *           UHS_CFG_TEMPORAL2 it selects the window form of the 2D two-step
  *           kernel (any 2D stencil), without it a 2D star takes the ring
- *           form and a 2D box is rejected;
+ *           form and a 2D box the sum form (one or two columns per
+ *           thread);
  *           UHS_CFG_ROWS2: 3D boxes compute two rows per thread (vec must
  *           be 1, tile_y even; tile_x * tile_y / 2 threads), halving the
  *           LDS reads per output;

@@ -74,7 +74,7 @@ def test_plan_kinds_and_default_configs(uhs, case):
 _T2, _NT, _RR = 32, 1, 128
 TWO_STEP_2D = {
     0: {"2d1r_star": (510, 1, 32, 4, _T2 | _RR), "2d2r_star": (508, 1, 16, 2, _T2 | _NT),
-        "2d3r_star": (506, 1, 64, 1, _T2), "2d1r_box": (510, 1, 32, 2, _T2 | _NT | _RR),
+        "2d3r_star": (506, 1, 64, 1, _T2), "2d1r_box": (510, 1, 32, 2, _T2 | _NT),
         "2d2r_box": (124, 1, 64, 1, _T2 | _NT | _RR)},
     1: {"2d1r_star": (510, 1, 16, 2, _T2 | _NT | _RR), "2d2r_star": (124, 1, 32, 2, _T2 | _NT | _RR),
         "2d3r_star": (506, 1, 32, 2, _T2), "2d1r_box": (254, 1, 16, 4, _T2 | _NT | _RR),
@@ -183,6 +183,8 @@ def test_two_step_config_checks_host_only(uhs):
           (e2, s3, F64, (58, 1, 32, 1, T2 | RR | NT)),
           (e2, b1, F64, (62, 1, 32, 2, T2 | RR)),
           (e2, b3, F32, (58, 1, 32, 1, T2 | RR)),        # FP32 radius-3 box, window form
+          (e2, b1, F64, (62, 1, 32, 2, T2)),             # 2D box, sum form
+          (e2, b3, F64, (58, 1, 32, 1, T2 | NT)),        # FP64 radius-3 box, sum form
           (e3, uhs.builtin("3d1r_box"), F64, (62, 14, 32, 2, T2 | NT)),
           (e3, uhs.builtin("3d1r_box"), F32, (14, 6, 8, 4, T2)),
           (e3, uhs.builtin("3d1r_star"), F64, (62, 14, 32, 2, T2))]
@@ -193,8 +195,8 @@ def test_two_step_config_checks_host_only(uhs):
            (e2, s1, F64, (1022, 1, 32, 1, T2)),          # 1024 threads
            (e2, s1, F64, (62, 1, 32, 2, T2 | PF2)),      # no deeper prefetch in 2D
            (e2, s3, F64, (58, 1, 32, 2, T2 | RR)),       # FP64 radius-3 window: one column per thread
-           (e2, b1, F64, (62, 1, 32, 2, T2)),            # a 2D box has no ring form
-           (e2, b3, F64, (58, 1, 32, 1, T2 | RR)),       # no FP64 radius-3 box two-step kernel
+           (e2, b1, F64, (62, 1, 32, 4, T2)),            # sum form: 1 or 2 columns per thread
+           (e2, b3, F64, (58, 1, 32, 1, T2 | RR)),       # no FP64 radius-3 box window form
            (e3, uhs.builtin("3d1r_box"), F64, (14, 6, 8, 4, T2)),          # FP64 box: vec <= 2
            (e3, uhs.builtin("3d1r_box"), F64, (62, 14, 32, 2, T2 | RR)),   # box flags: NT only
            (e3, uhs.builtin("3d2r_box"), F64, (28, 12, 32, 1, T2)),        # radius-2 box: none

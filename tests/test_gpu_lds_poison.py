@@ -91,10 +91,16 @@ def test_two_step_tiles_under_poison(poisoned, orc, case):
         check(poisoned, orc, case, [40, 44, 70], 5, cfg)
 
 
-@pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box"])
+@pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box", "2d3r_box"])
 def test_two_step_tiles_2d_under_poison(poisoned, orc, case):
     R = SHAPES[case]["radius"]
-    for form in ((0, RR) if "star" in case else (RR,)):  # ring form (stars), window form
+    if case == "2d3r_box":  # the sum form only (no FP64 window form at radius 3)
+        for bx, tz, v, fl in ((64, 16, 1, T2), (16, 5, 1, T2 | NT), (128, 7, 2, T2)):
+            check(poisoned, orc, case, [150, 301], 5, (bx - 2 * R, 1, tz, v, fl))
+        return
+    # without UHS_CFG_ROWRING the ring form (stars) or the sum form (boxes);
+    # with it the window form
+    for form in (0, RR):
         for bx, tz, v, fl in ((64, 16, 1, T2), (16, 5, 1, T2 | NT), (128, 7, 2 if R == 1 or not form else 1, T2)):
             check(poisoned, orc, case, [150, 301], 5, (bx - 2 * R, 1, tz, v, fl | form))
 

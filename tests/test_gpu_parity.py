@@ -243,15 +243,15 @@ def test_two_steps_per_launch_box3d(gpu, orc, dtype):
 
 def two_step_forms_2d(case, dtype):
     """(form flag, columns per thread) the 2D two-step kernel is compiled
-    for: the ring form (flag 0, stars only) at 1, 2 and 4 columns; the window
-    form (UHS_CFG_ROWRING) where its two windows fit the registers."""
+    for: without UHS_CFG_ROWRING (flag 0) the ring form of the stars at 1, 2
+    and 4 columns and the sum form of the boxes at 1 and 2; the window form
+    (UHS_CFG_ROWRING) where its two windows fit the registers."""
     R = SHAPES[case]["radius"]
     star = "star" in case
     f64 = dtype == "f64"
     win = [1, 2, 4] if R == 1 else ([1] if f64 else [1, 2]) if R == 2 else ([] if f64 and not star else [1])
     out = [(128, v) for v in win]
-    if star:
-        out += [(0, v) for v in (1, 2, 4)]
+    out += [(0, v) for v in ((1, 2, 4) if star else (1, 2))]
     return out
 
 
@@ -304,13 +304,15 @@ def test_two_steps_per_launch_config_checks_2d(gpu):
     assert u.plan_kind(ext, 3, s3, cfg=(58, 1, 32, 1, T2 | RR)) == "win2d"
     assert u.plan_kind(ext, 3, s3, cfg=(58, 1, 32, 4, T2)) == "win2d"
     assert u.plan_kind(ext, 1, b1, cfg=(62, 1, 32, 2, T2 | RR)) == "win2d"
+    assert u.plan_kind(ext, 1, b1, cfg=(62, 1, 32, 2, T2)) == "win2d"        # sum form
+    assert u.plan_kind(ext, 3, u.builtin("2d3r_box"), cfg=(58, 1, 32, 1, T2)) == "win2d"
     for st, bad in ((s1, (61, 1, 32, 2, T2)),                  # tile_x + 2r odd with vec 2
                     (s1, (62, 2, 32, 2, T2)),                  # tile_y 2
                     (s1, (1022, 1, 32, 1, T2)),                # 1024 threads
                     (s1, (62, 1, 32, 2, T2 | u.CFG_PREFETCH2)),
                     (s3, (58, 1, 32, 2, T2 | RR)),             # FP64 radius-3 window: one column per thread
-                    (b1, (62, 1, 32, 2, T2)),                  # a box has no ring form
-                    (u.builtin("2d3r_box"), (58, 1, 32, 1, T2 | RR))):  # no FP64 radius-3 box kernel
+                    (b1, (62, 1, 32, 4, T2)),                  # sum form: 1 or 2 columns per thread
+                    (u.builtin("2d3r_box"), (58, 1, 32, 1, T2 | RR))):  # no FP64 radius-3 box window form
         with pytest.raises(u.UhsError) as e:
             u.plan_kind(ext, st.radius, st, cfg=bad)
         assert e.value.code == -3, bad

@@ -190,11 +190,10 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
             // the window form (2r+1 rows of loads in flight in a register
             // ring, a (2r+1)^2 window per column and level); without it a
             // star takes the ring form (2r+2 rows of each column in
-            // registers, loads two rows ahead).
+            // registers, loads two rows ahead) and a box the sum form (2r+1
+            // running sums per column and level).
             if (c.flags & ~(UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE | UHS_CFG_ROWRING))
                 return bad("2D two steps per launch combines with UHS_CFG_NT_STORE and UHS_CFG_ROWRING only");
-            if (!(c.flags & UHS_CFG_ROWRING) && !kBuiltins[pl.prog].star)
-                return bad("a 2D box advances two steps per launch in the window form (UHS_CFG_ROWRING)");
             const long long bx = c.tile_x + 2LL * R;
             if (c.tile_y != 1) return bad("2D stencils need tile_y = 1");
             if (c.vec != 1 && c.vec != 2 && c.vec != 4)
@@ -320,8 +319,10 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 //   (2d1r_star 588 against 335, 2d2r_star 541 / 323, 2d3r_star 508 / 313,
 //   2d1r_box 602 / 335, 2d2r_box 431 / 311).  The radius-3 star stays one
 //   step per launch (209 against 293 GSt/s for its best two-step tile), and
-//   the 2D radius-3 box has no FP64 two-step kernel (its two windows do not
-//   fit the registers).
+//   so does the 2D radius-3 box: its two windows do not fit the registers,
+//   and its sum form (170 VGPRs, 3 waves per SIMD) times the same as one
+//   step (289 against 286).
+//   2d1r_box takes the sum form (585 against 570 for the window form).
 //   FP32 plain: 3d1r_star 727 against 627; 3d1r_box 564 against 451-483;
 //   2D 1.3-1.8x (2d1r_star 1080
 //   against 595, 2d2r_star 1029 / 619, 2d3r_star 909 / 601, 2d1r_box 1099 /
@@ -330,7 +331,8 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 //   / 582, 2d3r_star 747 / 532, 2d1r_box 982 / 590, 2d2r_box 464 / 418).
 // 2D stars take the window form (UHS_CFG_ROWRING) or the ring form, whichever
 // timed faster; the two are within 1-2% at radius 1 and 2, the ring form is
-// 4-17% faster at radius 3.
+// 4-17% faster at radius 3.  2D boxes likewise take the window form or the
+// sum form (no UHS_CFG_ROWRING).
 // Slower with two steps per launch, so absent: the 3D radius-2 and radius-3
 // stars in FP32 (537 against 586, 327 against 440 GSt/s), the compensated
 // 3d1r_star (630 against 636) and 3d1r_box (347 against 351-361), and the FP32
@@ -342,7 +344,7 @@ static const uhs_config kBuiltinTemporal2F64[12] = {
     {510, 1, 32, 4, T2 | RR},   // 2d1r_star
     {508, 1, 16, 2, T2NT},      // 2d2r_star
     {506, 1, 64, 1, T2},        // 2d3r_star
-    {510, 1, 32, 2, T2NT | RR}, // 2d1r_box
+    {510, 1, 32, 2, T2NT},      // 2d1r_box (sum form)
     {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
     kTemporal2None,             // 2d3r_box
     {62, 14, 32, 2, T2},        // 3d1r_star

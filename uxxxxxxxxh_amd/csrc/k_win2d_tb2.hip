@@ -364,9 +364,9 @@ __device__ __forceinline__ void win2d_tb2_star_tile(const T *__restrict__ in, T 
 // the sum whose last row this was then takes its tail terms, in term order,
 // from the ring, and is complete.  Schedule, staging and halo rules are
 // those of the window form (steps, level-0 row r0 = ys - 2R + s, level-1 row
-// r0 - R - 1, one barrier per row); the output row y of a level started
-// when row y - R arrived, at a step of phase (y - R - ys + 2R) mod 2R+1, and
-// keeps that slot.
+// r0 - R - 1, one barrier per row); the sum of an output row sits in the
+// ring slot of the phase of the step at which its first row (offset -R)
+// arrived, and completes 2R steps later.
 template <class S> constexpr int sum_row_part()
 {
     int t = 1;

@@ -68,19 +68,19 @@ def test_plan_kinds_and_default_configs(uhs, case):
     uhs.default_config(small, s["radius"], st)
 
 
-# The 2D two-step defaults at 8192^2 (tile_x, tile_y, tile_z, vec, flags):
-# UHS_CFG_TEMPORAL2 (32), with UHS_CFG_NT_STORE (1) and, for the window form,
-# UHS_CFG_ROWRING (128).
-_T2, _NT, _RR = 32, 1, 128
+# The 2D multi-step defaults at 8192^2 (tile_x, tile_y, tile_z, vec, flags):
+# the stars UHS_CFG_TEMPORAL3 (256), the boxes UHS_CFG_TEMPORAL2 (32), with
+# UHS_CFG_NT_STORE (1) and, for the window form, UHS_CFG_ROWRING (128).
+_T2, _NT, _RR, _T3 = 32, 1, 128, 256
 TWO_STEP_2D = {
-    0: {"2d1r_star": (510, 1, 32, 4, _T2 | _RR), "2d2r_star": (508, 1, 16, 2, _T2 | _NT),
-        "2d3r_star": (506, 1, 64, 1, _T2), "2d1r_box": (510, 1, 32, 2, _T2 | _NT),
+    0: {"2d1r_star": (1020, 1, 16, 4, _T3 | _NT), "2d2r_star": (504, 1, 32, 2, _T3 | _NT),
+        "2d3r_star": (244, 1, 64, 2, _T3 | _NT), "2d1r_box": (510, 1, 32, 2, _T2 | _NT),
         "2d2r_box": (124, 1, 64, 1, _T2 | _NT | _RR)},
-    1: {"2d1r_star": (510, 1, 16, 2, _T2 | _NT | _RR), "2d2r_star": (124, 1, 32, 2, _T2 | _NT | _RR),
-        "2d3r_star": (506, 1, 32, 2, _T2), "2d1r_box": (254, 1, 16, 4, _T2 | _NT | _RR),
+    1: {"2d1r_star": (508, 1, 32, 2, _T3 | _NT), "2d2r_star": (248, 1, 32, 2, _T3),
+        "2d3r_star": (244, 1, 64, 2, _T3 | _NT), "2d1r_box": (254, 1, 16, 4, _T2 | _NT | _RR),
         "2d2r_box": (252, 1, 64, 1, _T2 | _RR)},
-    2: {"2d1r_star": (254, 1, 32, 2, _T2 | _NT | _RR), "2d2r_star": (252, 1, 32, 2, _T2 | _NT),
-        "2d3r_star": (250, 1, 32, 2, _T2), "2d1r_box": (510, 1, 32, 2, _T2 | _RR),
+    2: {"2d1r_star": (252, 1, 32, 2, _T3 | _NT), "2d2r_star": (248, 1, 64, 2, _T3 | _NT),
+        "2d3r_star": (500, 1, 64, 2, _T3), "2d1r_box": (510, 1, 32, 2, _T2 | _RR),
         "2d2r_box": (124, 1, 64, 1, _T2 | _NT | _RR)},
 }
 
@@ -89,8 +89,8 @@ TWO_STEP_2D = {
 def test_one_step_configs(uhs, case):
     """Defaults that advance two steps per launch: the FP64 radius-1 and
     radius-2 3D stars, the plain FP32 radius-1 3D star, the FP64 and plain
-    FP32 radius-1 3D box, and the 2D stencils but the radius-3 box in every
-    precision.  Their one-step configuration
+    FP32 radius-1 3D box, and the 2D boxes but the radius-3 box in every
+    precision; three steps per launch: the 2D stars in every precision.  Their one-step configuration
     (leading steps, unbatched baseline) is the one-step default tile; every
     other default is its own one-step configuration."""
     s = SHAPES[case]
@@ -100,7 +100,7 @@ def test_one_step_configs(uhs, case):
         for dt in (uhs.DT_F64, uhs.DT_F32, uhs.DT_F32K):
             d = uhs.default_config(ext, s["radius"], st, dt)
             o = uhs.one_step_config(ext, s["radius"], st, dt)
-            assert not o.flags & uhs.CFG_TEMPORAL2
+            assert not o.flags & (uhs.CFG_TEMPORAL2 | uhs.CFG_TEMPORAL3)
             assert uhs.plan_kind(ext, s["radius"], st, dt, cfg=o.tuple()) == uhs.plan_kind(ext, s["radius"], st, dt)
             big = ext[0] == 512
             if case == "3d1r_star" and dt in (uhs.DT_F64, uhs.DT_F32):
@@ -188,6 +188,9 @@ def test_two_step_config_checks_host_only(uhs):
           (e3, uhs.builtin("3d1r_box"), F64, (62, 14, 32, 2, T2 | NT)),
           (e3, uhs.builtin("3d1r_box"), F32, (14, 6, 8, 4, T2)),
           (e3, uhs.builtin("3d1r_star"), F64, (62, 14, 32, 2, T2))]
+    T3 = uhs.CFG_TEMPORAL3
+    ok += [(e2, s1, F64, (252, 1, 32, 2, T3)),                # three steps per launch, 2D stars
+           (e2, s3, F32, (52, 1, 32, 4, T3 | NT))]
     for ext, st, dt, cfg in ok:
         assert uhs.plan_kind(ext, st.radius, st, dt, cfg=cfg) in ("win2d", "box3d", "star3d"), cfg
     bad = [(e2, s1, F64, (61, 1, 32, 2, T2)),            # tile_x + 2r odd with vec 2
@@ -200,7 +203,11 @@ def test_two_step_config_checks_host_only(uhs):
            (e3, uhs.builtin("3d1r_box"), F64, (14, 6, 8, 4, T2)),          # FP64 box: vec <= 2
            (e3, uhs.builtin("3d1r_box"), F64, (62, 14, 32, 2, T2 | RR)),   # box flags: NT only
            (e3, uhs.builtin("3d2r_box"), F64, (28, 12, 32, 1, T2)),        # radius-2 box: none
-           (e3, uhs.builtin("3d1r_box"), F64, (126, 14, 32, 2, T2))]       # 1024 threads
+           (e3, uhs.builtin("3d1r_box"), F64, (126, 14, 32, 2, T2)),       # 1024 threads
+           (e2, s1, F64, (62, 1, 32, 2, T3 | T2)),       # three steps: not with TEMPORAL2
+           (e2, s1, F64, (60, 1, 32, 2, T3 | RR)),       # three steps: ring form only
+           (e2, b1, F64, (60, 1, 32, 2, T3)),            # three steps: stars only
+           (e3, uhs.builtin("3d1r_star"), F64, (60, 14, 32, 2, T3))]  # three steps: 2D only
     for ext, st, dt, cfg in bad:
         with pytest.raises(uhs.UhsError) as e:
             uhs.plan_kind(ext, st.radius, st, dt, cfg=cfg)

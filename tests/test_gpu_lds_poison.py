@@ -91,6 +91,16 @@ def test_two_step_tiles_under_poison(poisoned, orc, case):
         check(poisoned, orc, case, [40, 44, 70], 5, cfg)
 
 
+@pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star"])
+def test_three_step_tiles_2d_under_poison(poisoned, orc, case):
+    R = SHAPES[case]["radius"]
+    T3 = poisoned.CFG_TEMPORAL3
+    for bx, tz, v, fl in ((64, 16, 1, T3), (16, 5, 1, T3 | NT), (128, 7, 2, T3), (256, 9, 4, T3)):
+        if bx // v < 2 * R:
+            continue
+        check(poisoned, orc, case, [150, 301], 7, (bx - 4 * R, 1, tz, v, fl))
+
+
 @pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box", "2d3r_box"])
 def test_two_step_tiles_2d_under_poison(poisoned, orc, case):
     R = SHAPES[case]["radius"]

@@ -64,8 +64,25 @@ def test_fill_matches_oracle(gpu, orc):
             assert np.array_equal(got, orc.fill(ext, h, 99, dt))
 
 
+def last_launch_steps(u, flags, T):
+    """Steps the last launch of a T-step sweep advances: the other grid then
+    holds state T minus that.  Two steps per launch: T mod 2 single steps
+    (plus two when T / 2 is odd), then an even number of two-step launches;
+    three steps per launch: T mod 3 single steps, then T / 3 launches."""
+    if flags & u.CFG_TEMPORAL3:
+        return 3 if T >= 3 else 1
+    if flags & u.CFG_TEMPORAL2:
+        twos = T // 2 - (T // 2) % 2
+        return 2 if twos else 1
+    return 1
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_f64_parity_all_T(gpu, orc, case):
+    """The default configuration: the result grid bitwise the oracle's FUSED
+    mode and within 1e-12 of its PLAIN mode, for T = 1..4; the other grid
+    holds the state before the last launch (T-1 after a single step, T-2 or
+    T-3 after a multi-step launch)."""
     u = gpu
     s = SHAPES[case]
     st = u.builtin(case)
@@ -73,15 +90,35 @@ def test_f64_parity_all_T(gpu, orc, case):
     st = st.with_coeffs(c)
     ext, h = small_ext(s["dims"]), s["radius"]
     A0, B0 = random_grids(ext, h, 7)
-    log(case, "kernel", u.plan_kind(ext, h, st), "ext", ext)
-    for T in (1, 2, 3):
+    flags = u.default_config(ext, h, st).flags
+    inner = tuple(slice(h, h + e) for e in ext)
+    log(case, "kernel", u.plan_kind(ext, h, st), "ext", ext, "flags", flags)
+    for T in (1, 2, 3, 4):
         gA, gB = run_gpu(u, st, A0, B0, ext, h, T)
-        A, B = A0.copy(), B0.copy()
-        orc.iterate(A, B, ext, h, s["offsets"], c, T, FUSED)
-        assert np.array_equal(gA, A) and np.array_equal(gB, B), (case, T)
-        A, B = A0.copy(), B0.copy()
-        orc.iterate(A, B, ext, h, s["offsets"], c, T, PLAIN)
-        assert rel_err(gA, A) <= 1e-12 and rel_err(gB, B) <= 1e-12
+        got, other = (gA, gB) if T % 2 == 0 else (gB, gA)
+        Tp = T - last_launch_steps(u, flags, T)
+        for mode in (FUSED, PLAIN):
+            # states[k]: the oracle's state after k single sweeps
+            states = {0: None}
+            A, B = A0.copy(), B0.copy()
+            for k in range(1, T + 1):
+                if k % 2:
+                    orc.iterate(A, B, ext, h, s["offsets"], c, 1, mode)
+                else:
+                    orc.iterate(B, A, ext, h, s["offsets"], c, 1, mode)
+                states[k] = (B if k % 2 else A).copy()
+            want = states[T]
+            # The other grid keeps its own halo; its interior holds state Tp
+            # (its initial content before any launch wrote it).
+            other0 = B0 if T % 2 == 0 else A0
+            want_other = other0.copy()
+            if Tp > 0:
+                want_other[inner] = states[Tp][inner]
+            if mode == FUSED:
+                assert np.array_equal(got, want), (case, T)
+                assert np.array_equal(other, want_other), (case, T, Tp)
+            else:
+                assert rel_err(got, want) <= 1e-12 and rel_err(other, want_other) <= 1e-12
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -317,6 +354,59 @@ def test_two_steps_per_launch_config_checks_2d(gpu):
             u.plan_kind(ext, st.radius, st, cfg=bad)
         assert e.value.code == -3, bad
     assert u.plan_kind(ext, 3, u.builtin("2d3r_box"), u.DT_F32, cfg=(58, 1, 32, 1, T2 | RR)) == "win2d"
+
+
+@pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star"])
+@pytest.mark.parametrize("dtype", ["f64", "f32", "f32k"])
+def test_three_steps_per_launch_2d(gpu, orc, case, dtype):
+    """UHS_CFG_TEMPORAL3 for the 2D stars: the result grid is bitwise the
+    oracle's for T = 1..7 and 9 (0-2 leading single steps, 0-3 three-step
+    launches), halos untouched and every state's halo read from the grid
+    that holds it, with edge tiles in both axes, slabs of 3 to 64 rows, 1, 2
+    and 4 columns per thread and halos wider than the radius."""
+    u = gpu
+    s = SHAPES[case]
+    R = s["radius"]
+    st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 16))
+    dt = np.float64 if dtype == "f64" else np.float32
+    T3, NT = u.CFG_TEMPORAL3, u.CFG_NT_STORE
+    for ext, h in (([67, 301], R), ([23, 45], R + 1), ([150, 40], R)):
+        A0, B0 = random_grids(ext, h, 5)
+        A0, B0 = A0.astype(dt), B0.astype(dt)
+        mask = np.ones(A0.shape, bool)
+        mask[tuple(slice(h, n - h) for n in A0.shape)] = False
+        for bx, tz, v, fl in ((64, 16, 1, T3), (16, 3, 1, T3 | NT), (128, 5, 2, T3), (256, 64, 4, T3 | NT),
+                              (64, 7, 4, T3)):
+            if bx // v < 2 * R:  # the ring needs 2r staging threads
+                continue
+            cfg = (bx - 4 * R, 1, tz, v, fl)
+            for T in (1, 2, 3, 4, 5, 6, 7, 9):
+                log(case, dtype, ext, h, cfg, "T", T)
+                gA, gB = run_gpu(u, st, A0, B0, ext, h, T, dt, dtype == "f32k", cfg=cfg)
+                A, B = A0.copy(), B0.copy()
+                orc.iterate(A, B, ext, h, s["offsets"], st.coeffs, T, KAHAN if dtype == "f32k" else FUSED)
+                got, want = (gA, A) if T % 2 == 0 else (gB, B)
+                assert np.array_equal(got, want), (ext, h, cfg, T)
+                assert np.array_equal(gA[mask], A0[mask]) and np.array_equal(gB[mask], B0[mask])
+
+
+def test_three_steps_per_launch_config_checks(gpu):
+    u = gpu
+    T2, T3, RR, NT = u.CFG_TEMPORAL2, u.CFG_TEMPORAL3, u.CFG_ROWRING, u.CFG_NT_STORE
+    ext = [64, 96]
+    s1, s3 = u.builtin("2d1r_star"), u.builtin("2d3r_star")
+    assert u.plan_kind(ext, 1, s1, cfg=(252, 1, 32, 2, T3)) == "win2d"
+    assert u.plan_kind(ext, 3, s3, cfg=(52, 1, 32, 1, T3 | NT)) == "win2d"
+    for st, e, bad in ((s1, ext, (60, 1, 32, 8, T3)),                   # vec 8
+                       (s1, ext, (1020, 1, 32, 1, T3)),                 # 1024 threads
+                       (s1, ext, (62, 1, 32, 2, T3 | T2)),              # with TEMPORAL2
+                       (s1, ext, (60, 1, 32, 2, T3 | RR)),              # no window form
+                       (s1, ext, (61, 1, 32, 2, T3)),                   # tile_x + 4r odd with vec 2
+                       (u.builtin("2d1r_box"), ext, (60, 1, 32, 2, T3)),  # stars only
+                       (u.builtin("3d1r_star"), [40, 44, 70], (60, 14, 32, 2, T3))):  # 2D only
+        with pytest.raises(u.UhsError) as ex:
+            u.plan_kind(e, st.radius, st, cfg=bad)
+        assert ex.value.code == -3, bad
 
 
 def test_two_steps_per_launch_config_checks(gpu):

@@ -57,7 +57,7 @@ def test_headline_default_bitwise_vs_oracle_512(gpu, orc, case):
     ext, h = full_ext(case), st.radius
     cfg = u.default_config(ext, h, st)
     log(case, ext, "T=4 default", cfg)
-    assert cfg.flags & u.CFG_TEMPORAL2
+    assert cfg.flags & (u.CFG_TEMPORAL2 | u.CFG_TEMPORAL3)
     A0, B0 = orc.fill(ext, h, 1), orc.fill(ext, h, 2)
     dA, dB = u.DeviceArray.from_host(A0), u.DeviceArray.from_host(B0)
     try:
@@ -98,16 +98,16 @@ def test_headline_T100_two_step_equals_one_step(gpu, case):
 @pytest.mark.parametrize("case,compensated", [("3d1r_star", False), ("3d1r_box", False)] +
                          [(c, k) for c in TWO_STEP_F64 if c.startswith("2d") for k in (False, True)])
 def test_f32_two_step_default_vs_oracle_512(gpu, orc, case, compensated):
-    """The FP32 defaults that advance two steps per launch: the plain
-    3d1r_star and 3d1r_box, and the 2D stencils but the radius-3 box, plain
-    and compensated."""
+    """The FP32 defaults that advance two or three steps per launch: the
+    plain 3d1r_star and 3d1r_box, and the 2D stencils but the radius-3 box,
+    plain and compensated (three steps for the stars)."""
     u = gpu
     st = normalised(u, case, 3)
     ext, h = full_ext(case), st.radius
     dt = u.DT_F32K if compensated else u.DT_F32
     cfg = u.default_config(ext, h, st, dt)
     log(case, "f32k" if compensated else "f32", ext, "T=4", cfg)
-    assert cfg.flags & u.CFG_TEMPORAL2
+    assert cfg.flags & (u.CFG_TEMPORAL2 | u.CFG_TEMPORAL3)
     A0, B0 = orc.fill(ext, h, 5, np.float32), orc.fill(ext, h, 6, np.float32)
     dA, dB = u.DeviceArray.from_host(A0), u.DeviceArray.from_host(B0)
     try:

@@ -124,6 +124,7 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
         return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0, planes);
     }
     case FAM_WIN2D:
+        if (c.flags & UHS_CFG_TEMPORAL3) return uhs_find_win2d_tb3(pl.prog, dt, c.vec);
         if (c.flags & UHS_CFG_TEMPORAL2)
             return uhs_find_win2d_tb2(pl.prog, dt, c.vec, (c.flags & UHS_CFG_ROWRING) ? 1 : 0);
         return uhs_find_win2d(pl.prog, dt, c.vec,
@@ -150,7 +151,7 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     };
     if (pl.family == FAM_GATHER) return UHS_OK;
     if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4 | UHS_CFG_ROWS2 |
-                    UHS_CFG_TEMPORAL2 | UHS_CFG_PLANES2 | UHS_CFG_ROWRING))
+                    UHS_CFG_TEMPORAL2 | UHS_CFG_PLANES2 | UHS_CFG_ROWRING | UHS_CFG_TEMPORAL3))
         return bad("unknown config flags");
     if ((c.flags & UHS_CFG_ROWRING) &&
         (pl.family != FAM_WIN2D || (c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_PREFETCH4))))
@@ -159,6 +160,29 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         (pl.family != FAM_BOX3D || (c.flags & UHS_CFG_PREFETCH2)))
         return bad("two planes of tile loads in flight exist for 3D boxes without deeper cell prefetch");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
+    if (c.flags & UHS_CFG_TEMPORAL3) {
+        // Three steps per launch, 2D stars (ring form): output columns
+        // tile_x plus 2r on each side, vec columns per thread, one row per
+        // step.
+        if (pl.family != FAM_WIN2D || !kBuiltins[pl.prog].star)
+            return bad("three steps per launch exists for 2D stars only");
+        if (c.flags & ~(UHS_CFG_TEMPORAL3 | UHS_CFG_NT_STORE))
+            return bad("three steps per launch combines with UHS_CFG_NT_STORE only");
+        const long long bx = c.tile_x + 4LL * R;
+        if (c.tile_y != 1) return bad("2D stencils need tile_y = 1");
+        if (c.vec != 1 && c.vec != 2 && c.vec != 4)
+            return bad(str("three steps per launch needs vec 1, 2 or 4 (got %lld)", c.vec));
+        if (bx % c.vec) return bad(str("tile_x + 4r = %lld not a multiple of vec %lld", bx, c.vec));
+        const long long nt = bx / c.vec;
+        if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
+        if (nt < 2 * R) return bad(str("%lld threads cannot stage a radius-%lld ring", nt, R));
+        const size_t esz = a.dtype == DT_F64 ? 8 : 4;
+        const size_t lds = esz * (size_t)(2 * (bx + 2 * R) + 2 * (2 * bx + 2 * R));
+        if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
+        if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
+        if (!uhs_pick_fn(pl, c)) return bad(str("three-step 2D kernel not compiled for vec %lld", c.vec));
+        return UHS_OK;
+    }
     if (c.flags & UHS_CFG_TEMPORAL2) {
         if (pl.family == FAM_BOX3D) {
             // Output tile plus an R-wide ghost zone: (tile_x + 2r) / vec *
@@ -309,9 +333,14 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
     {32, 32, 128, 1, UHS_CFG_ROWS2 | UHS_CFG_NT_STORE},      // 3d3r_box
 };
 
-// Defaults that advance two steps per launch (UHS_CFG_TEMPORAL2), where a
-// timed sweep at 512^3 / 8192^2 (T = 20 to 100) found them faster than the
-// one-step default.  The results are bitwise those of the one-step kernels.
+// Defaults that advance two steps per launch (UHS_CFG_TEMPORAL2) or three
+// (UHS_CFG_TEMPORAL3, the 2D stars), where a timed sweep at 512^3 / 8192^2
+// (T = 20 to 100) found them faster than the one-step default.  The results
+// are bitwise those of the one-step kernels.
+//   Three steps per launch for the 2D stars (T = 60, medians of 3):
+//   FP64 2d1r_star 817 against 583 GSt/s for the two-step default,
+//   2d2r_star 740 / 542, 2d3r_star 674 / 509; FP32 1457 / 1071, 1298 / 1023,
+//   1063 / 897; FP32 compensated 1335 / 1025, 1019 / 891, 784 / 739.
 //   FP64: 3d1r_star 446 against 337 GSt/s one step (62x14 output tile, 64x16
 //   with the ghost zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327
 //   against 320 (28x12 output tile, 64-plane slabs); 3d1r_box 373-376
@@ -338,41 +367,42 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 // 3d1r_star (630 against 636) and 3d1r_box (347 against 351-361), and the FP32
 // 2D radius-3 box (299 against 424).  The 3D radius-2 and radius-3 boxes,
 // bound by FP64 FMAs, have no two-step kernel.
-static const uhs_config kTemporal2None = {0, 0, 0, 0, 0};
+static const uhs_config kMultiStepNone = {0, 0, 0, 0, 0};
 constexpr int T2 = UHS_CFG_TEMPORAL2, T2NT = UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE, RR = UHS_CFG_ROWRING;
-static const uhs_config kBuiltinTemporal2F64[12] = {
-    {510, 1, 32, 4, T2 | RR},   // 2d1r_star
-    {508, 1, 16, 2, T2NT},      // 2d2r_star
-    {506, 1, 64, 1, T2},        // 2d3r_star
+constexpr int T3 = UHS_CFG_TEMPORAL3, T3NT = UHS_CFG_TEMPORAL3 | UHS_CFG_NT_STORE;
+static const uhs_config kBuiltinMultiStepF64[12] = {
+    {1020, 1, 16, 4, T3NT},     // 2d1r_star
+    {504, 1, 32, 2, T3NT},      // 2d2r_star
+    {244, 1, 64, 2, T3NT},      // 2d3r_star
     {510, 1, 32, 2, T2NT},      // 2d1r_box (sum form)
     {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
-    kTemporal2None,             // 2d3r_box
+    kMultiStepNone,             // 2d3r_box
     {62, 14, 32, 2, T2},        // 3d1r_star
     {28, 12, 64, 2, T2NT},      // 3d2r_star
-    kTemporal2None,             // 3d3r_star
+    kMultiStepNone,             // 3d3r_star
     {62, 14, 32, 2, T2NT},      // 3d1r_box
-    kTemporal2None, kTemporal2None,
+    kMultiStepNone, kMultiStepNone,
 };
-static const uhs_config kBuiltinTemporal2F32[12] = {
-    {510, 1, 16, 2, T2NT | RR}, // 2d1r_star
-    {124, 1, 32, 2, T2NT | RR}, // 2d2r_star
-    {506, 1, 32, 2, T2},        // 2d3r_star
+static const uhs_config kBuiltinMultiStepF32[12] = {
+    {508, 1, 32, 2, T3NT},      // 2d1r_star
+    {248, 1, 32, 2, T3},        // 2d2r_star
+    {244, 1, 64, 2, T3NT},      // 2d3r_star
     {254, 1, 16, 4, T2NT | RR}, // 2d1r_box
     {252, 1, 64, 1, T2 | RR},   // 2d2r_box
-    kTemporal2None,             // 2d3r_box
+    kMultiStepNone,             // 2d3r_box
     {62, 14, 32, 2, T2},        // 3d1r_star
-    kTemporal2None, kTemporal2None,
+    kMultiStepNone, kMultiStepNone,
     {30, 14, 32, 2, T2NT},      // 3d1r_box
-    kTemporal2None, kTemporal2None,
+    kMultiStepNone, kMultiStepNone,
 };
-static const uhs_config kBuiltinTemporal2F32K[12] = {
-    {254, 1, 32, 2, T2NT | RR}, // 2d1r_star
-    {252, 1, 32, 2, T2NT},      // 2d2r_star
-    {250, 1, 32, 2, T2},        // 2d3r_star
+static const uhs_config kBuiltinMultiStepF32K[12] = {
+    {252, 1, 32, 2, T3NT},      // 2d1r_star
+    {248, 1, 64, 2, T3NT},      // 2d2r_star
+    {500, 1, 64, 2, T3},        // 2d3r_star
     {510, 1, 32, 2, T2 | RR},   // 2d1r_box
     {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
-    kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None, kTemporal2None,
-    kTemporal2None,
+    kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone,
+    kMultiStepNone,
 };
 
 uhs_config uhs_heuristic_config(const Plan &pl, bool one_step)
@@ -382,9 +412,9 @@ uhs_config uhs_heuristic_config(const Plan &pl, bool one_step)
     if (pl.family == FAM_GATHER) return c;
     c = a.dtype == DT_F64 ? kBuiltinDefaults[pl.prog] : kBuiltinDefaultsF32[pl.prog];
     char why[128];
-    const uhs_config *two = a.dtype == DT_F64   ? &kBuiltinTemporal2F64[pl.prog]
-                            : a.dtype == DT_F32 ? &kBuiltinTemporal2F32[pl.prog]
-                                                : &kBuiltinTemporal2F32K[pl.prog];
+    const uhs_config *two = a.dtype == DT_F64   ? &kBuiltinMultiStepF64[pl.prog]
+                            : a.dtype == DT_F32 ? &kBuiltinMultiStepF32[pl.prog]
+                                                : &kBuiltinMultiStepF32K[pl.prog];
     if (!one_step && two->flags && uhs_check_config(pl, *two, why, sizeof why) == UHS_OK) c = *two;
     if (uhs_check_config(pl, c, why, sizeof why) != UHS_OK) {
         // e.g. an FP32 form whose vec variant is not compiled
@@ -481,7 +511,7 @@ int uhs_plan_apply(Plan &pl, const uhs_config &c)
     pl.fn = uhs_pick_fn(pl, c);
     pl.fn1 = nullptr;
     if (!pl.fn) return fail(UHS_E_UNSUPPORTED, "no kernel for this stencil and precision");
-    if (c.flags & UHS_CFG_TEMPORAL2) {
+    if (c.flags & (UHS_CFG_TEMPORAL2 | UHS_CFG_TEMPORAL3)) {
         Plan one = pl;
         const uhs_config c1 = uhs_heuristic_config(one, true);
         one.args.tx = c1.tile_x; one.args.ty = c1.tile_y; one.args.tz = c1.tile_z;
@@ -513,9 +543,19 @@ int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream)
     if (!A || !B) return fail(UHS_E_INVAL, "null grid pointer");
     if (A == B) return fail(UHS_E_INVAL, "A and B must be distinct grids");
     if (timesteps < 0) return fail(UHS_E_INVAL, "negative timestep count");
-    if (pl.cfg.flags & UHS_CFG_TEMPORAL2) {
+    if (pl.cfg.flags & (UHS_CFG_TEMPORAL2 | UHS_CFG_TEMPORAL3)) {
+        // UHS_CFG_TEMPORAL3: T mod 3 single steps, then T / 3 three-step
+        // launches.  Each launch advances an odd number of steps, so every
+        // state lands in the grid of its parity, as in the one-step
+        // ping-pong.
+        const int K = (pl.cfg.flags & UHS_CFG_TEMPORAL3) ? 3 : 2;
         int ones, twos;
-        temporal2_schedule(timesteps, &ones, &twos);
+        if (K == 3) {
+            ones = timesteps % 3;
+            twos = timesteps / 3;
+        } else {
+            temporal2_schedule(timesteps, &ones, &twos);
+        }
         for (int k = 1; k <= ones; k++) {
             pl.args1.in = (k & 1) ? A : B;
             pl.args1.out = (k & 1) ? B : A;
@@ -530,7 +570,7 @@ int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream)
             pl.args.out = (l & 1) ? A : B;
             pl.args.hA = A;
             pl.args.hB = B;
-            pl.args.k0 = ones + 2 * j;
+            pl.args.k0 = ones + K * j;
             hipError_t e = pl.fn(pl.args, stream);
             if (e != hipSuccess) return hip_fail(e, "sweep launch");
         }

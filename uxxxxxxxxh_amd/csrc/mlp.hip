@@ -274,10 +274,13 @@ int uhs_mlp_features(int dims, int radius, int star, int dtype, const int64_t *e
     else if (c->flags & UHS_CFG_ROWS2) threads = c->tile_x * c->tile_y / 2;
     else if (c->flags & UHS_CFG_TEMPORAL2)
         threads = (c->tile_x + 2 * radius) / v * (dims == 3 ? c->tile_y + 2 * radius : 1);
+    else if (c->flags & UHS_CFG_TEMPORAL3)
+        threads = (c->tile_x + 4 * radius) / v;
     f[11] = log2f((float)std::max(1, threads));
     f[12] = (float)c->vec;
     if (c->flags & UHS_CFG_TEMPORAL2) depth = std::max(depth, 2);
-    f[13 + (depth >= 4 ? 2 : depth >= 2 ? 1 : 0)] = 1.f;
+    if (c->flags & UHS_CFG_TEMPORAL3) depth = std::max(depth, 3);
+    f[13 + (depth >= 3 ? 2 : depth >= 2 ? 1 : 0)] = 1.f; // steps per launch: 1, 2, 3 or more
     // Terms of the stencil: a full star has 2 dims r + 1, a full box (2r+1)^dims.
     const int side = 2 * radius + 1;
     const int P = star ? 2 * dims * radius + 1 : (dims == 3 ? side * side * side : side * side);

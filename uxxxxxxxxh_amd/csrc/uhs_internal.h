@@ -52,6 +52,7 @@ LaunchFn uhs_find_win2d(int prog, int dtype, int vec, int prefetch);
 // Two steps per launch, 2D: window = 1 the window form (any stencil), 0 the
 // ring form (stars).
 LaunchFn uhs_find_win2d_tb2(int prog, int dtype, int vec, int window);
+LaunchFn uhs_find_win2d_tb3(int prog, int dtype, int vec); // three steps per launch (2D stars)
 LaunchFn uhs_find_gather(int dtype);
 
 // Limits baked into the kernels; the host rejects configurations beyond them.

@@ -112,11 +112,12 @@ typedef struct uhs_stencil {
  *           results are bitwise those of single sweeps.  The other grid's
  *           interior then holds state T-2, not T-1, when a two-step
  *           launch ran;
- *           UHS_CFG_TEMPORAL3: 2D stars advance three timesteps per launch
- *           (not with UHS_CFG_TEMPORAL2): (tile_x + 4r) / vec threads, tile_y
- *           1, vec 1, 2 or 4; T mod 3 leading single sweeps, then T / 3
- *           three-step launches; the other grid's interior then holds state
- *           T-3.
+ *           UHS_CFG_TEMPORALK: 2D stars advance K = tile_y timesteps per
+ *           launch, K = 3, 4, 5, 6 or 8 (not with UHS_CFG_TEMPORAL2):
+ *           (tile_x + 2(K-1)r) / vec threads, vec 1, 2 or 4; T mod K leading
+ *           single sweeps (K more when K is even and T / K odd, so that
+ *           even-K launches are even in number), then K-step launches; the
+ *           other grid's interior then holds state T-K.
  * Threads per workgroup = tile_x / vec * tile_y (64 * tile_y / vec with
  * UHS_CFG_REGTILE). */
 #define UHS_CFG_NT_STORE 1
@@ -127,7 +128,7 @@ typedef struct uhs_stencil {
 #define UHS_CFG_TEMPORAL2 32
 #define UHS_CFG_PLANES2 64
 #define UHS_CFG_ROWRING 128
-#define UHS_CFG_TEMPORAL3 256
+#define UHS_CFG_TEMPORALK 256
 typedef struct uhs_config {
     int tile_x, tile_y, tile_z, vec, flags;
 } uhs_config;

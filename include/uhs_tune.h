@@ -90,15 +90,14 @@ typedef struct uhs_mlp *uhs_mlp_t;
  * radius one-hot (3), star flag (1), log2 extents (3, axis 0 first, 0 for a
  * missing axis), log2 tile x/y/z (3), log2 threads per workgroup (1), points
  * per thread (1), temporal depth one-hot {1, 2, 3 or more} (3;
- * UHS_CFG_TEMPORAL2 counts as depth 2, UHS_CFG_TEMPORAL3 as 3, a batch
- * depth 4 as more), log2 of the stencil's term count (1; 2 dims r + 1 for a
+ * UHS_CFG_TEMPORAL2 counts as depth 2, UHS_CFG_TEMPORALK as K = tile_y), log2 of the stencil's term count (1; 2 dims r + 1 for a
  * star, (2r+1)^dims for a box), precision (1: 0 FP64, 1 FP32, 2 FP32
  * compensated), non-temporal store flag (1), kernel variant (1: 0 LDS-staged,
  * 1 deeper prefetch (two planes for stars and two rows for 2D, three cells
  * for boxes), 2 four rows of prefetch (2D), 3 register-tiled, 4 two rows per
  * thread, 5 two planes of box loads in flight, 6 both 4 and 5, 7 a register
  * ring of rows (2D)).  Threads per workgroup follow the variant (ghost zone
- * included for UHS_CFG_TEMPORAL2 and UHS_CFG_TEMPORAL3). */
+ * included for UHS_CFG_TEMPORAL2 and UHS_CFG_TEMPORALK). */
 int uhs_mlp_features(int dims, int radius, int star, int dtype, const int64_t *ext,
                      const uhs_config *cfg, int depth, float *out20);
 

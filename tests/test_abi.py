@@ -69,18 +69,18 @@ def test_plan_kinds_and_default_configs(uhs, case):
 
 
 # The 2D multi-step defaults at 8192^2 (tile_x, tile_y, tile_z, vec, flags):
-# the stars UHS_CFG_TEMPORAL3 (256), the boxes UHS_CFG_TEMPORAL2 (32), with
+# the stars UHS_CFG_TEMPORALK (256, K = tile_y), the boxes UHS_CFG_TEMPORAL2 (32), with
 # UHS_CFG_NT_STORE (1) and, for the window form, UHS_CFG_ROWRING (128).
 _T2, _NT, _RR, _T3 = 32, 1, 128, 256
 TWO_STEP_2D = {
-    0: {"2d1r_star": (1020, 1, 16, 4, _T3 | _NT), "2d2r_star": (504, 1, 32, 2, _T3 | _NT),
-        "2d3r_star": (244, 1, 64, 2, _T3 | _NT), "2d1r_box": (510, 1, 32, 2, _T2 | _NT),
+    0: {"2d1r_star": (242, 8, 64, 1, _T3 | _NT), "2d2r_star": (236, 6, 64, 1, _T3),
+        "2d3r_star": (238, 4, 64, 1, _T3), "2d1r_box": (510, 1, 32, 2, _T2 | _NT),
         "2d2r_box": (124, 1, 64, 1, _T2 | _NT | _RR)},
-    1: {"2d1r_star": (508, 1, 32, 2, _T3 | _NT), "2d2r_star": (248, 1, 32, 2, _T3),
-        "2d3r_star": (244, 1, 64, 2, _T3 | _NT), "2d1r_box": (254, 1, 16, 4, _T2 | _NT | _RR),
+    1: {"2d1r_star": (242, 8, 64, 1, _T3), "2d2r_star": (236, 6, 64, 1, _T3),
+        "2d3r_star": (488, 5, 64, 1, _T3), "2d1r_box": (254, 1, 16, 4, _T2 | _NT | _RR),
         "2d2r_box": (252, 1, 64, 1, _T2 | _RR)},
-    2: {"2d1r_star": (252, 1, 32, 2, _T3 | _NT), "2d2r_star": (248, 1, 64, 2, _T3 | _NT),
-        "2d3r_star": (500, 1, 64, 2, _T3), "2d1r_box": (510, 1, 32, 2, _T2 | _RR),
+    2: {"2d1r_star": (242, 8, 64, 1, _T3 | _NT), "2d2r_star": (236, 6, 64, 1, _T3),
+        "2d3r_star": (488, 5, 64, 1, _T3 | _NT), "2d1r_box": (510, 1, 32, 2, _T2 | _RR),
         "2d2r_box": (124, 1, 64, 1, _T2 | _NT | _RR)},
 }
 
@@ -100,7 +100,7 @@ def test_one_step_configs(uhs, case):
         for dt in (uhs.DT_F64, uhs.DT_F32, uhs.DT_F32K):
             d = uhs.default_config(ext, s["radius"], st, dt)
             o = uhs.one_step_config(ext, s["radius"], st, dt)
-            assert not o.flags & (uhs.CFG_TEMPORAL2 | uhs.CFG_TEMPORAL3)
+            assert not o.flags & (uhs.CFG_TEMPORAL2 | uhs.CFG_TEMPORALK)
             assert uhs.plan_kind(ext, s["radius"], st, dt, cfg=o.tuple()) == uhs.plan_kind(ext, s["radius"], st, dt)
             big = ext[0] == 512
             if case == "3d1r_star" and dt in (uhs.DT_F64, uhs.DT_F32):
@@ -118,7 +118,7 @@ def test_one_step_configs(uhs, case):
                 assert o.tuple() == (128, 16, 256 if big else 16, 8, uhs.CFG_NT_STORE)
             elif s["dims"] == 2 and case != "2d3r_box":
                 tb = TWO_STEP_2D[dt][case]
-                assert d.flags == tb[4] and (d.tile_x, d.tile_y, d.vec) == (tb[0], 1, tb[3])
+                assert d.flags == tb[4] and (d.tile_x, d.tile_y, d.vec) == (tb[0], tb[1], tb[3])
                 assert d.tile_z == tb[2] if ext[0] == 8192 else d.tile_z <= tb[2]
                 assert not o.flags & uhs.CFG_TEMPORAL2
             else:
@@ -188,9 +188,10 @@ def test_two_step_config_checks_host_only(uhs):
           (e3, uhs.builtin("3d1r_box"), F64, (62, 14, 32, 2, T2 | NT)),
           (e3, uhs.builtin("3d1r_box"), F32, (14, 6, 8, 4, T2)),
           (e3, uhs.builtin("3d1r_star"), F64, (62, 14, 32, 2, T2))]
-    T3 = uhs.CFG_TEMPORAL3
-    ok += [(e2, s1, F64, (252, 1, 32, 2, T3)),                # three steps per launch, 2D stars
-           (e2, s3, F32, (52, 1, 32, 4, T3 | NT))]
+    T3 = uhs.CFG_TEMPORALK
+    ok += [(e2, s1, F64, (252, 3, 32, 2, T3)),                # K = tile_y steps per launch, 2D stars
+           (e2, s1, F64, (242, 8, 32, 2, T3)),
+           (e2, s3, F32, (52, 3, 32, 4, T3 | NT))]
     for ext, st, dt, cfg in ok:
         assert uhs.plan_kind(ext, st.radius, st, dt, cfg=cfg) in ("win2d", "box3d", "star3d"), cfg
     bad = [(e2, s1, F64, (61, 1, 32, 2, T2)),            # tile_x + 2r odd with vec 2
@@ -204,10 +205,12 @@ def test_two_step_config_checks_host_only(uhs):
            (e3, uhs.builtin("3d1r_box"), F64, (62, 14, 32, 2, T2 | RR)),   # box flags: NT only
            (e3, uhs.builtin("3d2r_box"), F64, (28, 12, 32, 1, T2)),        # radius-2 box: none
            (e3, uhs.builtin("3d1r_box"), F64, (126, 14, 32, 2, T2)),       # 1024 threads
-           (e2, s1, F64, (62, 1, 32, 2, T3 | T2)),       # three steps: not with TEMPORAL2
-           (e2, s1, F64, (60, 1, 32, 2, T3 | RR)),       # three steps: ring form only
-           (e2, b1, F64, (60, 1, 32, 2, T3)),            # three steps: stars only
-           (e3, uhs.builtin("3d1r_star"), F64, (60, 14, 32, 2, T3))]  # three steps: 2D only
+           (e2, s1, F64, (62, 3, 32, 2, T3 | T2)),       # K steps: not with TEMPORAL2
+           (e2, s1, F64, (60, 3, 32, 2, T3 | RR)),       # K steps: ring form only
+           (e2, s1, F64, (62, 1, 32, 2, T3)),            # K steps: K = tile_y of 3, 4, 5, 6 or 8
+           (e2, s1, F64, (58, 7, 32, 2, T3)),
+           (e2, b1, F64, (60, 3, 32, 2, T3)),            # K steps: stars only
+           (e3, uhs.builtin("3d1r_star"), F64, (60, 3, 32, 2, T3))]  # K steps: 2D only
     for ext, st, dt, cfg in bad:
         with pytest.raises(uhs.UhsError) as e:
             uhs.plan_kind(ext, st.radius, st, dt, cfg=cfg)

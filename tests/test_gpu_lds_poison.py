@@ -92,13 +92,15 @@ def test_two_step_tiles_under_poison(poisoned, orc, case):
 
 
 @pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star"])
-def test_three_step_tiles_2d_under_poison(poisoned, orc, case):
+@pytest.mark.parametrize("K", [3, 4, 8])
+def test_k_step_tiles_2d_under_poison(poisoned, orc, case, K):
     R = SHAPES[case]["radius"]
-    T3 = poisoned.CFG_TEMPORAL3
-    for bx, tz, v, fl in ((64, 16, 1, T3), (16, 5, 1, T3 | NT), (128, 7, 2, T3), (256, 9, 4, T3)):
-        if bx // v < 2 * R:
+    TK = poisoned.CFG_TEMPORALK
+    for bx, tz, v, fl in ((64, 16, 1, TK), (16, 5, 1, TK | NT), (128, 7, 2, TK), (256, 9, 4, TK)):
+        tx = bx - 2 * (K - 1) * R
+        if tx < 1 or bx // v < 2 * R:
             continue
-        check(poisoned, orc, case, [150, 301], 7, (bx - 4 * R, 1, tz, v, fl))
+        check(poisoned, orc, case, [150, 301], 2 * K + 1, (tx, K, tz, v, fl))
 
 
 @pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box", "2d3r_box"])

@@ -64,17 +64,30 @@ def test_fill_matches_oracle(gpu, orc):
             assert np.array_equal(got, orc.fill(ext, h, 99, dt))
 
 
-def last_launch_steps(u, flags, T):
+def last_launch_steps(u, flags, T, K=2):
     """Steps the last launch of a T-step sweep advances: the other grid then
-    holds state T minus that.  Two steps per launch: T mod 2 single steps
-    (plus two when T / 2 is odd), then an even number of two-step launches;
-    three steps per launch: T mod 3 single steps, then T / 3 launches."""
-    if flags & u.CFG_TEMPORAL3:
-        return 3 if T >= 3 else 1
-    if flags & u.CFG_TEMPORAL2:
-        twos = T // 2 - (T // 2) % 2
-        return 2 if twos else 1
-    return 1
+    holds state T minus that.  With K steps per launch (UHS_CFG_TEMPORAL2:
+    K = 2; UHS_CFG_TEMPORALK: K = tile_y) the plan runs T / K K-step
+    launches, for even K an even number of them, after the leading steps:
+    launches of 5 and then 3 steps where they are shorter than K (the
+    defaults here have one column per thread, so both compile), then single
+    sweeps."""
+    if not flags & (u.CFG_TEMPORAL2 | u.CFG_TEMPORALK):
+        return 1
+    if not flags & u.CFG_TEMPORALK:
+        K = 2
+    launches, ones = T // K, T % K
+    if K % 2 == 0 and launches % 2:
+        launches -= 1
+        ones += K
+    seq = []
+    if flags & u.CFG_TEMPORALK:
+        for k in (5, 3):
+            while k < K and ones >= k:
+                seq.append(k)
+                ones -= k
+    seq += [1] * ones + [K] * launches
+    return seq[-1] if seq else 1
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -90,13 +103,14 @@ def test_f64_parity_all_T(gpu, orc, case):
     st = st.with_coeffs(c)
     ext, h = small_ext(s["dims"]), s["radius"]
     A0, B0 = random_grids(ext, h, 7)
-    flags = u.default_config(ext, h, st).flags
+    dcfg = u.default_config(ext, h, st)
+    flags = dcfg.flags
     inner = tuple(slice(h, h + e) for e in ext)
     log(case, "kernel", u.plan_kind(ext, h, st), "ext", ext, "flags", flags)
     for T in (1, 2, 3, 4):
         gA, gB = run_gpu(u, st, A0, B0, ext, h, T)
         got, other = (gA, gB) if T % 2 == 0 else (gB, gA)
-        Tp = T - last_launch_steps(u, flags, T)
+        Tp = T - last_launch_steps(u, flags, T, dcfg.tile_y)
         for mode in (FUSED, PLAIN):
             # states[k]: the oracle's state after k single sweeps
             states = {0: None}
@@ -358,29 +372,33 @@ def test_two_steps_per_launch_config_checks_2d(gpu):
 
 @pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star"])
 @pytest.mark.parametrize("dtype", ["f64", "f32", "f32k"])
-def test_three_steps_per_launch_2d(gpu, orc, case, dtype):
-    """UHS_CFG_TEMPORAL3 for the 2D stars: the result grid is bitwise the
-    oracle's for T = 1..7 and 9 (0-2 leading single steps, 0-3 three-step
-    launches), halos untouched and every state's halo read from the grid
-    that holds it, with edge tiles in both axes, slabs of 3 to 64 rows, 1, 2
-    and 4 columns per thread and halos wider than the radius."""
+@pytest.mark.parametrize("K", [3, 4, 5, 6, 8])
+def test_k_steps_per_launch_2d(gpu, orc, case, dtype, K):
+    """UHS_CFG_TEMPORALK for the 2D stars, K = tile_y steps per launch: the
+    result grid is bitwise the oracle's for T from 1 to 3K (leading single
+    steps, one and several K-step launches), halos untouched and every
+    state's halo read from the grid that holds it, with edge tiles in both
+    axes, slabs of 3 to 64 rows, 1, 2 and 4 columns per thread and halos
+    wider than the radius."""
     u = gpu
     s = SHAPES[case]
     R = s["radius"]
     st = u.builtin(case).with_coeffs(u.make_coeffs(u.builtin(case), 16))
     dt = np.float64 if dtype == "f64" else np.float32
-    T3, NT = u.CFG_TEMPORAL3, u.CFG_NT_STORE
+    TK, NT = u.CFG_TEMPORALK, u.CFG_NT_STORE
+    Ts = sorted({1, 2, K - 1, K, K + 1, 2 * K, 2 * K + 1, 3 * K})
     for ext, h in (([67, 301], R), ([23, 45], R + 1), ([150, 40], R)):
         A0, B0 = random_grids(ext, h, 5)
         A0, B0 = A0.astype(dt), B0.astype(dt)
         mask = np.ones(A0.shape, bool)
         mask[tuple(slice(h, n - h) for n in A0.shape)] = False
-        for bx, tz, v, fl in ((64, 16, 1, T3), (16, 3, 1, T3 | NT), (128, 5, 2, T3), (256, 64, 4, T3 | NT),
-                              (64, 7, 4, T3)):
-            if bx // v < 2 * R:  # the ring needs 2r staging threads
+        for bx, tz, v, fl in ((64, 16, 1, TK), (16, 3, 1, TK | NT), (128, 5, 2, TK), (256, 64, 4, TK | NT),
+                              (64, 7, 4, TK), (256, 9, 2, TK)):
+            tx = bx - 2 * (K - 1) * R
+            if tx < 1 or bx // v < 2 * R:  # the ring needs 2r staging threads
                 continue
-            cfg = (bx - 4 * R, 1, tz, v, fl)
-            for T in (1, 2, 3, 4, 5, 6, 7, 9):
+            cfg = (tx, K, tz, v, fl)
+            for T in Ts:
                 log(case, dtype, ext, h, cfg, "T", T)
                 gA, gB = run_gpu(u, st, A0, B0, ext, h, T, dt, dtype == "f32k", cfg=cfg)
                 A, B = A0.copy(), B0.copy()
@@ -390,20 +408,23 @@ def test_three_steps_per_launch_2d(gpu, orc, case, dtype):
                 assert np.array_equal(gA[mask], A0[mask]) and np.array_equal(gB[mask], B0[mask])
 
 
-def test_three_steps_per_launch_config_checks(gpu):
+def test_k_steps_per_launch_config_checks(gpu):
     u = gpu
-    T2, T3, RR, NT = u.CFG_TEMPORAL2, u.CFG_TEMPORAL3, u.CFG_ROWRING, u.CFG_NT_STORE
+    T2, TK, RR, NT = u.CFG_TEMPORAL2, u.CFG_TEMPORALK, u.CFG_ROWRING, u.CFG_NT_STORE
     ext = [64, 96]
     s1, s3 = u.builtin("2d1r_star"), u.builtin("2d3r_star")
-    assert u.plan_kind(ext, 1, s1, cfg=(252, 1, 32, 2, T3)) == "win2d"
-    assert u.plan_kind(ext, 3, s3, cfg=(52, 1, 32, 1, T3 | NT)) == "win2d"
-    for st, e, bad in ((s1, ext, (60, 1, 32, 8, T3)),                   # vec 8
-                       (s1, ext, (1020, 1, 32, 1, T3)),                 # 1024 threads
-                       (s1, ext, (62, 1, 32, 2, T3 | T2)),              # with TEMPORAL2
-                       (s1, ext, (60, 1, 32, 2, T3 | RR)),              # no window form
-                       (s1, ext, (61, 1, 32, 2, T3)),                   # tile_x + 4r odd with vec 2
-                       (u.builtin("2d1r_box"), ext, (60, 1, 32, 2, T3)),  # stars only
-                       (u.builtin("3d1r_star"), [40, 44, 70], (60, 14, 32, 2, T3))):  # 2D only
+    assert u.plan_kind(ext, 1, s1, cfg=(252, 3, 32, 2, TK)) == "win2d"
+    assert u.plan_kind(ext, 1, s1, cfg=(242, 8, 32, 2, TK)) == "win2d"
+    assert u.plan_kind(ext, 3, s3, cfg=(52, 3, 32, 1, TK | NT)) == "win2d"
+    for st, e, bad in ((s1, ext, (60, 3, 32, 8, TK)),                   # vec 8
+                       (s1, ext, (1020, 3, 32, 1, TK)),                 # 1024 threads
+                       (s1, ext, (62, 1, 32, 2, TK)),                   # K = 1
+                       (s1, ext, (58, 7, 32, 2, TK)),                   # K = 7 not compiled
+                       (s1, ext, (62, 3, 32, 2, TK | T2)),              # with TEMPORAL2
+                       (s1, ext, (60, 3, 32, 2, TK | RR)),              # no window form
+                       (s1, ext, (61, 3, 32, 2, TK)),                   # tile_x + 4r odd with vec 2
+                       (u.builtin("2d1r_box"), ext, (60, 3, 32, 2, TK)),  # stars only
+                       (u.builtin("3d1r_star"), [40, 44, 70], (60, 3, 32, 2, TK))):  # 2D only
         with pytest.raises(u.UhsError) as ex:
             u.plan_kind(e, st.radius, st, cfg=bad)
         assert ex.value.code == -3, bad

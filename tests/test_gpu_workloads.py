@@ -57,7 +57,7 @@ def test_headline_default_bitwise_vs_oracle_512(gpu, orc, case):
     ext, h = full_ext(case), st.radius
     cfg = u.default_config(ext, h, st)
     log(case, ext, "T=4 default", cfg)
-    assert cfg.flags & (u.CFG_TEMPORAL2 | u.CFG_TEMPORAL3)
+    assert cfg.flags & (u.CFG_TEMPORAL2 | u.CFG_TEMPORALK)
     A0, B0 = orc.fill(ext, h, 1), orc.fill(ext, h, 2)
     dA, dB = u.DeviceArray.from_host(A0), u.DeviceArray.from_host(B0)
     try:
@@ -107,7 +107,7 @@ def test_f32_two_step_default_vs_oracle_512(gpu, orc, case, compensated):
     dt = u.DT_F32K if compensated else u.DT_F32
     cfg = u.default_config(ext, h, st, dt)
     log(case, "f32k" if compensated else "f32", ext, "T=4", cfg)
-    assert cfg.flags & (u.CFG_TEMPORAL2 | u.CFG_TEMPORAL3)
+    assert cfg.flags & (u.CFG_TEMPORAL2 | u.CFG_TEMPORALK)
     A0, B0 = orc.fill(ext, h, 5, np.float32), orc.fill(ext, h, 6, np.float32)
     dA, dB = u.DeviceArray.from_host(A0), u.DeviceArray.from_host(B0)
     try:

@@ -20,7 +20,7 @@ ERRORS = {-1: "UHS_E_INVAL", -2: "UHS_E_UNSUPPORTED", -3: "UHS_E_CONFIG", -4: "U
 DT_F64, DT_F32, DT_F32K = 0, 1, 2
 CFG_NT_STORE, CFG_PREFETCH2, CFG_REGTILE, CFG_PREFETCH4, CFG_ROWS2, CFG_TEMPORAL2, CFG_PLANES2 = 1, 2, 4, 8, 16, 32, 64
 CFG_ROWRING = 128
-CFG_TEMPORAL3 = 256
+CFG_TEMPORALK = 256
 BATCH_BOX_ROWS2, BATCH_BOX_PLANES2, BATCH_SPLIT, BATCH_CONCURRENT = 1, 2, 4, 8
 
 

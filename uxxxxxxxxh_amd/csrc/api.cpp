@@ -124,7 +124,7 @@ LaunchFn uhs_pick_fn(const Plan &pl, const uhs_config &c)
         return uhs_find_box3d(pl.prog, dt, c.vec, (c.flags & UHS_CFG_PREFETCH2) ? 1 : 0, planes);
     }
     case FAM_WIN2D:
-        if (c.flags & UHS_CFG_TEMPORAL3) return uhs_find_win2d_tb3(pl.prog, dt, c.vec);
+        if (c.flags & UHS_CFG_TEMPORALK) return uhs_find_win2d_tbk(pl.prog, dt, c.vec, c.tile_y);
         if (c.flags & UHS_CFG_TEMPORAL2)
             return uhs_find_win2d_tb2(pl.prog, dt, c.vec, (c.flags & UHS_CFG_ROWRING) ? 1 : 0);
         return uhs_find_win2d(pl.prog, dt, c.vec,
@@ -151,7 +151,7 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
     };
     if (pl.family == FAM_GATHER) return UHS_OK;
     if (c.flags & ~(UHS_CFG_NT_STORE | UHS_CFG_PREFETCH2 | UHS_CFG_REGTILE | UHS_CFG_PREFETCH4 | UHS_CFG_ROWS2 |
-                    UHS_CFG_TEMPORAL2 | UHS_CFG_PLANES2 | UHS_CFG_ROWRING | UHS_CFG_TEMPORAL3))
+                    UHS_CFG_TEMPORAL2 | UHS_CFG_PLANES2 | UHS_CFG_ROWRING | UHS_CFG_TEMPORALK))
         return bad("unknown config flags");
     if ((c.flags & UHS_CFG_ROWRING) &&
         (pl.family != FAM_WIN2D || (c.flags & (UHS_CFG_PREFETCH2 | UHS_CFG_PREFETCH4))))
@@ -160,27 +160,29 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         (pl.family != FAM_BOX3D || (c.flags & UHS_CFG_PREFETCH2)))
         return bad("two planes of tile loads in flight exist for 3D boxes without deeper cell prefetch");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
-    if (c.flags & UHS_CFG_TEMPORAL3) {
-        // Three steps per launch, 2D stars (ring form): output columns
-        // tile_x plus 2r on each side, vec columns per thread, one row per
-        // step.
+    if (c.flags & UHS_CFG_TEMPORALK) {
+        // K = tile_y steps per launch, 2D stars (ring form): output columns
+        // tile_x plus (K-1)r on each side, vec columns per thread, one row
+        // per step.
+        const long long K = c.tile_y;
         if (pl.family != FAM_WIN2D || !kBuiltins[pl.prog].star)
-            return bad("three steps per launch exists for 2D stars only");
-        if (c.flags & ~(UHS_CFG_TEMPORAL3 | UHS_CFG_NT_STORE))
-            return bad("three steps per launch combines with UHS_CFG_NT_STORE only");
-        const long long bx = c.tile_x + 4LL * R;
-        if (c.tile_y != 1) return bad("2D stencils need tile_y = 1");
+            return bad("K steps per launch exist for 2D stars only");
+        if (c.flags & ~(UHS_CFG_TEMPORALK | UHS_CFG_NT_STORE))
+            return bad("K steps per launch combine with UHS_CFG_NT_STORE only");
+        if (K != 3 && K != 4 && K != 5 && K != 6 && K != 8)
+            return bad(str("K steps per launch: tile_y = K must be 3, 4, 5, 6 or 8 (got %lld)", K));
+        const long long bx = c.tile_x + 2 * (K - 1) * R;
         if (c.vec != 1 && c.vec != 2 && c.vec != 4)
-            return bad(str("three steps per launch needs vec 1, 2 or 4 (got %lld)", c.vec));
-        if (bx % c.vec) return bad(str("tile_x + 4r = %lld not a multiple of vec %lld", bx, c.vec));
+            return bad(str("%lld steps per launch need vec 1, 2 or 4 (got %lld)", K, c.vec));
+        if (bx % c.vec) return bad(str("extended tile of %lld columns not a multiple of vec %lld", bx, c.vec));
         const long long nt = bx / c.vec;
         if (nt > kMaxThreads) return bad(str("%lld threads per workgroup exceeds %lld", nt, kMaxThreads));
         if (nt < 2 * R) return bad(str("%lld threads cannot stage a radius-%lld ring", nt, R));
         const size_t esz = a.dtype == DT_F64 ? 8 : 4;
-        const size_t lds = esz * (size_t)(2 * (bx + 2 * R) + 2 * (2 * bx + 2 * R));
+        const size_t lds = esz * (size_t)(2 * (bx + 2 * R) + (K - 1) * (2 * bx + 2 * R));
         if (lds > kMaxLdsBytes) return bad(str("%lld bytes of LDS exceeds %lld", (long long)lds, (long long)kMaxLdsBytes));
         if (uhs_config_blocks(a, c, R) > INT_MAX) return bad("too many workgroups");
-        if (!uhs_pick_fn(pl, c)) return bad(str("three-step 2D kernel not compiled for vec %lld", c.vec));
+        if (!uhs_pick_fn(pl, c)) return bad(str("multi-step 2D kernel not compiled for vec %lld", c.vec));
         return UHS_OK;
     }
     if (c.flags & UHS_CFG_TEMPORAL2) {
@@ -333,14 +335,16 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
     {32, 32, 128, 1, UHS_CFG_ROWS2 | UHS_CFG_NT_STORE},      // 3d3r_box
 };
 
-// Defaults that advance two steps per launch (UHS_CFG_TEMPORAL2) or three
-// (UHS_CFG_TEMPORAL3, the 2D stars), where a timed sweep at 512^3 / 8192^2
+// Defaults that advance two steps per launch (UHS_CFG_TEMPORAL2) or K
+// (UHS_CFG_TEMPORALK, K = tile_y, the 2D stars), where a timed sweep at 512^3 / 8192^2
 // (T = 20 to 100) found them faster than the one-step default.  The results
 // are bitwise those of the one-step kernels.
-//   Three steps per launch for the 2D stars (T = 60, medians of 3):
-//   FP64 2d1r_star 817 against 583 GSt/s for the two-step default,
-//   2d2r_star 740 / 542, 2d3r_star 674 / 509; FP32 1457 / 1071, 1298 / 1023,
-//   1063 / 897; FP32 compensated 1335 / 1025, 1019 / 891, 784 / 739.
+//   K steps per launch for the 2D stars (T = 240, medians of 3, K of 3, 4,
+//   5, 6 and 8 and 108-120 tiles per star and precision): FP64 2d1r_star
+//   1318 GSt/s at K = 8 (583 for the two-step default), 2d2r_star 956 at
+//   K = 6 (542), 2d3r_star 696 at K = 4 (509); FP32 2143 (K = 8), 1542
+//   (6), 1177 (5) against 1071, 1023, 897; FP32 compensated 1811 (8), 1131
+//   (6), 791 (5) against 1025, 891, 739.
 //   FP64: 3d1r_star 446 against 337 GSt/s one step (62x14 output tile, 64x16
 //   with the ghost zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327
 //   against 320 (28x12 output tile, 64-plane slabs); 3d1r_box 373-376
@@ -369,11 +373,11 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 // bound by FP64 FMAs, have no two-step kernel.
 static const uhs_config kMultiStepNone = {0, 0, 0, 0, 0};
 constexpr int T2 = UHS_CFG_TEMPORAL2, T2NT = UHS_CFG_TEMPORAL2 | UHS_CFG_NT_STORE, RR = UHS_CFG_ROWRING;
-constexpr int T3 = UHS_CFG_TEMPORAL3, T3NT = UHS_CFG_TEMPORAL3 | UHS_CFG_NT_STORE;
+constexpr int TK = UHS_CFG_TEMPORALK, TKNT = UHS_CFG_TEMPORALK | UHS_CFG_NT_STORE;
 static const uhs_config kBuiltinMultiStepF64[12] = {
-    {1020, 1, 16, 4, T3NT},     // 2d1r_star
-    {504, 1, 32, 2, T3NT},      // 2d2r_star
-    {244, 1, 64, 2, T3NT},      // 2d3r_star
+    {242, 8, 64, 1, TKNT},      // 2d1r_star
+    {236, 6, 64, 1, TK},        // 2d2r_star
+    {238, 4, 64, 1, TK},        // 2d3r_star
     {510, 1, 32, 2, T2NT},      // 2d1r_box (sum form)
     {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
     kMultiStepNone,             // 2d3r_box
@@ -384,9 +388,9 @@ static const uhs_config kBuiltinMultiStepF64[12] = {
     kMultiStepNone, kMultiStepNone,
 };
 static const uhs_config kBuiltinMultiStepF32[12] = {
-    {508, 1, 32, 2, T3NT},      // 2d1r_star
-    {248, 1, 32, 2, T3},        // 2d2r_star
-    {244, 1, 64, 2, T3NT},      // 2d3r_star
+    {242, 8, 64, 1, TK},        // 2d1r_star
+    {236, 6, 64, 1, TK},        // 2d2r_star
+    {488, 5, 64, 1, TK},        // 2d3r_star
     {254, 1, 16, 4, T2NT | RR}, // 2d1r_box
     {252, 1, 64, 1, T2 | RR},   // 2d2r_box
     kMultiStepNone,             // 2d3r_box
@@ -396,9 +400,9 @@ static const uhs_config kBuiltinMultiStepF32[12] = {
     kMultiStepNone, kMultiStepNone,
 };
 static const uhs_config kBuiltinMultiStepF32K[12] = {
-    {252, 1, 32, 2, T3NT},      // 2d1r_star
-    {248, 1, 64, 2, T3NT},      // 2d2r_star
-    {500, 1, 64, 2, T3},        // 2d3r_star
+    {242, 8, 64, 1, TKNT},      // 2d1r_star
+    {236, 6, 64, 1, TK},        // 2d2r_star
+    {488, 5, 64, 1, TKNT},      // 2d3r_star
     {510, 1, 32, 2, T2 | RR},   // 2d1r_box
     {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
     kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone,
@@ -511,7 +515,7 @@ int uhs_plan_apply(Plan &pl, const uhs_config &c)
     pl.fn = uhs_pick_fn(pl, c);
     pl.fn1 = nullptr;
     if (!pl.fn) return fail(UHS_E_UNSUPPORTED, "no kernel for this stencil and precision");
-    if (c.flags & (UHS_CFG_TEMPORAL2 | UHS_CFG_TEMPORAL3)) {
+    if (c.flags & (UHS_CFG_TEMPORAL2 | UHS_CFG_TEMPORALK)) {
         Plan one = pl;
         const uhs_config c1 = uhs_heuristic_config(one, true);
         one.args.tx = c1.tile_x; one.args.ty = c1.tile_y; one.args.tz = c1.tile_z;
@@ -520,21 +524,41 @@ int uhs_plan_apply(Plan &pl, const uhs_config &c)
         pl.args1 = one.args;
         if (!pl.fn1) return fail(UHS_E_UNSUPPORTED, "no single-step kernel for the leading steps");
     }
+    pl.fnk[0] = pl.fnk[1] = nullptr;
+    pl.kk[0] = pl.kk[1] = 0;
+    if (c.flags & UHS_CFG_TEMPORALK) {
+        int n = 0;
+        for (int k : {5, 3}) {
+            if (k >= c.tile_y) continue;
+            const uhs_config ck{c.tile_x, k, c.tile_z, c.vec, c.flags};
+            char why[160];
+            Plan p2 = pl;
+            if (uhs_check_config(p2, ck, why, sizeof why) != UHS_OK) continue;
+            p2.args.ty = k;
+            pl.fnk[n] = uhs_pick_fn(p2, ck);
+            if (!pl.fnk[n]) continue;
+            pl.argsk[n] = p2.args;
+            pl.kk[n] = k;
+            n++;
+        }
+    }
     return UHS_OK;
 }
 
-// Steps advanced per launch of a UHS_CFG_TEMPORAL2 plan: `ones` single
-// steps, then `twos` two-step launches.  Every launch reads one grid and
-// writes the other, so the result lands where the one-step ping-pong leaves
-// it (A for even T, B for odd T) when the launch count has T's parity, i.e.
-// when `twos` is even.
-static void temporal2_schedule(int T, int *ones, int *twos)
+// Steps advanced per launch of a multi-step plan (UHS_CFG_TEMPORAL2, or
+// UHS_CFG_TEMPORALK with K = tile_y):
+// `ones` single steps, then `launches` K-step launches.  Every launch reads
+// one grid and writes the other, so the result lands where the one-step
+// ping-pong leaves it (A for even T, B for odd T) when each launch advances
+// an odd number of steps, or, for even K, when the K-step launches are even
+// in number.
+static void multi_step_schedule(int T, int K, int *ones, int *launches)
 {
-    *twos = T / 2;
-    *ones = T % 2;
-    if (*twos % 2) {
-        (*twos)--;
-        *ones += 2;
+    *launches = T / K;
+    *ones = T % K;
+    if (K % 2 == 0 && *launches % 2) {
+        (*launches)--;
+        *ones += K;
     }
 }
 
@@ -543,36 +567,50 @@ int uhs_run_plan(Plan &pl, void *A, void *B, int timesteps, hipStream_t stream)
     if (!A || !B) return fail(UHS_E_INVAL, "null grid pointer");
     if (A == B) return fail(UHS_E_INVAL, "A and B must be distinct grids");
     if (timesteps < 0) return fail(UHS_E_INVAL, "negative timestep count");
-    if (pl.cfg.flags & (UHS_CFG_TEMPORAL2 | UHS_CFG_TEMPORAL3)) {
-        // UHS_CFG_TEMPORAL3: T mod 3 single steps, then T / 3 three-step
-        // launches.  Each launch advances an odd number of steps, so every
-        // state lands in the grid of its parity, as in the one-step
-        // ping-pong.
-        const int K = (pl.cfg.flags & UHS_CFG_TEMPORAL3) ? 3 : 2;
+    if (pl.cfg.flags & (UHS_CFG_TEMPORAL2 | UHS_CFG_TEMPORALK)) {
+        // T mod K single steps, then K-step launches.  Launches of an odd
+        // number of steps leave every state in the grid of its parity, as in
+        // the one-step ping-pong; for even K the K-step launches are even in
+        // number.
+        const int K = (pl.cfg.flags & UHS_CFG_TEMPORALK) ? pl.cfg.tile_y : 2;
         int ones, twos;
-        if (K == 3) {
-            ones = timesteps % 3;
-            twos = timesteps / 3;
-        } else {
-            temporal2_schedule(timesteps, &ones, &twos);
-        }
-        for (int k = 1; k <= ones; k++) {
-            pl.args1.in = (k & 1) ? A : B;
-            pl.args1.out = (k & 1) ? B : A;
+        multi_step_schedule(timesteps, K, &ones, &twos);
+        // The leading steps: launches of an odd K' < K of the same tile
+        // (largest first), then single sweeps.  Odd launches and single
+        // sweeps keep every state in the grid of its parity, so they may
+        // precede the K-step launches in any mix.
+        int l = 0, k0 = 0; // launches so far, steps so far
+        auto multi = [&](LaunchFn fn, SweepArgs &a) {
+            a.in = (l & 1) ? B : A;
+            a.out = (l & 1) ? A : B;
+            a.hA = A;
+            a.hB = B;
+            a.k0 = k0;
+            return fn(a, stream);
+        };
+        for (int j = 0; j < 2; j++)
+            while (pl.fnk[j] && ones >= pl.kk[j]) {
+                hipError_t e = multi(pl.fnk[j], pl.argsk[j]);
+                if (e != hipSuccess) return hip_fail(e, "sweep launch");
+                ones -= pl.kk[j];
+                k0 += pl.kk[j];
+                l++;
+            }
+        for (; ones > 0; ones--) {
+            pl.args1.in = (l & 1) ? B : A;
+            pl.args1.out = (l & 1) ? A : B;
             hipError_t e = pl.fn1(pl.args1, stream);
             if (e != hipSuccess) return hip_fail(e, "sweep launch");
+            k0++;
+            l++;
         }
-        // Launch j reads the grid written by launch j-1; halo cells of each
+        // Launch l reads the grid written by launch l-1; halo cells of each
         // state come from the grid the one-step ping-pong would hold it in.
         for (int j = 0; j < twos; j++) {
-            const int l = ones + j; // launches before this one
-            pl.args.in = (l & 1) ? B : A;
-            pl.args.out = (l & 1) ? A : B;
-            pl.args.hA = A;
-            pl.args.hB = B;
-            pl.args.k0 = ones + K * j;
-            hipError_t e = pl.fn(pl.args, stream);
+            hipError_t e = multi(pl.fn, pl.args);
             if (e != hipSuccess) return hip_fail(e, "sweep launch");
+            k0 += K;
+            l++;
         }
         return UHS_OK;
     }

@@ -1,0 +1,4 @@
+// 4 timesteps per launch for the 2D stars (win2d_tbk with K = 4).
+#include "k_win2d_tbk.h"
+
+LaunchFn uhs_find_win2d_tb4(int prog, int dtype, int vec) { return win2d_tbk_find<4>(prog, dtype, vec); }

@@ -1,0 +1,4 @@
+// 6 timesteps per launch for the 2D stars (win2d_tbk with K = 6).
+#include "k_win2d_tbk.h"
+
+LaunchFn uhs_find_win2d_tb6(int prog, int dtype, int vec) { return win2d_tbk_find<6>(prog, dtype, vec); }

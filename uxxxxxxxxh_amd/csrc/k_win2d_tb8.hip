@@ -1,0 +1,4 @@
+// 8 timesteps per launch for the 2D stars (win2d_tbk with K = 8).
+#include "k_win2d_tbk.h"
+
+LaunchFn uhs_find_win2d_tb8(int prog, int dtype, int vec) { return win2d_tbk_find<8>(prog, dtype, vec); }

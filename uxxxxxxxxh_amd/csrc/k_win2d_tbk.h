@@ -1,0 +1,226 @@
+// win2d_tbk: K timesteps per launch for the 2D star stencils (temporal
+// blocking with a (K-1)R-column ghost zone), so each cell crosses HBM once
+// per K updates.  FP64, FP32 and FP32 compensated, 1, 2 or 4 columns per
+// thread.  Each k_win2d_tbK.hip instantiates one K (a translation unit per K
+// keeps the build parallel).
+#ifndef UHS_K_WIN2D_TBK_H
+#define UHS_K_WIN2D_TBK_H
+#include "sweep_kernels.h"
+
+namespace {
+
+// K steps per launch, ring form (2D stars; UHS_CFG_TEMPORALK with K =
+// tile_y).
+// The two-step ring form with K-1 computed levels: at iteration i level k
+// (1 <= k <= K) is computed at row z_k = z1 - (k-1)(R+1), z1 = z1s + i, from
+// the 2R+1 rows of level k-1 around it (its column terms, from level k-1's
+// register ring) and the centre row of level k-1 staged in LDS (its row
+// terms).  Every level keeps 2R+2 rows of each column in a register ring;
+// the row of level k computed at iteration i sits in slot i mod (2R+2), the
+// level-0 row p in slot (p - z1s + R) mod (2R+2), loaded two iterations
+// before its first use.  Level K is the output, stored for rows [ys, ye).
+// All levels share one barrier per row.  The workgroup covers tile_x output
+// columns plus (K-1)R on each side (the columns level 1 needs); level k is
+// only used (K-k)R columns into that margin, the rest of its columns are
+// computed and never read.  A level-k cell outside the interior takes the
+// halo of the grid that holds state k0 + k in the one-step ping-pong (hA for
+// even states, hB for odd).
+template <typename T, class S, int VX, bool KAHAN, int K, bool FULL, class CF>
+__device__ __forceinline__ void win2d_tbk_star_tile(const T *__restrict__ in, T *__restrict__ out,
+                                                    const T *__restrict__ hA, const T *__restrict__ hB, int k0,
+                                                    const KParams<T, S::P> &p, int x0, int ys, int ye, T *lds,
+                                                    CF &cf)
+{
+    constexpr int R = S::R, P = S::P, M0 = 2 * R + 2;
+    static_assert(K >= 2, "at least two steps");
+    const int tid = threadIdx.x, BXT = p.BXT;
+    const int BX = p.TX + 2 * (K - 1) * R, LW = BX + 2 * R;
+    // LDS: level 0 (2 x LW, columns x0-KR ..), then per level k = 1 .. K-1
+    // 2 x BX cells (columns x0-(K-1)R ..) with R spare cells either side.
+    T *L0 = lds;
+    auto Lk = [&](int k) { return lds + 2 * LW + (k - 1) * (2 * BX + 2 * R) + R; };
+    lds_poison(lds, 2 * LW + (K - 1) * (2 * BX + 2 * R));
+    const int n0 = p.n0, n2 = p.n2;
+    const long long sy = p.sy;
+    const bool nts = p.flags & UHS_CFG_NT_STORE;
+    const T *h0 = (k0 & 1) ? hB : hA;
+
+    // Column ex = tid + v*BXT is interior column x0 - (K-1)R + ex.
+    int xcl[VX];
+    bool xin[VX], core[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int ex = tid + v * BXT, xg = x0 - (K - 1) * R + ex;
+        xin[v] = FULL || (xg >= 0 && xg < n2);
+        core[v] = ex >= (K - 1) * R && ex < BX - (K - 1) * R && (FULL || xg < n2);
+        xcl[v] = FULL ? xg : clampi(xg, -R, n2 + R - 1);
+    }
+    const bool hthr = tid < 2 * R;
+    const int hlx = tid < R ? tid : BX + tid;
+    const int hxg = x0 - K * R + hlx;
+    const bool hin = FULL || (hxg >= 0 && hxg < n2);
+    const int hcl = FULL && hthr ? hxg : clampi(hxg, -R, n2 + R - 1);
+    auto lev0 = [&](bool cin, int r, int c) -> T {
+        if constexpr (FULL) {
+            return in[(long long)min(r, n0 + R - 1) * sy + c];
+        } else {
+            const T *base = (cin && r >= 0 && r < n0) ? in : h0;
+            return base[(long long)clampi(r, -R, n0 + R - 1) * sy + c];
+        }
+    };
+
+    const int z1s = ys - (K - 1) * R;                            // first level-1 row
+    const int NI = (ye - ys) - 1 + (K - 1) * (2 * R + 1);        // last iteration
+    // rr[0]: level-0 rows; rr[k]: level-k rows (k = 1 .. K-1).
+    T rr[K][VX][M0], rh[2];
+#pragma unroll
+    for (int k = 1; k < K; k++)
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+#pragma unroll
+            for (int j = 0; j < M0; j++) rr[k][v][j] = T(0);
+#pragma unroll
+    for (int j = 0; j < 2 * R; j++)
+#pragma unroll
+        for (int v = 0; v < VX; v++) rr[0][v][j] = lev0(xin[v], z1s - R + j, xcl[v]);
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+#pragma unroll
+        for (int v = 0; v < VX; v++) rr[0][v][2 * R + g] = lev0(xin[v], z1s + R + g, xcl[v]);
+        rh[g] = lev0(hin, z1s + g, hcl);
+    }
+
+    auto step = [&](auto phc, int i) {
+        constexpr int PH = decltype(phc)::value; // i mod M0 (M0 even: PH & 1 == i & 1)
+        constexpr int CS = (PH - 1 - R + 2 * M0) % M0; // slot of the centre row a level above reads
+        const int z1 = z1s + i;
+        T *S0 = L0 + (PH & 1) * LW;
+#pragma unroll
+        for (int v = 0; v < VX; v++) S0[R + tid + v * BXT] = rr[0][v][(PH + R) % M0];
+        if (hthr) S0[hlx] = rh[PH & 1];
+        static_for<K - 1>([&](auto kc) {
+            constexpr int k = decltype(kc)::value + 1;
+            T *Sk = Lk(k) + (PH & 1) * BX;
+#pragma unroll
+            for (int v = 0; v < VX; v++) Sk[tid + v * BXT] = rr[k][v][CS];
+        });
+        __syncthreads();
+        cf.refresh();
+        // Levels K .. 1.  Level k's input is level k-1: its row z_k + d0 is in
+        // slot (PH - R - 1 + d0) of ring k-1 (level 0: slot PH + R + d0), its
+        // centre row in LDS.
+        static_for<K>([&](auto kc) {
+            constexpr int k = K - decltype(kc)::value;
+            const int zk = z1 - (k - 1) * (R + 1);
+            const int first = ys - (K - k) * R; // first row of level k that is used
+            if (zk < first) return;
+            const T *Sin;
+            if constexpr (k == 1) Sin = S0 + R;
+            else Sin = Lk(k - 1) + (PH & 1) * BX;
+            T *orow = out + (long long)zk * sy;
+            const bool zin = FULL || (zk >= 0 && zk < n0);
+            const T *hk = ((k0 + k) & 1) ? hB : hA;
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                Acc<T, KAHAN> a;
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int d0 = S::OFF[t][0], d1 = S::OFF[t][1];
+                    T val;
+                    if constexpr (d1 == 0) {
+                        if constexpr (k == 1) val = rr[0][v][(PH + R + d0 + M0) % M0];
+                        else val = rr[k - 1][v][(PH - R - 1 + d0 + 2 * M0) % M0];
+                    } else {
+                        val = Sin[tid + v * BXT + d1];
+                    }
+                    if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                    else a.add(cf.template get<t>(), val);
+                });
+                if constexpr (k == K) {
+                    if (core[v]) store_out(orow + xcl[v], a.value(), nts);
+                } else {
+                    T val = a.value();
+                    if (!FULL && !(zin && xin[v])) val = hk[(long long)clampi(zk, -R, n0 + R - 1) * sy + xcl[v]];
+                    rr[k][v][PH] = val;
+                }
+            }
+        });
+        // Level-0 row z1+R+2 into the slot of row z1-R (last read above), the
+        // ring cells of row z1+2 into the slot of row z1 (staged above).
+#pragma unroll
+        for (int v = 0; v < VX; v++) rr[0][v][PH] = lev0(xin[v], z1 + R + 2, xcl[v]);
+        rh[PH & 1] = lev0(hin, z1 + 2, hcl);
+    };
+    int i0 = 0;
+    for (; i0 + M0 - 1 <= NI; i0 += M0)
+        static_for<M0>([&](auto phc) { step(phc, i0 + decltype(phc)::value); });
+    static_for<M0>([&](auto phc) {
+        const int i = i0 + decltype(phc)::value;
+        if (i <= NI) step(phc, i);
+    });
+}
+
+template <typename T, class S, int VX, bool KAHAN, int K>
+__global__ void __launch_bounds__(kMaxThreads) k_win2d_tbk(TB2Params<T, S::P> tp)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const KParams<T, S::P> &p = tp.k;
+    const int bid = xcd_remap(blockIdx.x, p.nblocks);
+    const int tile_x = bid % p.tiles_x, chunk = bid / p.tiles_x;
+    const int x0 = tile_x * p.TX, ys = chunk * p.ZC, ye = min(ys + p.ZC, p.n0);
+    Coefs<T, S::P> cf(p);
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    constexpr int KR = K * S::R;
+    const bool full = x0 >= KR && x0 + p.TX + KR <= p.n2 && ys >= KR && ye + KR <= p.n0;
+    if (full) win2d_tbk_star_tile<T, S, VX, KAHAN, K, true>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+    else win2d_tbk_star_tile<T, S, VX, KAHAN, K, false>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+}
+
+template <typename T, class S, int VX, bool KAHAN, int K>
+hipError_t launch_win2d_tbk(const SweepArgs &a, hipStream_t st)
+{
+    TB2Params<T, S::P> tp;
+    tp.k = make_params<T, S::P>(a);
+    constexpr int R = S::R;
+    const int bx = a.tx + 2 * (K - 1) * R;
+    tp.k.BXT = bx / VX;
+    const long long h = a.halo, origin = h * a.sy + h;
+    tp.hA = static_cast<const T *>(a.hA) + origin;
+    tp.hB = static_cast<const T *>(a.hB) + origin;
+    tp.k0 = a.k0;
+    const size_t lds = sizeof(T) * (2 * (size_t)(bx + 2 * R) + (K - 1) * (2 * (size_t)bx + 2 * R));
+    hipLaunchKernelGGL((k_win2d_tbk<T, S, VX, KAHAN, K>), dim3(tp.k.nblocks), dim3(bx / VX), lds, st, tp);
+    return hipGetLastError();
+}
+
+template <typename T, bool KAHAN, class S, int K> LaunchFn win2d_tbk_pick_vec(int vec)
+{
+    if constexpr (!S::STAR) return nullptr;
+    else switch (vec) {
+        case 1: return launch_win2d_tbk<T, S, 1, KAHAN, K>;
+        case 2: return launch_win2d_tbk<T, S, 2, KAHAN, K>;
+        case 4: return launch_win2d_tbk<T, S, 4, KAHAN, K>;
+        }
+    return nullptr;
+}
+template <class S, int K> LaunchFn win2d_tbk_pick(int dtype, int vec)
+{
+    switch (dtype) {
+    case DT_F64: return win2d_tbk_pick_vec<double, false, S, K>(vec);
+    case DT_F32: return win2d_tbk_pick_vec<float, false, S, K>(vec);
+    case DT_F32K: return win2d_tbk_pick_vec<float, true, S, K>(vec);
+    }
+    return nullptr;
+}
+template <int K> LaunchFn win2d_tbk_find(int prog, int dtype, int vec)
+{
+    switch (prog) {
+    case Prog_2d1r_star::ID: return win2d_tbk_pick<Prog_2d1r_star, K>(dtype, vec);
+    case Prog_2d2r_star::ID: return win2d_tbk_pick<Prog_2d2r_star, K>(dtype, vec);
+    case Prog_2d3r_star::ID: return win2d_tbk_pick<Prog_2d3r_star, K>(dtype, vec);
+    }
+    return nullptr;
+}
+
+} // namespace
+#endif

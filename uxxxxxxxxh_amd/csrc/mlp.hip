@@ -274,12 +274,12 @@ int uhs_mlp_features(int dims, int radius, int star, int dtype, const int64_t *e
     else if (c->flags & UHS_CFG_ROWS2) threads = c->tile_x * c->tile_y / 2;
     else if (c->flags & UHS_CFG_TEMPORAL2)
         threads = (c->tile_x + 2 * radius) / v * (dims == 3 ? c->tile_y + 2 * radius : 1);
-    else if (c->flags & UHS_CFG_TEMPORAL3)
-        threads = (c->tile_x + 4 * radius) / v;
+    else if (c->flags & UHS_CFG_TEMPORALK)
+        threads = (c->tile_x + 2 * (c->tile_y - 1) * radius) / v;
     f[11] = log2f((float)std::max(1, threads));
     f[12] = (float)c->vec;
     if (c->flags & UHS_CFG_TEMPORAL2) depth = std::max(depth, 2);
-    if (c->flags & UHS_CFG_TEMPORAL3) depth = std::max(depth, 3);
+    if (c->flags & UHS_CFG_TEMPORALK) depth = std::max(depth, c->tile_y);
     f[13 + (depth >= 3 ? 2 : depth >= 2 ? 1 : 0)] = 1.f; // steps per launch: 1, 2, 3 or more
     // Terms of the stencil: a full star has 2 dims r + 1, a full box (2r+1)^dims.
     const int side = 2 * radius + 1;

@@ -196,16 +196,18 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
                         if (tz > 2 * pl.args.n0 && tz > 16) continue;
                         out.push_back(c);
                     }
-        // three steps per launch (stars): output columns + 2r each side
-        for (int bx : {64, 128, 256, 512, 1024})
-            for (int tz : {16, 32, 64, 128})
-                for (int v : {1, 2, 4})
-                    for (int fl : {UHS_CFG_TEMPORAL3, UHS_CFG_TEMPORAL3 | UHS_CFG_NT_STORE}) {
-                        uhs_config c{bx - 4 * R, 1, tz, v, fl};
-                        if (c.tile_x < 1 || uhs_check_config(pl, c, why, sizeof why) != UHS_OK) continue;
-                        if (tz > 2 * pl.args.n0 && tz > 16) continue;
-                        out.push_back(c);
-                    }
+        // K = tile_y steps per launch (stars): output columns + (K-1)r each
+        // side
+        for (int K : {3, 4, 5, 6, 8})
+            for (int bx : {64, 128, 256, 512, 1024})
+                for (int tz : {16, 32, 64, 128})
+                    for (int v : {1, 2, 4})
+                        for (int fl : {UHS_CFG_TEMPORALK, UHS_CFG_TEMPORALK | UHS_CFG_NT_STORE}) {
+                            uhs_config c{bx - 2 * (K - 1) * R, K, tz, v, fl};
+                            if (c.tile_x < 1 || uhs_check_config(pl, c, why, sizeof why) != UHS_OK) continue;
+                            if (tz > 2 * pl.args.n0 && tz > 16) continue;
+                            out.push_back(c);
+                        }
     }
     if (pl.family == FAM_STAR3D) // register-tiled variants
         for (int rows : {2, 4, 8})
@@ -230,7 +232,7 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
 // Times `reps` sweeps of one configuration (after one warm-up sweep).  A
 // two-steps-per-launch configuration is timed over a multiple of 4 steps
 // (an even number of two-step launches, no leading single steps), a
-// three-steps-per-launch one over a multiple of 3.
+// K-steps-per-launch one over a multiple of K (of 2K for even K).
 int uhs_time_config(Plan &pl, void *A, void *B, const uhs_config &c, int reps, hipStream_t st,
                     double *ms)
 {
@@ -238,7 +240,11 @@ int uhs_time_config(Plan &pl, void *A, void *B, const uhs_config &c, int reps, h
     char why[160];
     if (uhs_check_config(p, c, why, sizeof why) != UHS_OK) return UHS_E_CONFIG;
     if (uhs_plan_apply(p, c) != UHS_OK) return UHS_E_CONFIG;
-    const int per = (c.flags & UHS_CFG_TEMPORAL3) ? 3 : (c.flags & UHS_CFG_TEMPORAL2) ? 4 : 1;
+    // K = tile_y steps per launch: 2K when K is even (an even number of
+    // launches), K when odd.
+    const int per = (c.flags & UHS_CFG_TEMPORALK)   ? (c.tile_y % 2 ? c.tile_y : 2 * c.tile_y)
+                    : (c.flags & UHS_CFG_TEMPORAL2) ? 4
+                                                    : 1;
     reps = (reps + per - 1) / per * per;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return UHS_E_HIP;

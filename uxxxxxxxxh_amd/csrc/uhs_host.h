@@ -14,9 +14,16 @@ struct Plan {
     uhs_config cfg;
     LaunchFn fn;
     SweepArgs args;
-    // UHS_CFG_TEMPORAL2: the single-step sweep that runs the leading steps.
+    // UHS_CFG_TEMPORAL2 / UHS_CFG_TEMPORALK: the single-step sweep that runs
+    // the leading steps.
     LaunchFn fn1;
     SweepArgs args1;
+    // UHS_CFG_TEMPORALK: the same tile with an odd K' < K steps per launch
+    // (K' = 5 and 3, where they pass the config check), which cover the
+    // leading steps in fewer launches.
+    LaunchFn fnk[2];
+    SweepArgs argsk[2];
+    int kk[2];
 };
 
 int uhs_set_error(int code, const char *msg);

@@ -52,7 +52,24 @@ LaunchFn uhs_find_win2d(int prog, int dtype, int vec, int prefetch);
 // Two steps per launch, 2D: window = 1 the window form (any stencil), 0 the
 // ring form (stars).
 LaunchFn uhs_find_win2d_tb2(int prog, int dtype, int vec, int window);
-LaunchFn uhs_find_win2d_tb3(int prog, int dtype, int vec); // three steps per launch (2D stars)
+// K steps per launch (2D stars), one function per compiled K; steps 3, 4, 5,
+// 6 or 8 (uhs_find_win2d_tbk dispatches).
+LaunchFn uhs_find_win2d_tb3(int prog, int dtype, int vec);
+LaunchFn uhs_find_win2d_tb4(int prog, int dtype, int vec);
+LaunchFn uhs_find_win2d_tb5(int prog, int dtype, int vec);
+LaunchFn uhs_find_win2d_tb6(int prog, int dtype, int vec);
+LaunchFn uhs_find_win2d_tb8(int prog, int dtype, int vec);
+inline LaunchFn uhs_find_win2d_tbk(int prog, int dtype, int vec, int steps)
+{
+    switch (steps) {
+    case 3: return uhs_find_win2d_tb3(prog, dtype, vec);
+    case 4: return uhs_find_win2d_tb4(prog, dtype, vec);
+    case 5: return uhs_find_win2d_tb5(prog, dtype, vec);
+    case 6: return uhs_find_win2d_tb6(prog, dtype, vec);
+    case 8: return uhs_find_win2d_tb8(prog, dtype, vec);
+    }
+    return nullptr;
+}
 LaunchFn uhs_find_gather(int dtype);
 
 // Limits baked into the kernels; the host rejects configurations beyond them.

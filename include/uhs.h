@@ -112,12 +112,14 @@ typedef struct uhs_stencil {
  *           results are bitwise those of single sweeps.  The other grid's
  *           interior then holds state T-2, not T-1, when a two-step
  *           launch ran;
- *           UHS_CFG_TEMPORALK: 2D stars advance K = tile_y timesteps per
- *           launch, K = 3, 4, 5, 6 or 8 (not with UHS_CFG_TEMPORAL2):
- *           (tile_x + 2(K-1)r) / vec threads, vec 1, 2 or 4; T mod K leading
- *           single sweeps (K more when K is even and T / K odd, so that
- *           even-K launches are even in number), then K-step launches; the
- *           other grid's interior then holds state T-K.
+ *           UHS_CFG_TEMPORALK: 2D stars and the radius-2 2D box advance K =
+ *           tile_y timesteps per launch, K = 3, 4, 5, 6 or 8 (not with
+ *           UHS_CFG_TEMPORAL2): (tile_x + 2(K-1)r) / vec threads, vec 1, 2 or
+ *           4 (the box 1 or 2, 1 in FP64);
+ *           the leading T mod K steps (K more when K is even and T / K odd,
+ *           so that even-K launches are even in number) run as 5- and 3-step
+ *           launches of the same tile and single sweeps, then the K-step
+ *           launches; the other grid's interior then holds state T-K.
  * Threads per workgroup = tile_x / vec * tile_y (64 * tile_y / vec with
  * UHS_CFG_REGTILE). */
 #define UHS_CFG_NT_STORE 1

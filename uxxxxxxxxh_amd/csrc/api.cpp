@@ -161,12 +161,13 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return bad("two planes of tile loads in flight exist for 3D boxes without deeper cell prefetch");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
     if (c.flags & UHS_CFG_TEMPORALK) {
-        // K = tile_y steps per launch, 2D stars (ring form): output columns
+        // K = tile_y steps per launch, 2D stars (ring form) and the radius-2
+        // 2D box, whose terms come row by row (sum form): output columns
         // tile_x plus (K-1)r on each side, vec columns per thread, one row
         // per step.
         const long long K = c.tile_y;
-        if (pl.family != FAM_WIN2D || !kBuiltins[pl.prog].star)
-            return bad("K steps per launch exist for 2D stars only");
+        if (pl.family != FAM_WIN2D || (!kBuiltins[pl.prog].star && R != 2))
+            return bad("K steps per launch exist for 2D stars and the radius-2 2D box only");
         if (c.flags & ~(UHS_CFG_TEMPORALK | UHS_CFG_NT_STORE))
             return bad("K steps per launch combine with UHS_CFG_NT_STORE only");
         if (K != 3 && K != 4 && K != 5 && K != 6 && K != 8)
@@ -344,7 +345,10 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 //   1318 GSt/s at K = 8 (583 for the two-step default), 2d2r_star 956 at
 //   K = 6 (542), 2d3r_star 696 at K = 4 (509); FP32 2143 (K = 8), 1542
 //   (6), 1177 (5) against 1071, 1023, 897; FP32 compensated 1811 (8), 1131
-//   (6), 791 (5) against 1025, 891, 739.
+//   (6), 791 (5) against 1025, 891, 739.  The radius-2 box (sum form, its
+//   25 terms row by row) in FP64: 604 at K = 4 against 510 for two steps;
+//   in FP32 (854 against 860) and compensated (453 against 496) it stays at
+//   two steps.
 //   FP64: 3d1r_star 446 against 337 GSt/s one step (62x14 output tile, 64x16
 //   with the ghost zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327
 //   against 320 (28x12 output tile, 64-plane slabs); 3d1r_box 373-376
@@ -379,7 +383,7 @@ static const uhs_config kBuiltinMultiStepF64[12] = {
     {236, 6, 64, 1, TK},        // 2d2r_star
     {238, 4, 64, 1, TK},        // 2d3r_star
     {510, 1, 32, 2, T2NT},      // 2d1r_box (sum form)
-    {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
+    {244, 4, 64, 1, TKNT},      // 2d2r_box (sum form)
     kMultiStepNone,             // 2d3r_box
     {62, 14, 32, 2, T2},        // 3d1r_star
     {28, 12, 64, 2, T2NT},      // 3d2r_star

@@ -1,6 +1,6 @@
-// win2d_tbk: K timesteps per launch for the 2D star stencils (temporal
-// blocking with a (K-1)R-column ghost zone), so each cell crosses HBM once
-// per K updates.  FP64, FP32 and FP32 compensated, 1, 2 or 4 columns per
+// win2d_tbk: K timesteps per launch for the 2D stars and the radius-2 2D
+// box (temporal blocking with a (K-1)R-column ghost zone), so
+// each cell crosses HBM once per K updates.  FP64, FP32 and FP32 compensated, 1, 2 or 4 columns per
 // thread.  Each k_win2d_tbK.hip instantiates one K (a translation unit per K
 // keeps the build parallel).
 #ifndef UHS_K_WIN2D_TBK_H
@@ -160,6 +160,157 @@ __device__ __forceinline__ void win2d_tbk_star_tile(const T *__restrict__ in, T 
     });
 }
 
+// Every term of S in non-decreasing row offset (the sum form's condition).
+template <class S> constexpr bool win2d_rows_in_order()
+{
+    for (int t = 1; t < S::P; t++)
+        if (S::OFF[t][0] < S::OFF[t - 1][0]) return false;
+    return true;
+}
+
+// K steps per launch, sum form (2D boxes whose terms come row by row in
+// non-decreasing row offset: of the benchmark boxes, the radius-2 box; the
+// radius-1 box lists its centre row last).  The two-step
+// sum form with K levels: every level keeps, per column, one running sum per
+// pending output row (2R+1 of them, in a ring indexed by the step's phase).
+// At step s level-0 row r0 = ys - KR + s arrives through LDS, and so does
+// each intermediate level's row computed at step s-1; each arriving row's
+// cells are applied, term by term in term order, to the pending sums of the
+// level above, and the sum whose last row this was completes: level k row
+// r0 - k(R+1) + 1.  Level K is the output.  The levels of a step read only
+// rows staged before its barrier, so one barrier per row serves all.  Rows
+// and columns a level computes outside the range the level above uses are
+// never read.  Same columns, ghost zone and halo rules as the ring form.
+template <typename T, class S, int VX, bool KAHAN, int K, bool FULL, class CF>
+__device__ __forceinline__ void win2d_tbk_sum_tile(const T *__restrict__ in, T *__restrict__ out,
+                                                   const T *__restrict__ hA, const T *__restrict__ hB, int k0,
+                                                   const KParams<T, S::P> &p, int x0, int ys, int ye, T *lds,
+                                                   CF &cf)
+{
+    constexpr int R = S::R, P = S::P, W = 2 * R + 1;
+    static_assert(K >= 2, "at least two steps");
+    static_assert(S::OFF[0][0] == -R, "the first term lies on the first row of the box");
+    static_assert(win2d_rows_in_order<S>(), "every term in non-decreasing row offset");
+    const int tid = threadIdx.x, BXT = p.BXT;
+    const int BX = p.TX + 2 * (K - 1) * R, LW = BX + 2 * R;
+    T *L0 = lds;
+    auto Lk = [&](int k) { return lds + 2 * LW + (k - 1) * (2 * BX + 2 * R) + R; };
+    lds_poison(lds, 2 * LW + (K - 1) * (2 * BX + 2 * R));
+    const int n0 = p.n0, n2 = p.n2;
+    const long long sy = p.sy;
+    const bool nts = p.flags & UHS_CFG_NT_STORE;
+    const T *h0 = (k0 & 1) ? hB : hA;
+
+    int xcl[VX];
+    bool xin[VX], core[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int ex = tid + v * BXT, xg = x0 - (K - 1) * R + ex;
+        xin[v] = FULL || (xg >= 0 && xg < n2);
+        core[v] = ex >= (K - 1) * R && ex < BX - (K - 1) * R && (FULL || xg < n2);
+        xcl[v] = FULL ? xg : clampi(xg, -R, n2 + R - 1);
+    }
+    const bool hthr = tid < 2 * R;
+    const int hlx = tid < R ? tid : BX + tid;
+    const int hxg = x0 - K * R + hlx;
+    const bool hin = FULL || (hxg >= 0 && hxg < n2);
+    const int hcl = FULL && hthr ? hxg : clampi(hxg, -R, n2 + R - 1);
+    auto lev0 = [&](bool cin, int r, int c) -> T {
+        if constexpr (FULL) {
+            return in[(long long)min(r, n0 + R - 1) * sy + c];
+        } else {
+            const T *base = (cin && r >= 0 && r < n0) ? in : h0;
+            return base[(long long)clampi(r, -R, n0 + R - 1) * sy + c];
+        }
+    };
+    const int rs0 = ys - K * R; // level-0 row of step 0
+    // Prefetch ring: slot f holds the level-0 row staged at the next step of
+    // phase f, loaded 2R+1 rows ahead.
+    T nrow[W][VX], nh[W];
+#pragma unroll
+    for (int f = 0; f < W; f++) {
+#pragma unroll
+        for (int v = 0; v < VX; v++) nrow[f][v] = lev0(xin[v], rs0 + f, xcl[v]);
+        nh[f] = lev0(hin, rs0 + f, hcl);
+    }
+    // a[k-1]: the level-k sums; vk[k-1]: level k's row completed at the
+    // previous step (staged at this one), k = 1 .. K-1.
+    Acc<T, KAHAN> a[K][VX][W];
+    T vk[K - 1][VX];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+#pragma unroll
+            for (int j = 0; j < W; j++) a[k][v][j].zero();
+#pragma unroll
+    for (int k = 0; k < K - 1; k++)
+#pragma unroll
+        for (int v = 0; v < VX; v++) vk[k][v] = T(0);
+    const int N = (ye - ys) + K * (2 * R + 1) - 1; // steps
+
+    auto step = [&](auto phc, int s) {
+        constexpr int PH = decltype(phc)::value;
+        constexpr int fs = (PH + 1) % W; // the slot of the sums that complete
+        const int r0 = rs0 + s;
+        T *S0 = L0 + (s & 1) * LW;
+#pragma unroll
+        for (int v = 0; v < VX; v++) S0[R + tid + v * BXT] = nrow[PH][v];
+        if (hthr) S0[hlx] = nh[PH];
+        static_for<K - 1>([&](auto kc) {
+            constexpr int k = decltype(kc)::value + 1;
+            T *Sk = Lk(k) + (s & 1) * BX;
+#pragma unroll
+            for (int v = 0; v < VX; v++) Sk[tid + v * BXT] = vk[k - 1][v];
+        });
+#pragma unroll
+        for (int v = 0; v < VX; v++) nrow[PH][v] = lev0(xin[v], r0 + W, xcl[v]);
+        nh[PH] = lev0(hin, r0 + W, hcl);
+        __syncthreads();
+        cf.refresh();
+        static_for<K>([&](auto kc) {
+            constexpr int k = K - decltype(kc)::value; // level K first
+            const int yk = r0 - k * (R + 1) + 1;       // the level-k row that completes
+            const T *row;                               // arriving row, cell of column x - R
+            if constexpr (k == 1) row = S0;
+            else row = Lk(k - 1) + (s & 1) * BX - R;
+            const bool yin = FULL || (yk >= 0 && yk < n0);
+            const T *hk = ((k0 + k) & 1) ? hB : hA;
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+                T e[W];
+#pragma unroll
+                for (int b = 0; b < W; b++) e[b] = row[tid + v * BXT + b];
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int dy = S::OFF[t][0], dx = S::OFF[t][1];
+                    constexpr int slot = ((PH - dy - R) % W + W) % W;
+                    if constexpr (t == 0) a[k - 1][v][slot].first(cf.template get<t>(), e[R + dx]);
+                    else a[k - 1][v][slot].add(cf.template get<t>(), e[R + dx]);
+                });
+                // Pin the level's sums: its FMAs finish before the next
+                // level's LDS reads, so one level's cells are live at a time.
+#pragma unroll
+                for (int j = 0; j < W; j++) a[k - 1][v][j].pin();
+                const T o = a[k - 1][v][fs].value();
+                if constexpr (k == K) {
+                    if (yk >= ys && core[v]) store_out(out + (long long)yk * sy + xcl[v], o, nts);
+                } else {
+                    vk[k - 1][v] = (FULL || (yin && xin[v])) ? o
+                                                            : hk[(long long)clampi(yk, -R, n0 + R - 1) * sy + xcl[v]];
+                }
+            }
+        });
+    };
+    int s0 = 0;
+    for (; s0 + W <= N; s0 += W)
+        static_for<W>([&](auto ic) { step(ic, s0 + decltype(ic)::value); });
+    static_for<W - 1>([&](auto ic) {
+        const int s = s0 + decltype(ic)::value;
+        if (s < N) step(ic, s);
+    });
+}
+
 template <typename T, class S, int VX, bool KAHAN, int K>
 __global__ void __launch_bounds__(kMaxThreads) k_win2d_tbk(TB2Params<T, S::P> tp)
 {
@@ -172,8 +323,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d_tbk(TB2Params<T, S::P> tp
     T *lds = reinterpret_cast<T *>(smem_raw);
     constexpr int KR = K * S::R;
     const bool full = x0 >= KR && x0 + p.TX + KR <= p.n2 && ys >= KR && ye + KR <= p.n0;
-    if (full) win2d_tbk_star_tile<T, S, VX, KAHAN, K, true>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
-    else win2d_tbk_star_tile<T, S, VX, KAHAN, K, false>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+    if constexpr (S::STAR) {
+        if (full) win2d_tbk_star_tile<T, S, VX, KAHAN, K, true>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+        else win2d_tbk_star_tile<T, S, VX, KAHAN, K, false>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+    } else {
+        if (full) win2d_tbk_sum_tile<T, S, VX, KAHAN, K, true>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+        else win2d_tbk_sum_tile<T, S, VX, KAHAN, K, false>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+    }
 }
 
 template <typename T, class S, int VX, bool KAHAN, int K>
@@ -193,13 +349,20 @@ hipError_t launch_win2d_tbk(const SweepArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
+// Stars: 1, 2 or 4 columns per thread.  Boxes whose terms come row by row
+// (sum form; the radius-2 box): 1 or 2 (1 in FP64, where two columns would
+// spill).
 template <typename T, bool KAHAN, class S, int K> LaunchFn win2d_tbk_pick_vec(int vec)
 {
-    if constexpr (!S::STAR) return nullptr;
+    if constexpr (!S::STAR && !win2d_rows_in_order<S>()) return nullptr;
     else switch (vec) {
         case 1: return launch_win2d_tbk<T, S, 1, KAHAN, K>;
-        case 2: return launch_win2d_tbk<T, S, 2, KAHAN, K>;
-        case 4: return launch_win2d_tbk<T, S, 4, KAHAN, K>;
+        case 2:
+            if constexpr (!S::STAR && S::R == 2 && sizeof(T) == 8) return nullptr;
+            else return launch_win2d_tbk<T, S, 2, KAHAN, K>;
+        case 4:
+            if constexpr (S::STAR) return launch_win2d_tbk<T, S, 4, KAHAN, K>;
+            else return nullptr;
         }
     return nullptr;
 }
@@ -218,6 +381,7 @@ template <int K> LaunchFn win2d_tbk_find(int prog, int dtype, int vec)
     case Prog_2d1r_star::ID: return win2d_tbk_pick<Prog_2d1r_star, K>(dtype, vec);
     case Prog_2d2r_star::ID: return win2d_tbk_pick<Prog_2d2r_star, K>(dtype, vec);
     case Prog_2d3r_star::ID: return win2d_tbk_pick<Prog_2d3r_star, K>(dtype, vec);
+    case Prog_2d2r_box::ID: return win2d_tbk_pick<Prog_2d2r_box, K>(dtype, vec);
     }
     return nullptr;
 }

@@ -74,13 +74,13 @@ def test_plan_kinds_and_default_configs(uhs, case):
 _T2, _NT, _RR, _T3 = 32, 1, 128, 256
 TWO_STEP_2D = {
     0: {"2d1r_star": (242, 8, 64, 1, _T3 | _NT), "2d2r_star": (236, 6, 64, 1, _T3),
-        "2d3r_star": (238, 4, 64, 1, _T3), "2d1r_box": (510, 1, 32, 2, _T2 | _NT),
+        "2d3r_star": (238, 4, 64, 1, _T3), "2d1r_box": (246, 6, 64, 1, _T3 | _NT),
         "2d2r_box": (244, 4, 64, 1, _T3 | _NT)},
     1: {"2d1r_star": (242, 8, 64, 1, _T3), "2d2r_star": (236, 6, 64, 1, _T3),
-        "2d3r_star": (488, 5, 64, 1, _T3), "2d1r_box": (254, 1, 16, 4, _T2 | _NT | _RR),
+        "2d3r_star": (488, 5, 64, 1, _T3), "2d1r_box": (118, 6, 64, 1, _T3 | _NT),
         "2d2r_box": (252, 1, 64, 1, _T2 | _RR)},
     2: {"2d1r_star": (242, 8, 64, 1, _T3 | _NT), "2d2r_star": (236, 6, 64, 1, _T3),
-        "2d3r_star": (488, 5, 64, 1, _T3 | _NT), "2d1r_box": (510, 1, 32, 2, _T2 | _RR),
+        "2d3r_star": (488, 5, 64, 1, _T3 | _NT), "2d1r_box": (250, 4, 64, 1, _T3 | _NT),
         "2d2r_box": (124, 1, 64, 1, _T2 | _NT | _RR)},
 }
 
@@ -89,9 +89,9 @@ TWO_STEP_2D = {
 def test_one_step_configs(uhs, case):
     """Defaults that advance two steps per launch: the FP64 radius-1 and
     radius-2 3D stars, the plain FP32 radius-1 3D star, the FP64 and plain
-    FP32 radius-1 3D box, and the 2D boxes but the radius-3 box in every
-    precision (but the FP64 radius-2 box); K steps per launch (K = tile_y):
-    the 2D stars in every precision and the FP64 radius-2 box.  Their one-step configuration
+    FP32 radius-1 3D box, and the plain and compensated FP32 radius-2 2D
+    box; K steps per launch (K = tile_y): the 2D stars and the radius-1 2D
+    box in every precision and the FP64 radius-2 box.  Their one-step configuration
     (leading steps, unbatched baseline) is the one-step default tile; every
     other default is its own one-step configuration."""
     s = SHAPES[case]
@@ -190,7 +190,8 @@ def test_two_step_config_checks_host_only(uhs):
           (e3, uhs.builtin("3d1r_box"), F32, (14, 6, 8, 4, T2)),
           (e3, uhs.builtin("3d1r_star"), F64, (62, 14, 32, 2, T2))]
     T3 = uhs.CFG_TEMPORALK
-    ok += [(e2, s1, F64, (252, 3, 32, 2, T3)),                # K = tile_y steps per launch, 2D stars
+    ok += [(e2, b1, F64, (60, 3, 32, 2, T3)),                 # K steps, radius-1 box (window form)
+           (e2, s1, F64, (252, 3, 32, 2, T3)),                # K = tile_y steps per launch, 2D stars
            (e2, s1, F64, (242, 8, 32, 2, T3)),
            (e2, s3, F32, (52, 3, 32, 4, T3 | NT))]
     for ext, st, dt, cfg in ok:
@@ -210,7 +211,7 @@ def test_two_step_config_checks_host_only(uhs):
            (e2, s1, F64, (60, 3, 32, 2, T3 | RR)),       # K steps: ring form only
            (e2, s1, F64, (62, 1, 32, 2, T3)),            # K steps: K = tile_y of 3, 4, 5, 6 or 8
            (e2, s1, F64, (58, 7, 32, 2, T3)),
-           (e2, b1, F64, (60, 3, 32, 2, T3)),            # K steps: stars and the radius-2 box
+           (e2, b3, F64, (52, 3, 32, 1, T3)),            # K steps: not the radius-3 box
            (e3, uhs.builtin("3d1r_star"), F64, (60, 3, 32, 2, T3))]  # K steps: 2D only
     for ext, st, dt, cfg in bad:
         with pytest.raises(uhs.UhsError) as e:

@@ -91,7 +91,7 @@ def test_two_step_tiles_under_poison(poisoned, orc, case):
         check(poisoned, orc, case, [40, 44, 70], 5, cfg)
 
 
-@pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d2r_box"])
+@pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box"])
 @pytest.mark.parametrize("K", [3, 4, 8])
 def test_k_step_tiles_2d_under_poison(poisoned, orc, case, K):
     R = SHAPES[case]["radius"]

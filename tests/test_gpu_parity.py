@@ -370,12 +370,12 @@ def test_two_steps_per_launch_config_checks_2d(gpu):
     assert u.plan_kind(ext, 3, u.builtin("2d3r_box"), u.DT_F32, cfg=(58, 1, 32, 1, T2 | RR)) == "win2d"
 
 
-@pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d2r_box"])
+@pytest.mark.parametrize("case", ["2d1r_star", "2d2r_star", "2d3r_star", "2d1r_box", "2d2r_box"])
 @pytest.mark.parametrize("dtype", ["f64", "f32", "f32k"])
 @pytest.mark.parametrize("K", [3, 4, 5, 6, 8])
 def test_k_steps_per_launch_2d(gpu, orc, case, dtype, K):
-    """UHS_CFG_TEMPORALK for the 2D stars (ring form) and the radius-2 box
-    (sum form), K = tile_y steps per launch: the result grid is bitwise the
+    """UHS_CFG_TEMPORALK for the 2D stars (ring form), the radius-2 box (sum
+    form) and the radius-1 box (window form), K = tile_y steps per launch: the result grid is bitwise the
     oracle's for T from 1 to 3K (leading steps, one and several K-step
     launches), halos untouched and every state's halo read from the grid
     that holds it, with edge tiles in both axes, slabs of 3 to 64 rows, 1, 2
@@ -399,8 +399,8 @@ def test_k_steps_per_launch_2d(gpu, orc, case, dtype, K):
             if tx < 1 or bx // v < 2 * R:  # the ring needs 2r staging threads
                 continue
             cfg = (tx, K, tz, v, fl)
-            if "box" in case and (v == 4 or (v == 2 and dtype == "f64")):
-                with pytest.raises(u.UhsError):  # the box: 1 or 2 columns, 1 in FP64
+            if "box" in case and (v == 4 or (v == 2 and dtype == "f64" and R == 2)):
+                with pytest.raises(u.UhsError):  # boxes: 1 or 2 columns (the FP64 radius-2 box 1)
                     u.plan_kind(ext, h, st, {"f64": u.DT_F64, "f32": u.DT_F32, "f32k": u.DT_F32K}[dtype], cfg=cfg)
                 continue
             for T in Ts:
@@ -428,8 +428,8 @@ def test_k_steps_per_launch_config_checks(gpu):
                        (s1, ext, (62, 3, 32, 2, TK | T2)),              # with TEMPORAL2
                        (s1, ext, (60, 3, 32, 2, TK | RR)),              # no window form
                        (s1, ext, (61, 3, 32, 2, TK)),                   # tile_x + 4r odd with vec 2
-                       (u.builtin("2d1r_box"), ext, (60, 3, 32, 2, TK)),  # its terms are not row by row
-                       (u.builtin("2d2r_box"), ext, (60, 3, 32, 2, TK)),  # FP64 box: 1 column
+                       (u.builtin("2d1r_box"), ext, (60, 3, 32, 4, TK)),  # box: 1 or 2 columns
+                       (u.builtin("2d2r_box"), ext, (60, 3, 32, 2, TK)),  # FP64 radius-2 box: 1 column
                        (u.builtin("2d3r_box"), ext, (52, 3, 32, 1, TK)),
                        (u.builtin("3d1r_star"), [40, 44, 70], (60, 3, 32, 2, TK))):  # 2D only
         with pytest.raises(u.UhsError) as ex:

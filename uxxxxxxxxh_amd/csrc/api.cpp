@@ -161,13 +161,13 @@ int uhs_check_config(const Plan &pl, const uhs_config &c, char *why, size_t whyl
         return bad("two planes of tile loads in flight exist for 3D boxes without deeper cell prefetch");
     if (c.tile_x < 1 || c.tile_z < 1 || c.tile_y < 1) return bad("tile sizes must be positive");
     if (c.flags & UHS_CFG_TEMPORALK) {
-        // K = tile_y steps per launch, 2D stars (ring form) and the radius-2
-        // 2D box, whose terms come row by row (sum form): output columns
-        // tile_x plus (K-1)r on each side, vec columns per thread, one row
-        // per step.
+        // K = tile_y steps per launch, 2D stars (ring form), the radius-2 2D
+        // box, whose terms come row by row (sum form), and the radius-1 box
+        // (window form): output columns tile_x plus (K-1)r on each side, vec
+        // columns per thread, one row per step.
         const long long K = c.tile_y;
-        if (pl.family != FAM_WIN2D || (!kBuiltins[pl.prog].star && R != 2))
-            return bad("K steps per launch exist for 2D stars and the radius-2 2D box only");
+        if (pl.family != FAM_WIN2D || (!kBuiltins[pl.prog].star && R > 2))
+            return bad("K steps per launch exist for 2D stars and the radius-1 and radius-2 2D boxes only");
         if (c.flags & ~(UHS_CFG_TEMPORALK | UHS_CFG_NT_STORE))
             return bad("K steps per launch combine with UHS_CFG_NT_STORE only");
         if (K != 3 && K != 4 && K != 5 && K != 6 && K != 8)
@@ -348,7 +348,9 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 //   (6), 791 (5) against 1025, 891, 739.  The radius-2 box (sum form, its
 //   25 terms row by row) in FP64: 604 at K = 4 against 510 for two steps;
 //   in FP32 (854 against 860) and compensated (453 against 496) it stays at
-//   two steps.
+//   two steps.  The radius-1 box (window form, its centre row last): 1103
+//   at K = 6 against 590 (FP64), 1700 at K = 6 against 1066 (FP32), 1096 at
+//   K = 4 against 997 (compensated).
 //   FP64: 3d1r_star 446 against 337 GSt/s one step (62x14 output tile, 64x16
 //   with the ghost zone, 2 columns per thread, 32-plane slabs); 3d2r_star 327
 //   against 320 (28x12 output tile, 64-plane slabs); 3d1r_box 373-376
@@ -382,7 +384,7 @@ static const uhs_config kBuiltinMultiStepF64[12] = {
     {242, 8, 64, 1, TKNT},      // 2d1r_star
     {236, 6, 64, 1, TK},        // 2d2r_star
     {238, 4, 64, 1, TK},        // 2d3r_star
-    {510, 1, 32, 2, T2NT},      // 2d1r_box (sum form)
+    {246, 6, 64, 1, TKNT},      // 2d1r_box (window form)
     {244, 4, 64, 1, TKNT},      // 2d2r_box (sum form)
     kMultiStepNone,             // 2d3r_box
     {62, 14, 32, 2, T2},        // 3d1r_star
@@ -395,7 +397,7 @@ static const uhs_config kBuiltinMultiStepF32[12] = {
     {242, 8, 64, 1, TK},        // 2d1r_star
     {236, 6, 64, 1, TK},        // 2d2r_star
     {488, 5, 64, 1, TK},        // 2d3r_star
-    {254, 1, 16, 4, T2NT | RR}, // 2d1r_box
+    {118, 6, 64, 1, TKNT},      // 2d1r_box (window form)
     {252, 1, 64, 1, T2 | RR},   // 2d2r_box
     kMultiStepNone,             // 2d3r_box
     {62, 14, 32, 2, T2},        // 3d1r_star
@@ -407,7 +409,7 @@ static const uhs_config kBuiltinMultiStepF32K[12] = {
     {242, 8, 64, 1, TKNT},      // 2d1r_star
     {236, 6, 64, 1, TK},        // 2d2r_star
     {488, 5, 64, 1, TKNT},      // 2d3r_star
-    {510, 1, 32, 2, T2 | RR},   // 2d1r_box
+    {250, 4, 64, 1, TKNT},      // 2d1r_box (window form)
     {124, 1, 64, 1, T2NT | RR}, // 2d2r_box
     kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone, kMultiStepNone,
     kMultiStepNone,

@@ -1,5 +1,5 @@
-// win2d_tbk: K timesteps per launch for the 2D stars and the radius-2 2D
-// box (temporal blocking with a (K-1)R-column ghost zone), so
+// win2d_tbk: K timesteps per launch for the 2D stars and the radius-1 and
+// radius-2 2D boxes (temporal blocking with a (K-1)R-column ghost zone), so
 // each cell crosses HBM once per K updates.  FP64, FP32 and FP32 compensated, 1, 2 or 4 columns per
 // thread.  Each k_win2d_tbK.hip instantiates one K (a translation unit per K
 // keeps the build parallel).
@@ -311,6 +311,138 @@ __device__ __forceinline__ void win2d_tbk_sum_tile(const T *__restrict__ in, T *
     });
 }
 
+// K steps per launch, window form (2D boxes in any term order; compiled for
+// the radius-1 box, whose centre row comes last).  The two-step window form
+// with K levels: every level keeps, per column, the (2R+1)^2 window of the
+// level below in a register ring of 2R+1 rows (slot: the phase of the step
+// that staged the row).  At step s level-0 row r0 = ys - KR + s and each
+// intermediate level's row computed at step s-1 arrive through LDS; level k
+// then computes row r0 - k(R+1) + 1 from its window, in term order.  Same
+// schedule, columns, ghost zone and halo rules as the sum form.
+template <typename T, class S, int VX, bool KAHAN, int K, bool FULL, class CF>
+__device__ __forceinline__ void win2d_tbk_win_tile(const T *__restrict__ in, T *__restrict__ out,
+                                                   const T *__restrict__ hA, const T *__restrict__ hB, int k0,
+                                                   const KParams<T, S::P> &p, int x0, int ys, int ye, T *lds,
+                                                   CF &cf)
+{
+    constexpr int R = S::R, P = S::P, W = 2 * R + 1;
+    static_assert(K >= 2, "at least two steps");
+    const int tid = threadIdx.x, BXT = p.BXT;
+    const int BX = p.TX + 2 * (K - 1) * R, LW = BX + 2 * R;
+    T *L0 = lds;
+    auto Lk = [&](int k) { return lds + 2 * LW + (k - 1) * (2 * BX + 2 * R) + R; };
+    lds_poison(lds, 2 * LW + (K - 1) * (2 * BX + 2 * R));
+    const int n0 = p.n0, n2 = p.n2;
+    const long long sy = p.sy;
+    const bool nts = p.flags & UHS_CFG_NT_STORE;
+    const T *h0 = (k0 & 1) ? hB : hA;
+
+    int xcl[VX];
+    bool xin[VX], core[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        const int ex = tid + v * BXT, xg = x0 - (K - 1) * R + ex;
+        xin[v] = FULL || (xg >= 0 && xg < n2);
+        core[v] = ex >= (K - 1) * R && ex < BX - (K - 1) * R && (FULL || xg < n2);
+        xcl[v] = FULL ? xg : clampi(xg, -R, n2 + R - 1);
+    }
+    const bool hthr = tid < 2 * R;
+    const int hlx = tid < R ? tid : BX + tid;
+    const int hxg = x0 - K * R + hlx;
+    const bool hin = FULL || (hxg >= 0 && hxg < n2);
+    const int hcl = FULL && hthr ? hxg : clampi(hxg, -R, n2 + R - 1);
+    auto lev0 = [&](bool cin, int r, int c) -> T {
+        if constexpr (FULL) {
+            return in[(long long)min(r, n0 + R - 1) * sy + c];
+        } else {
+            const T *base = (cin && r >= 0 && r < n0) ? in : h0;
+            return base[(long long)clampi(r, -R, n0 + R - 1) * sy + c];
+        }
+    };
+    const int rs0 = ys - K * R; // level-0 row of step 0
+    T nrow[W][VX], nh[W];
+#pragma unroll
+    for (int f = 0; f < W; f++) {
+#pragma unroll
+        for (int v = 0; v < VX; v++) nrow[f][v] = lev0(xin[v], rs0 + f, xcl[v]);
+        nh[f] = lev0(hin, rs0 + f, hcl);
+    }
+    // w[k-1][v][slot][b]: the window level k computes from (level k-1 rows,
+    // column x - R + b); vk[k-1]: level k's row computed at the previous
+    // step (staged at this one), k = 1 .. K-1.
+    T w[K][VX][W][W], vk[K - 1][VX];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int v = 0; v < VX; v++)
+#pragma unroll
+            for (int j = 0; j < W; j++)
+#pragma unroll
+                for (int b = 0; b < W; b++) w[k][v][j][b] = T(0);
+#pragma unroll
+    for (int k = 0; k < K - 1; k++)
+#pragma unroll
+        for (int v = 0; v < VX; v++) vk[k][v] = T(0);
+    const int N = (ye - ys) + K * (2 * R + 1) - 1; // steps
+
+    auto step = [&](auto phc, int s) {
+        constexpr int PH = decltype(phc)::value;
+        const int r0 = rs0 + s;
+        T *S0 = L0 + (s & 1) * LW;
+#pragma unroll
+        for (int v = 0; v < VX; v++) S0[R + tid + v * BXT] = nrow[PH][v];
+        if (hthr) S0[hlx] = nh[PH];
+        static_for<K - 1>([&](auto kc) {
+            constexpr int k = decltype(kc)::value + 1;
+            T *Sk = Lk(k) + (s & 1) * BX;
+#pragma unroll
+            for (int v = 0; v < VX; v++) Sk[tid + v * BXT] = vk[k - 1][v];
+        });
+#pragma unroll
+        for (int v = 0; v < VX; v++) nrow[PH][v] = lev0(xin[v], r0 + W, xcl[v]);
+        nh[PH] = lev0(hin, r0 + W, hcl);
+        __syncthreads();
+        cf.refresh();
+        static_for<K>([&](auto kc) {
+            constexpr int k = K - decltype(kc)::value; // level K first
+            const int yk = r0 - k * (R + 1) + 1;       // the level-k row computed
+            const T *row;                               // arriving row, cell of column x - R
+            if constexpr (k == 1) row = S0;
+            else row = Lk(k - 1) + (s & 1) * BX - R;
+            const bool yin = FULL || (yk >= 0 && yk < n0);
+            const T *hk = ((k0 + k) & 1) ? hB : hA;
+#pragma unroll
+            for (int v = 0; v < VX; v++) {
+#pragma unroll
+                for (int b = 0; b < W; b++) w[k - 1][v][PH][b] = row[tid + v * BXT + b];
+                Acc<T, KAHAN> a;
+                static_for<P>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int dy = S::OFF[t][0], dx = S::OFF[t][1];
+                    const T val = w[k - 1][v][(PH + R + 1 + dy) % W][R + dx];
+                    if constexpr (t == 0) a.first(cf.template get<t>(), val);
+                    else a.add(cf.template get<t>(), val);
+                });
+                a.pin();
+                if constexpr (k == K) {
+                    if (yk >= ys && core[v]) store_out(out + (long long)yk * sy + xcl[v], a.value(), nts);
+                } else {
+                    vk[k - 1][v] = (FULL || (yin && xin[v]))
+                                       ? a.value()
+                                       : hk[(long long)clampi(yk, -R, n0 + R - 1) * sy + xcl[v]];
+                }
+            }
+        });
+    };
+    int s0 = 0;
+    for (; s0 + W <= N; s0 += W)
+        static_for<W>([&](auto ic) { step(ic, s0 + decltype(ic)::value); });
+    static_for<W - 1>([&](auto ic) {
+        const int s = s0 + decltype(ic)::value;
+        if (s < N) step(ic, s);
+    });
+}
+
 template <typename T, class S, int VX, bool KAHAN, int K>
 __global__ void __launch_bounds__(kMaxThreads) k_win2d_tbk(TB2Params<T, S::P> tp)
 {
@@ -326,9 +458,12 @@ __global__ void __launch_bounds__(kMaxThreads) k_win2d_tbk(TB2Params<T, S::P> tp
     if constexpr (S::STAR) {
         if (full) win2d_tbk_star_tile<T, S, VX, KAHAN, K, true>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
         else win2d_tbk_star_tile<T, S, VX, KAHAN, K, false>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
-    } else {
+    } else if constexpr (win2d_rows_in_order<S>()) {
         if (full) win2d_tbk_sum_tile<T, S, VX, KAHAN, K, true>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
         else win2d_tbk_sum_tile<T, S, VX, KAHAN, K, false>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+    } else {
+        if (full) win2d_tbk_win_tile<T, S, VX, KAHAN, K, true>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
+        else win2d_tbk_win_tile<T, S, VX, KAHAN, K, false>(p.in, p.out, tp.hA, tp.hB, tp.k0, p, x0, ys, ye, lds, cf);
     }
 }
 
@@ -351,10 +486,10 @@ hipError_t launch_win2d_tbk(const SweepArgs &a, hipStream_t st)
 
 // Stars: 1, 2 or 4 columns per thread.  Boxes whose terms come row by row
 // (sum form; the radius-2 box): 1 or 2 (1 in FP64, where two columns would
-// spill).
+// spill).  Other radius-1 boxes (window form): 1 or 2.
 template <typename T, bool KAHAN, class S, int K> LaunchFn win2d_tbk_pick_vec(int vec)
 {
-    if constexpr (!S::STAR && !win2d_rows_in_order<S>()) return nullptr;
+    if constexpr (!S::STAR && !win2d_rows_in_order<S>() && S::R > 1) return nullptr;
     else switch (vec) {
         case 1: return launch_win2d_tbk<T, S, 1, KAHAN, K>;
         case 2:
@@ -381,6 +516,7 @@ template <int K> LaunchFn win2d_tbk_find(int prog, int dtype, int vec)
     case Prog_2d1r_star::ID: return win2d_tbk_pick<Prog_2d1r_star, K>(dtype, vec);
     case Prog_2d2r_star::ID: return win2d_tbk_pick<Prog_2d2r_star, K>(dtype, vec);
     case Prog_2d3r_star::ID: return win2d_tbk_pick<Prog_2d3r_star, K>(dtype, vec);
+    case Prog_2d1r_box::ID: return win2d_tbk_pick<Prog_2d1r_box, K>(dtype, vec);
     case Prog_2d2r_box::ID: return win2d_tbk_pick<Prog_2d2r_box, K>(dtype, vec);
     }
     return nullptr;

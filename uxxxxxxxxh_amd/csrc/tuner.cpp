@@ -196,7 +196,7 @@ std::vector<uhs_config> uhs_config_space(const Plan &pl)
                         if (tz > 2 * pl.args.n0 && tz > 16) continue;
                         out.push_back(c);
                     }
-        // K = tile_y steps per launch (stars, the radius-2 box):
+        // K = tile_y steps per launch (stars, radius-1 and radius-2 boxes):
         // output columns + (K-1)r each side
         for (int K : {3, 4, 5, 6, 8})
             for (int bx : {64, 128, 256, 512, 1024})

@@ -361,7 +361,8 @@ static const uhs_config kBuiltinDefaultsF32[12] = {
 //   so does the 2D radius-3 box: its two windows do not fit the registers,
 //   and its sum form (170 VGPRs, 3 waves per SIMD) times the same as one
 //   step (289 against 286).
-//   2d1r_box takes the sum form (585 against 570 for the window form).
+//   With two steps per launch 2d1r_box ran faster in the sum form (585 against
+//   570 for the window form); it now takes K steps per launch.
 //   FP32 plain: 3d1r_star 727 against 627; 3d1r_box 564 against 451-483;
 //   2D 1.3-1.8x (2d1r_star 1080
 //   against 595, 2d2r_star 1029 / 619, 2d3r_star 909 / 601, 2d1r_box 1099 /

@@ -1,8 +1,11 @@
-// win2d_tbk: K timesteps per launch for the 2D stars and the radius-1 and
-// radius-2 2D boxes (temporal blocking with a (K-1)R-column ghost zone), so
-// each cell crosses HBM once per K updates.  FP64, FP32 and FP32 compensated, 1, 2 or 4 columns per
-// thread.  Each k_win2d_tbK.hip instantiates one K (a translation unit per K
-// keeps the build parallel).
+// win2d_tbk: K timesteps per launch (UHS_CFG_TEMPORALK, K = tile_y) for the
+// 2D stars and the radius-1 and radius-2 2D boxes (temporal blocking with a
+// (K-1)R-column ghost zone), so each cell crosses HBM once per K updates.
+// FP64, FP32 and FP32 compensated.  Three forms: the ring form (stars; 1, 2
+// or 4 columns per thread), the sum form (boxes whose terms come row by row:
+// the radius-2 box) and the window form (the radius-1 box); the boxes take
+// 1 or 2 columns per thread.  Each k_win2d_tbK.hip instantiates one K (a
+// translation unit per K keeps the build parallel).
 #ifndef UHS_K_WIN2D_TBK_H
 #define UHS_K_WIN2D_TBK_H
 #include "sweep_kernels.h"

@@ -115,7 +115,8 @@ typedef struct uhs_stencil {
  *           UHS_CFG_TEMPORALK: 2D stars and the radius-1 and radius-2 2D
  *           boxes advance K = tile_y timesteps per launch, K = 3, 4, 5, 6 or 8
  *           (not with UHS_CFG_TEMPORAL2): (tile_x + 2(K-1)r) / vec threads,
- *           vec 1, 2 or 4 (boxes 1 or 2, the FP64 radius-2 box 1);
+ *           vec 1 or 2 (4 also for the radius-1 star; the FP64 radius-2
+ *           box 1);
  *           the leading T mod K steps (K more when K is even and T / K odd,
  *           so that even-K launches are even in number) run as 5- and 3-step
  *           launches of the same tile and single sweeps, then the K-step

@@ -193,7 +193,7 @@ def test_two_step_config_checks_host_only(uhs):
     ok += [(e2, b1, F64, (60, 3, 32, 2, T3)),                 # K steps, radius-1 box (window form)
            (e2, s1, F64, (252, 3, 32, 2, T3)),                # K = tile_y steps per launch, 2D stars
            (e2, s1, F64, (242, 8, 32, 2, T3)),
-           (e2, s3, F32, (52, 3, 32, 4, T3 | NT))]
+           (e2, s3, F32, (52, 3, 32, 2, T3 | NT))]
     for ext, st, dt, cfg in ok:
         assert uhs.plan_kind(ext, st.radius, st, dt, cfg=cfg) in ("win2d", "box3d", "star3d"), cfg
     bad = [(e2, s1, F64, (61, 1, 32, 2, T2)),            # tile_x + 2r odd with vec 2
@@ -211,6 +211,7 @@ def test_two_step_config_checks_host_only(uhs):
            (e2, s1, F64, (60, 3, 32, 2, T3 | RR)),       # K steps: ring form only
            (e2, s1, F64, (62, 1, 32, 2, T3)),            # K steps: K = tile_y of 3, 4, 5, 6 or 8
            (e2, s1, F64, (58, 7, 32, 2, T3)),
+           (e2, s3, F64, (52, 3, 32, 4, T3)),            # K steps: 4 columns for the radius-1 star only
            (e2, b3, F64, (52, 3, 32, 1, T3)),            # K steps: not the radius-3 box
            (e3, uhs.builtin("3d1r_star"), F64, (60, 3, 32, 2, T3))]  # K steps: 2D only
     for ext, st, dt, cfg in bad:

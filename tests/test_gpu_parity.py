@@ -399,8 +399,10 @@ def test_k_steps_per_launch_2d(gpu, orc, case, dtype, K):
             if tx < 1 or bx // v < 2 * R:  # the ring needs 2r staging threads
                 continue
             cfg = (tx, K, tz, v, fl)
-            if "box" in case and (v == 4 or (v == 2 and dtype == "f64" and R == 2)):
-                with pytest.raises(u.UhsError):  # boxes: 1 or 2 columns (the FP64 radius-2 box 1)
+            if v == 4 and case != "2d1r_star" or ("box" in case and v == 2 and dtype == "f64" and R == 2):
+                # four columns for the radius-1 star only; boxes 1 or 2 (the
+                # FP64 radius-2 box 1)
+                with pytest.raises(u.UhsError):
                     u.plan_kind(ext, h, st, {"f64": u.DT_F64, "f32": u.DT_F32, "f32k": u.DT_F32K}[dtype], cfg=cfg)
                 continue
             for T in Ts:
@@ -421,7 +423,9 @@ def test_k_steps_per_launch_config_checks(gpu):
     assert u.plan_kind(ext, 1, s1, cfg=(252, 3, 32, 2, TK)) == "win2d"
     assert u.plan_kind(ext, 1, s1, cfg=(242, 8, 32, 2, TK)) == "win2d"
     assert u.plan_kind(ext, 3, s3, cfg=(52, 3, 32, 1, TK | NT)) == "win2d"
+    assert u.plan_kind(ext, 1, s1, cfg=(236, 3, 32, 4, TK)) == "win2d"
     for st, e, bad in ((s1, ext, (60, 3, 32, 8, TK)),                   # vec 8
+                       (s3, ext, (52, 3, 32, 4, TK)),                   # 4 columns: radius-1 star only
                        (s1, ext, (1020, 3, 32, 1, TK)),                 # 1024 threads
                        (s1, ext, (62, 1, 32, 2, TK)),                   # K = 1
                        (s1, ext, (58, 7, 32, 2, TK)),                   # K = 7 not compiled

@@ -1,10 +1,10 @@
 // win2d_tbk: K timesteps per launch (UHS_CFG_TEMPORALK, K = tile_y) for the
 // 2D stars and the radius-1 and radius-2 2D boxes (temporal blocking with a
 // (K-1)R-column ghost zone), so each cell crosses HBM once per K updates.
-// FP64, FP32 and FP32 compensated.  Three forms: the ring form (stars; 1, 2
-// or 4 columns per thread), the sum form (boxes whose terms come row by row:
-// the radius-2 box) and the window form (the radius-1 box); the boxes take
-// 1 or 2 columns per thread.  Each k_win2d_tbK.hip instantiates one K (a
+// FP64, FP32 and FP32 compensated.  Three forms: the ring form (stars; 1 or
+// 2 columns per thread, and 4 for the radius-1 star), the sum form (boxes
+// whose terms come row by row: the radius-2 box) and the window form (the
+// radius-1 box); the boxes take 1 or 2 columns per thread.  Each k_win2d_tbK.hip instantiates one K (a
 // translation unit per K keeps the build parallel).
 #ifndef UHS_K_WIN2D_TBK_H
 #define UHS_K_WIN2D_TBK_H
@@ -487,7 +487,7 @@ hipError_t launch_win2d_tbk(const SweepArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-// Stars: 1, 2 or 4 columns per thread.  Boxes whose terms come row by row
+// Stars: 1 or 2 columns per thread, and 4 for the radius-1 star.  Boxes whose terms come row by row
 // (sum form; the radius-2 box): 1 or 2 (1 in FP64, where two columns would
 // spill).  Other radius-1 boxes (window form): 1 or 2.
 template <typename T, bool KAHAN, class S, int K> LaunchFn win2d_tbk_pick_vec(int vec)
@@ -498,8 +498,8 @@ template <typename T, bool KAHAN, class S, int K> LaunchFn win2d_tbk_pick_vec(in
         case 2:
             if constexpr (!S::STAR && S::R == 2 && sizeof(T) == 8) return nullptr;
             else return launch_win2d_tbk<T, S, 2, KAHAN, K>;
-        case 4:
-            if constexpr (S::STAR) return launch_win2d_tbk<T, S, 4, KAHAN, K>;
+        case 4: // the radius-1 star only: wider stars spill at four columns
+            if constexpr (S::STAR && S::R == 1) return launch_win2d_tbk<T, S, 4, KAHAN, K>;
             else return nullptr;
         }
     return nullptr;

@@ -393,17 +393,22 @@ def costmodel_leg(u, problems_per_shape=9, configs_per_problem=150, epochs=300, 
     idx = np.random.default_rng(seed).permutation(len(X))
     n_val = max(1, len(X) // 10)
     val, tr = idx[:n_val], idx[n_val:]
-    m = u.MLP(seed)
+    # Three models from three seeds; configurations are ranked by the mean of
+    # their predicted log times.
+    models = [u.MLP(seed + i) for i in range(3)]
     t1 = time.time()
-    loss = m.train(X[tr], y[tr], epochs=epochs, batch=256, lr=1e-3, fit_norm=True, seed=seed)
+    for i, m in enumerate(models):
+        m.train(X[tr], y[tr], epochs=epochs, batch=256, lr=1e-3, fit_norm=True, seed=seed + i)
     t_train = time.time() - t1
-    pv = m.predict(X[val])
+    pv = np.mean([m.predict(X[val]) for m in models], axis=0)
     rk = lambda v: np.argsort(np.argsort(v))
     out = {"timed_configs": n, "rows": int(len(X)), "collect_s": round(t_collect, 1), "train_s": round(t_train, 1),
+           "ensemble": len(models),
            "val_rmse_log_ms": round(float(np.sqrt(np.mean((pv - y[val]) ** 2))), 4),
            "val_rank_corr": round(float(np.corrcoef(rk(pv), rk(y[val]))[0, 1]), 4),
-           "held_out": cm.evaluate(m, topk, seed)}
-    m.close()
+           "held_out": cm.evaluate(models, topk, seed)}
+    for m in models:
+        m.close()
     return out
 
 

@@ -136,6 +136,13 @@ int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *ext,
                          uhs_mlp_t model, int topk, int reps,
                          const char *db_path, uhs_tune_result *out,
                          uhs_stream_t stream);
+/* The same search ranked by an ensemble of nmodels models: the mean of
+ * their predicted log times.  uhs_tune_sweep_model is the one-model case. */
+int uhs_tune_sweep_models(void *A, void *B, int dims, const int64_t *ext,
+                          int halo, const uhs_stencil *st, int dtype,
+                          const uhs_mlp_t *models, int nmodels, int topk,
+                          int reps, const char *db_path,
+                          uhs_tune_result *out, uhs_stream_t stream);
 
 /* ---- batch configurations: candidate space, timed search, model --------- */
 

@@ -2,7 +2,11 @@
 the GPU by the library's kernels on timings the C++ autotuner collects
 (random problems of all 12 stencils, FP64 and both FP32 forms), judged on the
 two held-out problems -- 3d3r_box FP64 384x512x640 and 2d3r_star FP64
-12288x6144 -- never seen in training.  The model's top-5, timed, must come
+12288x6144 -- never seen in training.  Three models trained from three
+seeds rank the space by the mean of their predicted log times (one model's
+ranking among the 2D K-step depths, whose best tiles lie within 5-10 % of
+each other, swings with the seed and the timing noise of the collection).
+The ensemble's top-5, timed, must come
 within 10 % of the exhaustive timed search's best with at least 20x less
 tuning time; its ranking of the whole space must follow the measured one.
 
@@ -29,12 +33,14 @@ def test_costmodel_held_out_pair(gpu, tmp_path):
     X, y = cm.load_rows(db)
     log("cost model: %d timed configurations, %d rows" % (n, len(X)))
     assert len(X) > 5000 and set(np.unique(X[:, 17]).tolist()) == {0.0, 1.0, 2.0}
-    m = u.MLP(11)
+    models = [u.MLP(s) for s in (11, 12, 13)]
     try:
-        m.train(X, y, epochs=300, batch=256, lr=1e-3, fit_norm=True, seed=11)
-        ev = cm.evaluate(m, topk=5, seed=11)
+        for s, m in zip((11, 12, 13), models):
+            m.train(X, y, epochs=300, batch=256, lr=1e-3, fit_norm=True, seed=s)
+        ev = cm.evaluate(models, topk=5, seed=11)
     finally:
-        m.close()
+        for m in models:
+            m.close()
         u.check(u.lib().uhs_tune_clear())
     assert set(ev) == {"3d3r_box_384x512x640", "2d3r_star_12288x6144"}
     for name, r in ev.items():

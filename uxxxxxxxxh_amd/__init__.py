@@ -144,6 +144,8 @@ _SIGS = {
                             ctypes.POINTER(TuneResult), _P]),
     "uhs_tune_sweep_model": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, _P, _I, _I, ctypes.c_char_p,
                                   ctypes.POINTER(TuneResult), _P]),
+    "uhs_tune_sweep_models": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(_P), _I, _I, _I,
+                                   ctypes.c_char_p, ctypes.POINTER(TuneResult), _P]),
     "uhs_tune_sweep_list": (_I, [_P, _P, _I, _I64P, _I, ctypes.POINTER(Stencil), _I, ctypes.POINTER(Config), _I, _I,
                                  ctypes.c_char_p, ctypes.POINTER(TuneResult), _P]),
     "uhs_batch_space": (_I, [ctypes.POINTER(BatchConfig), _I]),
@@ -672,6 +674,12 @@ def tune_sweep(A, B, ext, halo, stencil, dtype=DT_F64, reps=3, db=None, model=No
     if model is None:
         check(lib().uhs_tune_sweep(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c), dtype,
                                    reps, dbp, ctypes.byref(r), stream))
+    elif isinstance(model, (list, tuple)):
+        # An ensemble: configurations ranked by the mean of the members'
+        # predicted log times.
+        hs = (_P * len(model))(*[m.h for m in model])
+        check(lib().uhs_tune_sweep_models(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
+                                          dtype, hs, len(model), topk, reps, dbp, ctypes.byref(r), stream))
     else:
         check(lib().uhs_tune_sweep_model(A.ptr, B.ptr, len(ext), ext_array(ext), halo, ctypes.byref(stencil.c),
                                          dtype, model.h, topk, reps, dbp, ctypes.byref(r), stream))

@@ -134,12 +134,14 @@ def retime(A, B, ext, h, st, cfgs, rounds=7, reps=5):
 
 
 def evaluate(model, topk, seed, draws=200):
-    """Per held-out problem: the exhaustive timed search (the judge), the
-    model's top-k timed, the hand-set default and one random k-sample timed;
+    """Per held-out problem: the exhaustive timed search (the yardstick), the
+    model's top-k timed (`model` is one model or a list ranked by the mean
+    of their predicted log times), the hand-set default and one random k-sample timed;
     read from the exhaustive search's own table (the same timings for every
     method), the model's top-k and the median of `draws` random k-samples;
     and the four picks re-timed under one protocol."""
     import tempfile
+    models = list(model) if isinstance(model, (list, tuple)) else [model]
     rng = np.random.default_rng(seed)
     out = {}
     for case, ext in HELD_OUT:
@@ -150,8 +152,8 @@ def evaluate(model, topk, seed, draws=200):
             ex = tune_sweep(A, B, ext, h, st, reps=3, db=db)
             table = _table(db, space)
         check(lib().uhs_tune_clear())
-        pred = model.predict(np.array([mlp_features(len(ext), st.radius, case.endswith("star"), ext, c)
-                                       for c in space]))
+        feats = np.array([mlp_features(len(ext), st.radius, case.endswith("star"), ext, c) for c in space])
+        pred = np.mean([m.predict(feats) for m in models], axis=0)
         top = np.argsort(pred)[:topk]
         valid = np.flatnonzero(np.isfinite(table))
         rand = [table[rng.choice(valid, topk, replace=False)].min() for _ in range(draws)]
@@ -315,6 +317,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--topk", type=int, default=5)
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--ensemble", type=int, default=3, help="models trained from seeds seed, seed+1, ...")
     ap.add_argument("--skip-collect", action="store_true")
     a = ap.parse_args(argv)
     lib()
@@ -327,25 +330,29 @@ def main(argv=None):
     idx = np.random.default_rng(a.seed).permutation(len(X))
     n_val = max(1, len(X) // 10)
     val, tr = idx[:n_val], idx[n_val:]
-    m = MLP(a.seed)
+    models = [MLP(a.seed + i) for i in range(max(1, a.ensemble))]
     t1 = time.time()
-    loss = m.train(X[tr], y[tr], epochs=a.epochs, batch=256, lr=1e-3, fit_norm=True, seed=a.seed)
+    losses = [m.train(X[tr], y[tr], epochs=a.epochs, batch=256, lr=1e-3, fit_norm=True, seed=a.seed + i)
+              for i, m in enumerate(models)]
+    loss = float(np.mean(losses))
     t_train = time.time() - t1
-    pv = m.predict(X[val])
+    pv = np.mean([m.predict(X[val]) for m in models], axis=0)
     val_rmse = float(np.sqrt(np.mean((pv - y[val]) ** 2)))
     # Spearman rank correlation on the validation rows
     rk = lambda v: np.argsort(np.argsort(v))
     rho = float(np.corrcoef(rk(pv), rk(y[val]))[0, 1])
     log("train loss %.4f, validation RMSE(log ms) %.3f, rank corr %.3f, train %.1f s" % (loss, val_rmse, rho, t_train))
-    m.save(a.model)
-    ev = evaluate(m, a.topk, a.seed)
+    for i, m in enumerate(models):
+        m.save(a.model if i == 0 else "%s.%d" % (a.model, i))
+    ev = evaluate(models, a.topk, a.seed)
     rep = {"rows": int(len(X)), "train_rows": int(len(tr)), "val_rows": int(n_val), "collect_s": t_collect,
-           "train_s": t_train, "epochs": a.epochs, "final_train_loss": loss, "val_rmse_log_ms": val_rmse,
+           "train_s": t_train, "ensemble": len(models), "epochs": a.epochs, "final_train_loss": loss, "val_rmse_log_ms": val_rmse,
            "val_rank_corr": rho, "held_out": ev}
     with open(a.out, "w") as fh:
         json.dump(rep, fh, indent=1)
     print(json.dumps(rep))
-    m.close()
+    for m in models:
+        m.close()
 
 
 if __name__ == "__main__":

@@ -525,11 +525,14 @@ int uhs_search(Plan &pl, void *A, void *B, int dims, const int64_t *ext, int hal
                const std::vector<uhs_config> &cands, int reps, const char *db_path, hipStream_t st,
                uhs_tune_result *res);
 
-extern "C" int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *ext, int halo,
-                                    const uhs_stencil *st, int dtype, uhs_mlp_t model, int topk, int reps,
-                                    const char *db_path, uhs_tune_result *out, uhs_stream_t stream)
+extern "C" int uhs_tune_sweep_models(void *A, void *B, int dims, const int64_t *ext, int halo,
+                                     const uhs_stencil *st, int dtype, const uhs_mlp_t *models, int nmodels,
+                                     int topk, int reps, const char *db_path, uhs_tune_result *out,
+                                     uhs_stream_t stream)
 {
-    if (!model || topk < 1) return uhs_set_error(UHS_E_INVAL, "model search needs a model and topk >= 1");
+    if (!models || nmodels < 1 || topk < 1) return uhs_set_error(UHS_E_INVAL, "model search needs models and topk >= 1");
+    for (int m = 0; m < nmodels; m++)
+        if (!models[m]) return uhs_set_error(UHS_E_INVAL, "null model in the ensemble");
     const auto t0 = std::chrono::steady_clock::now();
     Plan pl;
     int rc = uhs_make_plan(dims, ext, halo, st, dtype, nullptr, &pl);
@@ -540,7 +543,14 @@ extern "C" int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *e
     const BuiltinStencil &b = kBuiltins[pl.prog];
     for (size_t i = 0; i < sp.size(); i++)
         if ((rc = uhs_mlp_features(dims, b.radius, b.star, dtype, ext, &sp[i], 1, &feat[i * IN]))) return rc;
-    if ((rc = uhs_mlp_predict(model, feat.data(), (int)sp.size(), pred.data()))) return rc;
+    // The ensemble's prediction is the mean of its members' predicted log
+    // times: members trained from different seeds disagree most where the
+    // data leave the ranking open, and the mean damps those disagreements.
+    std::vector<float> one(sp.size());
+    for (int m = 0; m < nmodels; m++) {
+        if ((rc = uhs_mlp_predict(models[m], feat.data(), (int)sp.size(), one.data()))) return rc;
+        for (size_t i = 0; i < sp.size(); i++) pred[i] = (m ? pred[i] : 0.f) + one[i] / (float)nmodels;
+    }
     std::vector<size_t> ord(sp.size());
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t c) { return pred[a] < pred[c]; });
@@ -552,4 +562,11 @@ extern "C" int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *e
     r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (out) *out = r;
     return UHS_OK;
+}
+
+extern "C" int uhs_tune_sweep_model(void *A, void *B, int dims, const int64_t *ext, int halo,
+                                    const uhs_stencil *st, int dtype, uhs_mlp_t model, int topk, int reps,
+                                    const char *db_path, uhs_tune_result *out, uhs_stream_t stream)
+{
+    return uhs_tune_sweep_models(A, B, dims, ext, halo, st, dtype, &model, 1, topk, reps, db_path, out, stream);
 }

@@ -51,7 +51,7 @@ int uhs_tune_time(void *A, void *B, int dims, const int64_t *ext, int halo,
                   const uhs_stencil *st, int dtype, const uhs_config *cfg,
                   int reps, double *ms, uhs_stream_t stream);
 /* Mock timing backend.  With fn set, the searches (uhs_tune_sweep,
- * uhs_tune_sweep_list, uhs_tune_sweep_model) call fn for every candidate
+ * uhs_tune_sweep_list, uhs_tune_sweep_model[s]) call fn for every candidate
  * that passes the configuration check, instead of timing sweeps on the GPU:
  * fn returns a UHS_* status (UHS_E_HIP acts as a launch failure: the search
  * records it and stops) and writes the milliseconds per sweep.  The search

@@ -163,6 +163,23 @@ def test_probe_argument_checks(uhs):
     assert L.uhs_copy_f64(None, None, 10, None) == -1
 
 
+def test_model_search_argument_checks(uhs):
+    # The ensemble search rejects a missing model list, an empty ensemble, a
+    # null member and topk < 1 before planning or any device call.
+    import ctypes
+    L = uhs.lib()
+    st = uhs.builtin("2d3r_star")
+    ext = (ctypes.c_int64 * 2)(6144, 12288)
+    r = uhs.TuneResult()
+    none = (ctypes.c_void_p * 2)(None, None)
+    for models, n, topk in ((None, 1, 5), (none, 0, 5), (none, 2, 5), (none, 1, 0)):
+        rc = L.uhs_tune_sweep_models(None, None, 2, ext, 3, ctypes.byref(st.c), uhs.DT_F64, models, n, topk, 1,
+                                     None, ctypes.byref(r), None)
+        assert rc == -1, (n, topk)
+    assert L.uhs_tune_sweep_model(None, None, 2, ext, 3, ctypes.byref(st.c), uhs.DT_F64, None, 5, 1, None,
+                                  ctypes.byref(r), None) == -1
+
+
 def test_no_gpu_needed_for_planning(uhs):
     # Planning, defaults and coefficient generation are host-only.
     st = uhs.builtin("2d3r_box")
